@@ -1,0 +1,206 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg;
+never by the stir_amd package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional, Tuple
+
+import numpy as np
+
+from stir_amd.geometry import (GeometryStruct, ProjDataInfoCylindrical, RayTracingMatrixParameters,
+                               VoxelsOnCartesianGrid, make_geometry_struct)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborpmrt.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i16p = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp = ctypes.c_void_p
+        L.or_plan_create.restype = vp
+        L.or_plan_create.argtypes = [ctypes.POINTER(GeometryStruct), ctypes.c_char_p, ctypes.c_int]
+        L.or_plan_destroy.argtypes = [vp]
+        L.or_symmetry_flags.argtypes = [vp, _i32p]
+        L.or_num_bins.restype = ctypes.c_int64
+        L.or_num_bins.argtypes = [vp]
+        L.or_bin_index.restype = ctypes.c_int64
+        L.or_bin_index.argtypes = [vp] + [ctypes.c_int] * 4
+        for fn in (L.or_get_row, L.or_trace_row):
+            fn.restype = ctypes.c_int
+            fn.argtypes = [vp] + [ctypes.c_int] * 4 + [_i16p, _i16p, _i16p, _f32p, ctypes.c_int]
+        L.or_bin_geometry.argtypes = [vp] + [ctypes.c_int] * 4 + [_f32p]
+        L.or_subset_view_segments.restype = ctypes.c_int
+        L.or_subset_view_segments.argtypes = [vp, ctypes.c_int, ctypes.c_int, _i32p, _i32p, ctypes.c_int]
+        L.or_subsets_balanced.restype = ctypes.c_int
+        L.or_subsets_balanced.argtypes = [vp, ctypes.c_int]
+        for fn in (L.or_forward_project, L.or_forward_project_omp, L.or_back_project, L.or_back_project_omp):
+            fn.argtypes = [vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int]
+        for fn in (L.or_forward_project_f64, L.or_back_project_f64):
+            fn.argtypes = [vp, _f64p, _f64p, ctypes.c_int, ctypes.c_int]
+        L.or_nnz.restype = ctypes.c_int64
+        L.or_nnz.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.or_subset_sensitivity.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
+        L.or_subset_gradient.argtypes = [vp, _f32p, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _f32p]
+        L.or_log_likelihood.restype = ctypes.c_double
+        L.or_log_likelihood.argtypes = [vp, _f32p, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_osem.argtypes = [vp, _f32p, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _opt_ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.ctypes.data_as(ctypes.c_void_p), a
+
+
+class OraclePlan:
+    """The oracle for one geometry. Arrays use STIR layouts: images
+    [z][y][x] (shape (nz,ny,nx)), projection data flat in ProjDataInMemory order."""
+
+    def __init__(self, pdi: ProjDataInfoCylindrical, img: VoxelsOnCartesianGrid,
+                 pm: RayTracingMatrixParameters = RayTracingMatrixParameters()):
+        self.pdi, self.img, self.pm = pdi, img, pm
+        self._g = make_geometry_struct(pdi, img, pm)
+        err = ctypes.create_string_buffer(256)
+        self._p = lib().or_plan_create(ctypes.byref(self._g), err, 256)
+        if not self._p:
+            raise ValueError("oracle: " + err.value.decode())
+        self.num_bins = int(lib().or_num_bins(self._p))
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.or_plan_destroy(self._p)
+            self._p = None
+
+    # ------------------------------------------------------------------
+    def symmetry_flags(self) -> Tuple[int, ...]:
+        out = np.zeros(5, np.int32)
+        lib().or_symmetry_flags(self._p, out)
+        return tuple(int(v) for v in out)
+
+    def bin_index(self, seg, view, ax, tang) -> int:
+        return int(lib().or_bin_index(self._p, seg, view, ax, tang))
+
+    def _row(self, fn, seg, view, ax, tang):
+        cap = 4096
+        while True:
+            z = np.empty(cap, np.int16); y = np.empty(cap, np.int16); x = np.empty(cap, np.int16)
+            w = np.empty(cap, np.float32)
+            n = fn(self._p, seg, view, ax, tang, z, y, x, w, cap)
+            if n >= 0:
+                return z[:n].copy(), y[:n].copy(), x[:n].copy(), w[:n].copy()
+            cap = -n
+
+    def row(self, seg, view, ax, tang):
+        """Row of a bin via basic bin + symmetry operation: (z, y, x, w)."""
+        return self._row(lib().or_get_row, seg, view, ax, tang)
+
+    def trace_row(self, seg, view, ax, tang):
+        """Row traced directly for this bin (no symmetry)."""
+        return self._row(lib().or_trace_row, seg, view, ax, tang)
+
+    def bin_geometry(self, seg, view, ax, tang):
+        out = np.zeros(5, np.float32)
+        lib().or_bin_geometry(self._p, seg, view, ax, tang, out)
+        return out
+
+    def subset_view_segments(self, subset: int, num_subsets: int):
+        cap = 4 * self.pdi.num_views * len(self.pdi.num_axial_poss) + 16
+        v = np.zeros(cap, np.int32); s = np.zeros(cap, np.int32)
+        n = lib().or_subset_view_segments(self._p, subset, num_subsets, v, s, cap)
+        return list(zip(s[:n].tolist(), v[:n].tolist()))
+
+    def subsets_balanced(self, num_subsets: int) -> bool:
+        return bool(lib().or_subsets_balanced(self._p, num_subsets))
+
+    # ------------------------------------------------------------------
+    def _img(self, a, dtype=np.float32):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        assert a.size == self.img.num_voxels
+        return a
+
+    def _sino(self, a, dtype=np.float32):
+        a = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+        assert a.size == self.num_bins
+        return a
+
+    def forward_project(self, image, subset=-1, num_subsets=1, omp=True, out=None):
+        image = self._img(image)
+        if out is None:
+            out = np.zeros(self.num_bins, np.float32)
+        fn = lib().or_forward_project_omp if omp else lib().or_forward_project
+        fn(self._p, image.reshape(-1), out, subset, num_subsets)
+        return out
+
+    def forward_project_f64(self, image, subset=-1, num_subsets=1):
+        image = self._img(image, np.float64)
+        out = np.zeros(self.num_bins, np.float64)
+        lib().or_forward_project_f64(self._p, image.reshape(-1), out, subset, num_subsets)
+        return out
+
+    def back_project(self, sino, subset=-1, num_subsets=1, omp=False):
+        sino = self._sino(sino)
+        out = np.zeros(self.img.shape, np.float32)
+        fn = lib().or_back_project_omp if omp else lib().or_back_project
+        fn(self._p, sino, out.reshape(-1), subset, num_subsets)
+        return out
+
+    def back_project_f64(self, sino, subset=-1, num_subsets=1):
+        sino = self._sino(sino, np.float64)
+        out = np.zeros(self.img.shape, np.float64)
+        lib().or_back_project_f64(self._p, sino, out.reshape(-1), subset, num_subsets)
+        return out
+
+    def nnz(self, subset=-1, num_subsets=1) -> int:
+        return int(lib().or_nnz(self._p, subset, num_subsets))
+
+    def subset_sensitivity(self, subset, num_subsets, zero_end_planes=False):
+        out = np.zeros(self.img.shape, np.float32)
+        lib().or_subset_sensitivity(self._p, subset, num_subsets, int(zero_end_planes), out.reshape(-1))
+        return out
+
+    def subset_gradient(self, y, x, subset, num_subsets, bkg=None, zero_end_planes=False, add_sensitivity=False):
+        y = self._sino(y); x = self._img(x)
+        bp = _opt_ptr(None if bkg is None else self._sino(bkg))
+        out = np.zeros(self.img.shape, np.float32)
+        lib().or_subset_gradient(self._p, y, bp[0] if bp else None, x.reshape(-1), subset, num_subsets,
+                                 int(zero_end_planes), int(add_sensitivity), out.reshape(-1))
+        return out
+
+    def log_likelihood(self, y, x, bkg=None, subset=-1, num_subsets=1, zero_end_planes=False) -> float:
+        y = self._sino(y); x = self._img(x)
+        bp = _opt_ptr(None if bkg is None else self._sino(bkg))
+        return float(lib().or_log_likelihood(self._p, y, bp[0] if bp else None, x.reshape(-1), subset,
+                                             num_subsets, int(zero_end_planes)))
+
+    def osem(self, y, x0, num_subsets, num_subiterations, bkg=None, start_subset=0, zero_end_planes=False,
+             omp=True):
+        y = self._sino(y)
+        x = np.array(self._img(x0), dtype=np.float32, copy=True)
+        bp = _opt_ptr(None if bkg is None else self._sino(bkg))
+        lib().or_osem(self._p, y, bp[0] if bp else None, x.reshape(-1), num_subsets, num_subiterations,
+                      start_subset, int(zero_end_planes), int(omp))
+        return x

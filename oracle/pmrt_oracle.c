@@ -1,0 +1,1186 @@
+/*
+ * CPU oracle (test infrastructure only) -- restatement of STIR's
+ * ray-tracing projection matrix, its symmetries and the OSEM update.
+ * See pmrt_oracle.h for scope and the list of pins.
+ *
+ * Build: gcc -O2 -std=c11 -ffp-contract=off -fopenmp -fPIC -shared
+ * (no -ffast-math: the float evaluation order below mirrors the reference
+ * and must not be re-associated or contracted).
+ */
+#include "pmrt_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_PI 3.14159265358979323846264338327950288
+
+/* ------------------------------------------------------------------ */
+/* small helpers with the reference's exact semantics                  */
+/* ------------------------------------------------------------------ */
+
+/* stir::round(float) -- round.inl:59-66 */
+static int round_f(float x) { return x >= 0 ? (int)(x + 0.5F) : -(int)(-x + 0.5F); }
+/* std::min / std::max on floats (first argument wins ties) */
+static float minf_(float a, float b) { return (b < a) ? b : a; }
+static float maxf_(float a, float b) { return (a < b) ? b : a; }
+static int mini_(int a, int b) { return (b < a) ? b : a; }
+/* stir::modulo(float,float) -- modulo.h:52-79 */
+static float modulo_f(float a, float b)
+{
+  const double r = fmod((double)a, (double)b);
+  float res = (float)(r < 0 ? r + fabs((double)b) : r);
+  const float abs_b = b >= 0 ? b : -b;
+  if (res >= abs_b) res -= abs_b;
+  return res;
+}
+/* norm() of a float 3-vector: float inner product then double sqrt
+ * (BasicCoordinate.inl:408-429; order z,y,x = [1],[2],[3]) */
+static double norm3(float z, float y, float x)
+{
+  float acc = 0.F;
+  acc = acc + z * z;
+  acc = acc + y * y;
+  acc = acc + x * x;
+  return sqrt((double)acc);
+}
+/* RayTraceVoxelsOnCartesianGrid.cxx:45-49 */
+static int is_half_integer(float a) { return fabsf(floorf(a) + .5F - a) < .0001F; }
+
+/* ------------------------------------------------------------------ */
+/* row container (ProjMatrixElemsForOneBin)                           */
+/* ------------------------------------------------------------------ */
+typedef struct { int16_t z, y, x; float w; } elem;
+typedef struct { elem* e; int n, cap; } row;
+
+static void row_reserve(row* r, int cap)
+{
+  if (cap <= r->cap) return;
+  int nc = r->cap ? r->cap : 16;
+  while (nc < cap) nc *= 2;
+  r->e = (elem*)realloc(r->e, (size_t)nc * sizeof(elem));
+  r->cap = nc;
+}
+static void row_push(row* r, int z, int y, int x, float w)
+{
+  if (r->n == r->cap) row_reserve(r, r->n + 1);
+  r->e[r->n].z = (int16_t)z; r->e[r->n].y = (int16_t)y; r->e[r->n].x = (int16_t)x; r->e[r->n].w = w;
+  r->n++;
+}
+static void row_free(row* r) { free(r->e); r->e = NULL; r->n = r->cap = 0; }
+static void row_copy(row* dst, const row* src)
+{
+  row_reserve(dst, src->n);
+  memcpy(dst->e, src->e, (size_t)src->n * sizeof(elem));
+  dst->n = src->n;
+}
+/* coordinates_less: z, then y, then x (ProjMatrixElemsForOneBinValue.inl) */
+static int coord_cmp(const void* a, const void* b)
+{
+  const elem* p = (const elem*)a; const elem* q = (const elem*)b;
+  if (p->z != q->z) return p->z < q->z ? -1 : 1;
+  if (p->y != q->y) return p->y < q->y ? -1 : 1;
+  if (p->x != q->x) return p->x < q->x ? -1 : 1;
+  return 0;
+}
+static void row_sort(row* r) { if (r->n > 1) qsort(r->e, (size_t)r->n, sizeof(elem), coord_cmp); }
+/* ProjMatrixElemsForOneBin::merge -- ProjMatrixElemsForOneBin.cxx:162-277:
+ * sorts both operands, then a sorted union where equal coordinates sum
+ * (this.value + other.value). Returns early (no sort) if other is empty. */
+static void row_merge(row* lor, row* lor2)
+{
+  if (lor2->n == 0) return;
+  row_sort(lor);
+  row_sort(lor2);
+  const int out_cap = lor->n + lor2->n;
+  elem* out = (elem*)malloc((size_t)out_cap * sizeof(elem));
+  int i = 0, j = 0, k = 0;
+  while (j < lor2->n) {
+    if (i == lor->n) { while (j < lor2->n) out[k++] = lor2->e[j++]; break; }
+    const int c = coord_cmp(&lor2->e[j], &lor->e[i]);
+    if (c == 0) { out[k] = lor->e[i]; out[k].w = lor->e[i].w + lor2->e[j].w; k++; i++; j++; }
+    else if (c < 0) { out[k++] = lor2->e[j++]; }
+    else { out[k++] = lor->e[i++]; }
+  }
+  while (i < lor->n) out[k++] = lor->e[i++];
+  free(lor->e);
+  lor->e = out; lor->n = k; lor->cap = out_cap;
+}
+
+/* ------------------------------------------------------------------ */
+/* plan                                                               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int inplane; /* 0 id,1 x->-x,2 y->-y,3 both,4 swap,5 (x,y)->(-y,x),6 (y,-x),7 (-y,-x) */
+  int zsign;   /* +1 or -1 */
+  int zadd;    /* z' = zsign*z + zadd */
+} symop;
+
+struct or_plan {
+  or_geometry g;
+  int nseg;
+  /* projection geometry */
+  float dphi, phi_offset, dalpha, radius, ring_spacing;
+  int max_tang;
+  float ax_samp[OR_MAX_SEGMENTS], m_offset[OR_MAX_SEGMENTS], delta[OR_MAX_SEGMENTS];
+  int64_t seg_offset[OR_MAX_SEGMENTS]; /* STIR storage offset per segment index */
+  int64_t num_bins;
+  /* image */
+  int max_z, max_y, max_x;
+  /* symmetries */
+  int sym90, sym180, symseg, syms, symz;
+  int ppa[OR_MAX_SEGMENTS];
+  int ppr;
+  float ap2z[OR_MAX_SEGMENTS];
+  /* basic row cache: index (seg-min)*nviews*ntang + view*ntang + (tang-min_tang) */
+  row* basic;
+};
+
+#define SEGI(p, s) ((s) - (p)->g.min_segment_num)
+
+static float get_axial_sampling(const or_plan* p, int seg)
+{
+  const int i = SEGI(p, seg);
+  /* ProjDataInfoCylindrical.inl:121-129 */
+  return p->ring_spacing / (p->g.max_ring_diff[i] != p->g.min_ring_diff[i] ? 2 : 1);
+}
+
+/* get_s -- ProjDataInfoCylindricalNoArcCorr.inl:79-82 (double libm sin, see header),
+ * ProjDataInfoCylindricalArcCorr.inl:31-34 */
+static float get_s(const or_plan* p, int tang)
+{
+  if (p->g.arc_corrected) return tang * p->g.bin_size;
+  return (float)((double)p->radius * sin((double)(tang * p->dalpha)));
+}
+/* ProjDataInfoCylindrical.inl:65-68 */
+static float get_phi(const or_plan* p, int view) { return view * p->dphi + p->phi_offset; }
+/* ProjDataInfoCylindrical.inl:85-93 */
+static float get_tantheta(const or_plan* p, int seg, int tang)
+{
+  const float delta = p->delta[SEGI(p, seg)];
+  if (fabsf(delta) < 0.0001F) return 0;
+  const float R = p->radius;
+  const float s = get_s(p, tang);
+  return (float)((delta * p->ring_spacing) / (2 * sqrt((double)(R * R - s * s))));
+}
+/* ProjDataInfo.inl:200-203 (double sqrt) */
+static float get_costheta(const or_plan* p, int seg, int tang)
+{
+  const float tt = get_tantheta(p, seg, tang);
+  return (float)(1 / sqrt((double)(1 + tt * tt)));
+}
+/* ProjDataInfoCylindrical.inl:71-82 */
+static float get_m(const or_plan* p, int seg, int ax)
+{
+  return ax * get_axial_sampling(p, seg) - p->m_offset[SEGI(p, seg)];
+}
+static float get_t(const or_plan* p, int seg, int ax, int tang)
+{
+  return get_m(p, seg, ax) * get_costheta(p, seg, tang);
+}
+/* ProjDataInfo.cxx:110-115 and ProjDataInfoCylindricalArcCorr.h:70 */
+static float get_sampling_in_s(const or_plan* p, int tang)
+{
+  if (p->g.arc_corrected) return p->g.bin_size;
+  return fabsf(get_s(p, tang + 1) - get_s(p, tang - 1)) / 2;
+}
+
+/* ------------------------------------------------------------------ */
+/* Siddon tracer -- RayTraceVoxelsOnCartesianGrid.cxx:52-269           */
+/* ------------------------------------------------------------------ */
+static void ray_trace(row* lor, float sz, float sy, float sx, float ez, float ey, float ex,
+                      float vz, float vy, float vx, float nc)
+{
+  const float dz = ez - sz, dy = ey - sy, dx = ex - sx;
+  if (norm3(dz, dy, dx) <= .00001F) return;
+  const float d12 = (float)(norm3(dz * vz, dy * vy, dx * vx) * (double)nc);
+  const int sign_x = dx >= 0 ? 1 : -1;
+  const int sign_y = dy >= 0 ? 1 : -1;
+  const int sign_z = dz >= 0 ? 1 : -1;
+  const float small_difference = 1.E-4F;
+  const int zdx = fabsf(dx) <= small_difference;
+  const int zdy = fabsf(dy) <= small_difference;
+  const int zdz = fabsf(dz) <= small_difference;
+  {
+    float iz = 0, iy = 0, ix = 0;
+    if (zdz && is_half_integer(sz)) iz = .5F;
+    else if (zdy && is_half_integer(sy)) iy = .5F;
+    else if (zdx && is_half_integer(sx)) ix = .5F;
+    if (norm3(iz, iy, ix) > .1) {
+      ray_trace(lor, sz - iz, sy - iy, sx - ix, ez - iz, ey - iy, ex - ix, vz, vy, vx, nc / 2);
+      ray_trace(lor, sz + iz, sy + iy, sx + ix, ez + iz, ey + iy, ex + ix, vz, vy, vx, nc / 2);
+      row_sort(lor);
+      return;
+    }
+  }
+  const float inc_x = zdx ? d12 * 1000000.F : d12 / fabsf(dx);
+  const float inc_y = zdy ? d12 * 1000000.F : d12 / fabsf(dy);
+  const float inc_z = zdz ? d12 * 1000000.F : d12 / fabsf(dz);
+  const float xmax = round_f(ex) + sign_x * 0.5F;
+  const float ymax = round_f(ey) + sign_y * 0.5F;
+  const float zmax = round_f(ez) + sign_z * 0.5F;
+  const float axend = zdx ? d12 * 1000000.F : (xmax - sx) * inc_x * sign_x * .9999F;
+  const float ayend = zdy ? d12 * 1000000.F : (ymax - sy) * inc_y * sign_y * .9999F;
+  const float azend = zdz ? d12 * 1000000.F : (zmax - sz) * inc_z * sign_z * .9999F;
+  const float amax = minf_(axend, minf_(ayend, azend));
+  int cz = round_f(sz), cy = round_f(sy), cx = round_f(sx);
+  float az = zdz ? -inc_z : ((cz - sz) - sign_z * 0.5F) * inc_z * sign_z;
+  float ax = zdx ? -inc_x : ((cx - sx) - sign_x * 0.5F) * inc_x * sign_x;
+  float ay = zdy ? -inc_y : ((cy - sy) - sign_y * 0.5F) * inc_y * sign_y;
+  float a = maxf_(ax, maxf_(ay, az));
+  if (zdx) ax = axend; else ax += inc_x;
+  if (zdy) ay = ayend; else ay += inc_y;
+  if (zdz) az = azend; else az += inc_z;
+  while (a < amax) {
+    if (ax < ay) {
+      if (ax < az) { row_push(lor, cz, cy, cx, ax - a); a = ax; ax += inc_x; cx += sign_x; }
+      else { row_push(lor, cz, cy, cx, az - a); a = az; az += inc_z; cz += sign_z; }
+    } else if (ay < az) { row_push(lor, cz, cy, cx, ay - a); a = ay; ay += inc_y; cy += sign_y; }
+    else { row_push(lor, cz, cy, cx, az - a); a = az; az += inc_z; cz += sign_z; }
+  }
+}
+
+/* ray_trace_one_lor -- ProjMatrixByBinUsingRayTracing.cxx:438-579 */
+static void ray_trace_one_lor(row* lor, float s, float t, float cphi, float sphi, float costheta,
+                              float tantheta, float offset_in_z, float fovrad, float vz, float vy,
+                              float vx, int restrict_cyl, int num_lors)
+{
+  float max_a, min_a;
+  if (restrict_cyl) {
+    if (fabsf(s) > fovrad) return;
+    if (fabsf(s) == fovrad) { max_a = min_a = 0; }
+    else { max_a = sqrtf(fovrad * fovrad - s * s); min_a = -max_a; }
+  } else {
+    if (fabsf(cphi) < 1.E-3 || fabsf(sphi) < 1.E-3) {
+      if (fovrad < fabsf(s)) return;
+      max_a = fovrad; min_a = -fovrad;
+    } else {
+      const int sgs = sphi < 0 ? -1 : 1, sgc = cphi < 0 ? -1 : 1;
+      max_a = minf_((fovrad * sgs - s * cphi) / sphi, (fovrad * sgc + s * sphi) / cphi);
+      min_a = maxf_((-fovrad * sgs - s * cphi) / sphi, (-fovrad * sgc + s * sphi) / cphi);
+      if (min_a > max_a - 1.E-3 * vx) return;
+    }
+  }
+  const float st_x = (s * cphi + max_a * sphi) / vx;
+  const float st_y = (s * sphi - max_a * cphi) / vy;
+  const float st_z = (t / costheta + offset_in_z - max_a * tantheta) / vz;
+  const float sp_x = (s * cphi + min_a * sphi) / vx;
+  const float sp_y = (s * sphi - min_a * cphi) / vy;
+  const float sp_z = (t / costheta + offset_in_z - min_a * tantheta) / vz;
+  const int fwd = st_z < sp_z || (st_z == sp_z && (st_y < sp_y || (st_y == sp_y && st_x <= sp_x)));
+  const float nc = 1 / vx / num_lors;
+  if (fwd) ray_trace(lor, st_z, st_y, st_x, sp_z, sp_y, sp_x, vz, vy, vx, nc);
+  else ray_trace(lor, sp_z, sp_y, sp_x, st_z, st_y, st_x, vz, vy, vx, nc);
+}
+
+/* add_adjacent_z -- ProjMatrixByBinUsingRayTracing.cxx:832-954 */
+static void add_adjacent_z(row* lor, float zf, float right)
+{
+  const int org = lor->n;
+  const int z0 = lor->e[0].z;
+  for (int zi = 1;; ++zi) {
+    const float ov = minf_(right, zf + zi + .5F) - maxf_(0.F, zf + zi - .5F);
+    if (ov <= 0.0001) break;
+    const int nz = z0 + zi;
+    row_reserve(lor, lor->n + org);
+    for (int k = 0; k < org; ++k) {
+      const elem e = lor->e[k];
+      row_push(lor, nz, e.y, e.x, ov > .9999 ? e.w : e.w * ov);
+    }
+  }
+  if (zf > 0.5) {
+    const float ov = zf - .5F;
+    const int nz = z0 - 1;
+    row_reserve(lor, lor->n + org);
+    for (int k = 0; k < org; ++k) {
+      const elem e = lor->e[k];
+      row_push(lor, nz, e.y, e.x, ov > .9999 ? e.w : e.w * ov);
+    }
+  }
+  {
+    const float ov = minf_(right, zf + .5F) - maxf_(0.F, zf - .5F);
+    if (ov < .9999)
+      for (int k = 0; k < org; ++k) lor->e[k].w *= ov;
+  }
+}
+
+/* calculate_proj_matrix_elems_for_one_bin -- ProjMatrixByBinUsingRayTracing.cxx:582-829
+ * (use_actual_detector_boundaries = false). */
+static int calc_row(const or_plan* p, int seg, int view, int ax, int tang, row* lor)
+{
+  const or_geometry* g = &p->g;
+  lor->n = 0;
+  const float s_in_mm = get_s(p, tang);
+  const float phi = get_phi(p, view);
+  const float cphi = cosf(phi);
+  const float sphi = sinf(phi);
+  const float tantheta = get_tantheta(p, seg, tang);
+  const float costheta = 1 / sqrtf(1 + tantheta * tantheta);
+  const float t_in_mm = get_t(p, seg, ax, tang);
+  const float samp_z = (get_axial_sampling(p, seg) * get_costheta(p, seg, tang)) / costheta;
+  const int num_lors = (int)ceil(samp_z / g->vox_z - 1.E-3);
+  if (fabs(samp_z / g->vox_z - num_lors) > 1E-3) return -1;
+  const float zfirst = (-samp_z / (2 * num_lors) * (num_lors - 1)) - g->origin_z;
+  float offset_in_z = zfirst + (p->max_z + g->min_z) / 2.F * g->vox_z;
+  if (tantheta == 0) {
+    if (fabs(modulo_f((t_in_mm + offset_in_z) / g->vox_z, 1.F) - .5) < .001)
+      offset_in_z -= .1F * g->vox_z;
+  }
+  const float fovrad = minf_((mini_(p->max_x, -g->min_x)) * g->vox_x, (mini_(p->max_y, -g->min_y)) * g->vox_y);
+  if (g->num_tangential_LORs == 1) {
+    ray_trace_one_lor(lor, s_in_mm, t_in_mm, cphi, sphi, costheta, tantheta, offset_in_z, fovrad,
+                      g->vox_z, g->vox_y, g->vox_x, g->restrict_to_cylindrical_FOV, num_lors);
+  } else {
+    row tmp = {0};
+    const int n = g->num_tangential_LORs;
+    const float s_inc = 1 * get_sampling_in_s(p, tang) / n;
+    float cur = s_in_mm - s_inc * (n - 1) / 2.F;
+    for (int k = 1; k <= n; ++k, cur += s_inc) {
+      tmp.n = 0;
+      ray_trace_one_lor(&tmp, cur, t_in_mm, cphi, sphi, costheta, tantheta, offset_in_z, fovrad,
+                        g->vox_z, g->vox_y, g->vox_x, g->restrict_to_cylindrical_FOV, num_lors * n);
+      row_merge(lor, &tmp);
+    }
+    row_free(&tmp);
+  }
+  if (lor->n > 0) {
+    if (tantheta == 0) {
+      const float zf = lor->e[0].z + g->origin_z / g->vox_z - (p->max_z + g->min_z) / 2.F;
+      const float left = (t_in_mm - samp_z / 2) / g->vox_z;
+      const float right = (t_in_mm + samp_z / 2) / g->vox_z;
+      add_adjacent_z(lor, zf - left, right - left);
+    } else if (num_lors > 1) {
+      row nxt = {0};
+      row_copy(&nxt, lor);
+      for (int zi = 1; zi < num_lors; ++zi) {
+        for (int k = 0; k < nxt.n; ++k) nxt.e[k].z = (int16_t)(nxt.e[k].z + 1);
+        row_merge(lor, &nxt);
+      }
+      row_free(&nxt);
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* symmetries -- DataSymmetriesForBins_PET_CartesianGrid.{cxx,inl},    */
+/* SymmetryOperations_PET_CartesianGrid.inl                            */
+/* ------------------------------------------------------------------ */
+
+/* find_transform_z -- .inl:67-118 */
+static int find_transform_z(const or_plan* p, int seg, int ax)
+{
+  const int i = SEGI(p, seg);
+  const float delta = p->delta[i];
+  const float tz = (2 * p->ppa[i] * (ax) + p->ppr * delta + 2 * p->ap2z[i]);
+  return (int)floor(tz + 0.5);
+}
+
+enum { OP_ID = 0, OP_XMX = 1, OP_YMY = 2, OP_XMX_YMY = 3, OP_XY_YX = 4, OP_XMY_YX = 5, OP_XY_YMX = 6, OP_XMY_YMX = 7 };
+
+static symop mkop(int inplane, int zq, int q, int z_shift)
+{
+  symop o;
+  o.inplane = inplane;
+  o.zsign = zq ? -1 : 1;
+  o.zadd = zq ? q + z_shift : z_shift;
+  return o;
+}
+
+/* find_sym_op_bin0 -- .inl:120-220 (cylindrical) */
+static symop find_sym_op_bin0(const or_plan* p, int seg, int view, int ax)
+{
+  const int q = find_transform_z(p, abs(seg), p->symz ? 0 : ax);
+  const int z_shift = p->symz ? p->ppa[SEGI(p, seg)] * ax : 0;
+  const int v180 = p->g.num_views, v135 = v180 / 4 * 3, v90 = v180 / 2, v45 = v180 / 4;
+  if (p->sym90 && view > v90 && view <= v135)
+    return (!p->symseg || seg >= 0) ? mkop(OP_XMY_YX, 0, q, z_shift) : mkop(OP_XMY_YX, 1, q, z_shift);
+  else if (p->sym90 && view > v45 && view <= v90)
+    return (!p->symseg || seg >= 0) ? mkop(OP_XY_YX, 1, q, z_shift) : mkop(OP_XY_YX, 0, q, z_shift);
+  else if (p->sym180 && view > v90)
+    return (!p->symseg || seg >= 0) ? mkop(OP_XMX, 1, q, z_shift) : mkop(OP_XMX, 0, q, z_shift);
+  else {
+    if (p->symseg && seg < 0) return mkop(OP_ID, 1, q, z_shift);
+    return mkop(OP_ID, 0, q, z_shift);
+  }
+}
+
+/* find_sym_op_general_bin -- .inl:223-396 (cylindrical) */
+static symop find_sym_op_general_bin(const or_plan* p, int s, int seg, int view, int ax)
+{
+  const int q = find_transform_z(p, abs(seg), p->symz ? 0 : ax);
+  const int z_shift = p->symz ? p->ppa[SEGI(p, seg)] * ax : 0;
+  const int v180 = p->g.num_views, v135 = v180 / 4 * 3, v90 = v180 / 2, v45 = v180 / 4;
+  const int spos = !p->syms || s > 0;
+  if (p->sym90 && view > v90 && view <= v135) {
+    if (!p->symseg || seg > 0)
+      return spos ? mkop(OP_XMY_YX, 0, q, z_shift) : mkop(OP_XY_YMX, 1, q, z_shift);
+    else if (seg < 0)
+      return spos ? mkop(OP_XMY_YX, 1, q, z_shift) : mkop(OP_XY_YMX, 0, q, z_shift);
+    else
+      return spos ? mkop(OP_XMY_YX, 0, q, z_shift) : mkop(OP_XY_YMX, 0, q, z_shift);
+  } else if (p->sym90 && view > v45 && view <= v90) {
+    if (!p->symseg || seg > 0)
+      return spos ? mkop(OP_XY_YX, 1, q, z_shift) : mkop(OP_XMY_YMX, 0, q, z_shift);
+    else if (seg < 0)
+      return spos ? mkop(OP_XY_YX, 0, q, z_shift) : mkop(OP_XMY_YMX, 1, q, z_shift);
+    else
+      return spos ? mkop(OP_XY_YX, 0, q, z_shift) : mkop(OP_XMY_YMX, 0, q, z_shift);
+  } else if (p->sym180 && view > v90) {
+    if (!p->symseg || seg > 0)
+      return spos ? mkop(OP_XMX, 1, q, z_shift) : mkop(OP_YMY, 0, q, z_shift);
+    else
+      return spos ? mkop(OP_XMX, 0, q, z_shift) : mkop(OP_YMY, 1, q, z_shift);
+  } else {
+    const int sneg = p->syms && s < 0;
+    if (!p->symseg || seg > 0)
+      return sneg ? mkop(OP_XMX_YMY, 1, q, z_shift) : mkop(OP_ID, 0, q, z_shift);
+    else if (seg < 0)
+      return sneg ? mkop(OP_XMX_YMY, 0, q, z_shift) : mkop(OP_ID, 1, q, z_shift);
+    else
+      return sneg ? mkop(OP_XMX_YMY, 0, q, z_shift) : mkop(OP_ID, 0, q, z_shift);
+  }
+}
+
+/* find_basic_view_segment_numbers -- .inl:398-447 */
+static int find_basic_vs(const or_plan* p, int* view, int* seg)
+{
+  int change = 0;
+  const int nv = p->g.num_views, v90 = nv >> 1, v45 = v90 >> 1, v135 = v90 + v45;
+  if (p->symseg && *seg < 0) { *seg = -*seg; change = 1; }
+  if (p->sym90) {
+    if (*view >= v135) { *view = nv - *view; return 1; }
+    else if (*view >= v90) { *view = *view - v90; return 1; }
+    else if (*view > v45) { *view = v90 - *view; return 1; }
+  } else if (p->sym180) {
+    if (*view > v90) { *view = nv - *view; return 1; }
+  }
+  return change;
+}
+/* find_basic_bin -- .inl:449-478 */
+static void find_basic_bin(const or_plan* p, int* seg, int* view, int* ax, int* tang)
+{
+  find_basic_vs(p, view, seg);
+  if (p->syms && *tang < 0) *tang = -*tang;
+  if (p->symz && *ax != 0) *ax = 0;
+}
+
+static void apply_op(const symop* o, int z, int y, int x, int* oz, int* oy, int* ox)
+{
+  int nx = x, ny = y;
+  switch (o->inplane) {
+    case OP_ID: break;
+    case OP_XMX: nx = -x; break;
+    case OP_YMY: ny = -y; break;
+    case OP_XMX_YMY: nx = -x; ny = -y; break;
+    case OP_XY_YX: nx = y; ny = x; break;
+    case OP_XMY_YX: nx = -y; ny = x; break;
+    case OP_XY_YMX: nx = y; ny = -x; break;
+    case OP_XMY_YMX: nx = -y; ny = -x; break;
+  }
+  *ox = nx; *oy = ny; *oz = o->zsign * z + o->zadd;
+}
+
+/* find_symmetry_operation_from_basic_bin -- .inl:664-674 */
+static symop sym_op_for_bin(const or_plan* p, int* seg, int* view, int* ax, int* tang)
+{
+  symop o = (*tang == 0) ? find_sym_op_bin0(p, *seg, *view, *ax)
+                         : find_sym_op_general_bin(p, *tang, *seg, *view, *ax);
+  find_basic_bin(p, seg, view, ax, tang);
+  return o;
+}
+
+/* get_related_view_segment_numbers -- .inl:800-849 */
+static int related_vs(const or_plan* p, int view, int seg, int* rv, int* rs)
+{
+  const int nv = p->g.num_views;
+  const int symz = p->symseg && seg != 0;
+  int n = 0;
+  rv[n] = view; rs[n++] = seg;
+  if (symz) { rv[n] = view; rs[n++] = -seg; }
+  if (p->sym180 && p->sym90 && (view % (nv / 2)) != nv / 4) {
+    const int r = view < nv / 2 ? view + nv / 2 : view - nv / 2;
+    rv[n] = r; rs[n++] = seg;
+    if (symz) { rv[n] = r; rs[n++] = -seg; }
+  }
+  if (p->sym180 && (view % (nv / 2)) != 0) {
+    rv[n] = nv - view; rs[n++] = seg;
+    if (symz) { rv[n] = nv - view; rs[n++] = -seg; }
+  }
+  if (p->sym90 && (view % (nv / 4)) != 0) {
+    const int r = (nv / 2 - view + nv) % nv;
+    rv[n] = r; rs[n++] = seg;
+    if (symz) { rv[n] = r; rs[n++] = -seg; }
+  }
+  return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* plan creation                                                       */
+/* ------------------------------------------------------------------ */
+static size_t basic_index(const or_plan* p, int seg, int view, int tang)
+{
+  return ((size_t)SEGI(p, seg) * p->g.num_views + view) * p->g.num_tangential_poss +
+         (tang - p->g.min_tangential_pos_num);
+}
+
+or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
+{
+  or_plan* p = (or_plan*)calloc(1, sizeof(or_plan));
+  p->g = *g;
+  p->nseg = g->max_segment_num - g->min_segment_num + 1;
+  if (p->nseg <= 0 || p->nseg > OR_MAX_SEGMENTS) { snprintf(err, errlen, "bad segment range"); free(p); return NULL; }
+  /* ProjDataInfoCylindrical ctor -- ProjDataInfoCylindrical.cxx:56-117 */
+  p->dphi = (float)(OR_PI / g->num_views);
+  p->phi_offset = g->intrinsic_tilt;
+  {
+    const int ndet = g->num_detectors_per_ring;
+    if (ndet > 2 && g->num_views * 2 != ndet && (ndet % (g->num_views * 2)) == 0) {
+      const int mash = ndet / 2 / g->num_views;
+      const float offset_inc = (float)(OR_PI / (ndet / 2) * (mash - 1) / 2.F);
+      p->phi_offset += offset_inc;
+    }
+  }
+  p->radius = g->effective_ring_radius;
+  p->ring_spacing = g->ring_spacing;
+  p->dalpha = (float)(OR_PI / g->num_detectors_per_ring); /* NoArcCorr.cxx:77-93 */
+  p->max_tang = g->min_tangential_pos_num + g->num_tangential_poss - 1;
+  for (int s = g->min_segment_num; s <= g->max_segment_num; ++s) {
+    const int i = SEGI(p, s);
+    p->ax_samp[i] = get_axial_sampling(p, s);
+    /* m_offset -- ProjDataInfoCylindrical.cxx:139-145 */
+    p->m_offset[i] = ((g->num_axial_poss[i] - 1 + 0) * p->ax_samp[i]) / 2;
+    /* get_average_ring_difference -- .inl:132-138 */
+    p->delta[i] = (g->min_ring_diff[i] + g->max_ring_diff[i]) / 2.F;
+  }
+  /* storage offsets in segment order 0,+1,-1,... (ProjData.cxx:641-661) */
+  {
+    int64_t off = 0;
+    int seq[OR_MAX_SEGMENTS], ns = 0;
+    seq[ns++] = 0;
+    for (int s = 1; ns < p->nseg; ++s) {
+      if (s <= g->max_segment_num) seq[ns++] = s;
+      if (-s >= g->min_segment_num) seq[ns++] = -s;
+    }
+    for (int k = 0; k < ns; ++k) {
+      const int i = SEGI(p, seq[k]);
+      p->seg_offset[i] = off;
+      off += (int64_t)g->num_axial_poss[i] * g->num_views * g->num_tangential_poss;
+    }
+    p->num_bins = off;
+  }
+  p->max_z = g->min_z + g->nz - 1;
+  p->max_y = g->min_y + g->ny - 1;
+  p->max_x = g->min_x + g->nx - 1;
+  /* symmetries -- DataSymmetriesForBins_PET_CartesianGrid.cxx:241-378 */
+  p->sym90 = g->do_symmetry_90degrees_min_phi;
+  p->sym180 = g->do_symmetry_90degrees_min_phi || g->do_symmetry_180degrees_min_phi;
+  p->symseg = g->do_symmetry_swap_segment;
+  p->syms = g->do_symmetry_swap_s;
+  p->symz = g->do_symmetry_shift_z;
+  {
+    const float zo = g->origin_z / g->vox_z;
+    if (fabsf(round_f(zo) - zo) > 1.E-3F) { snprintf(err, errlen, "z origin not a multiple of plane spacing"); free(p); return NULL; }
+    if (fabsf(g->vox_y - g->vox_x) > 2.E-3F) p->sym90 = 0;
+    if (g->num_views % 4 != 0) p->sym90 = 0;
+    if (g->num_views % 2 != 0) p->sym180 = 0;
+    if (fabsf(get_phi(p, 0)) > 1.E-4F && (p->sym90 || p->sym180)) p->sym90 = p->sym180 = 0;
+    if (fabsf(g->origin_x) > .01F || fabsf(g->origin_y) > .01F) p->sym90 = p->sym180 = p->symseg = p->syms = 0;
+    if (g->min_segment_num > g->max_segment_num) { free(p); return NULL; }
+    if (fabsf(get_tantheta(p, 0, 0)) > 1.E-4F) { snprintf(err, errlen, "segment 0 is not direct"); free(p); return NULL; }
+  }
+  /* find_relation_between_coordinate_systems -- .cxx:51-125 */
+  {
+    const float ips = g->vox_z;
+    const float ppr_f = p->ring_spacing / ips;
+    p->ppr = round_f(ppr_f);
+    if (fabsf(ppr_f - p->ppr) > 1.E-2) { snprintf(err, errlen, "z spacing must divide the ring spacing"); free(p); return NULL; }
+    for (int s = g->min_segment_num; s <= g->max_segment_num; ++s) {
+      const int i = SEGI(p, s);
+      const float f = p->ax_samp[i] / ips;
+      p->ppa[i] = round_f(f);
+      if (fabsf(f - p->ppa[i]) > 1.E-2) { snprintf(err, errlen, "z spacing must divide the axial sampling"); free(p); return NULL; }
+      const float delta = p->delta[i];
+      p->ap2z[i] = (p->max_z + g->min_z) / 2.F - g->origin_z / ips -
+                   (p->ppa[i] * (g->num_axial_poss[i] - 1 + 0) + p->ppr * delta) / 2;
+    }
+  }
+  /* PMRT restrictions -- ProjMatrixByBinUsingRayTracing.cxx:276-277 */
+  if (fabsf(g->origin_x) > .05F || fabsf(g->origin_y) > .05F) {
+    snprintf(err, errlen, "shifted x/y origin is not supported");
+    free(p);
+    return NULL;
+  }
+  /* basic rows (the reference's basic-bin cache) */
+  const size_t nb = (size_t)p->nseg * g->num_views * g->num_tangential_poss;
+  p->basic = (row*)calloc(nb, sizeof(row));
+  int fail = 0;
+#pragma omp parallel for schedule(dynamic) collapse(2) reduction(| : fail)
+  for (int si = 0; si < p->nseg; ++si)
+    for (int v = 0; v < g->num_views; ++v) {
+      const int s = si + g->min_segment_num;
+      for (int t = g->min_tangential_pos_num; t <= p->max_tang; ++t) {
+        int bs = s, bv = v, ba = 0, bt = t;
+        find_basic_bin(p, &bs, &bv, &ba, &bt);
+        if (bs != s || bv != v || bt != t || ba != 0) continue;
+        if (calc_row(p, s, v, 0, t, &p->basic[basic_index(p, s, v, t)]) != 0) fail |= 1;
+      }
+    }
+  if (fail) {
+    snprintf(err, errlen, "axial sampling is not an integer multiple of the z voxel size");
+    or_plan_destroy(p);
+    return NULL;
+  }
+  return p;
+}
+
+void or_plan_destroy(or_plan* p)
+{
+  if (!p) return;
+  if (p->basic) {
+    const size_t nb = (size_t)p->nseg * p->g.num_views * p->g.num_tangential_poss;
+    for (size_t i = 0; i < nb; ++i) row_free(&p->basic[i]);
+    free(p->basic);
+  }
+  free(p);
+}
+
+void or_symmetry_flags(const or_plan* p, int32_t* out)
+{
+  out[0] = p->sym90; out[1] = p->sym180; out[2] = p->symseg; out[3] = p->syms; out[4] = p->symz;
+}
+
+int64_t or_num_bins(const or_plan* p) { return p->num_bins; }
+
+int64_t or_bin_index(const or_plan* p, int seg, int view, int ax, int tang)
+{
+  const int i = SEGI(p, seg);
+  return p->seg_offset[i] + ((int64_t)ax * p->g.num_views + view) * p->g.num_tangential_poss +
+         (tang - p->g.min_tangential_pos_num);
+}
+
+/* basic row + symmetry op of a bin (ProjMatrixByBin.inl:48-80) */
+static const row* row_and_op(const or_plan* p, int seg, int view, int ax, int tang, symop* op)
+{
+  int bs = seg, bv = view, ba = ax, bt = tang;
+  *op = sym_op_for_bin(p, &bs, &bv, &ba, &bt);
+  return &p->basic[basic_index(p, bs, bv, bt)];
+}
+
+static int export_row(const row* r, int16_t* z, int16_t* y, int16_t* x, float* w, int cap)
+{
+  if (r->n > cap) return -r->n;
+  for (int k = 0; k < r->n; ++k) { z[k] = r->e[k].z; y[k] = r->e[k].y; x[k] = r->e[k].x; w[k] = r->e[k].w; }
+  return r->n;
+}
+
+int or_get_row(const or_plan* p, int seg, int view, int ax, int tang,
+               int16_t* z, int16_t* y, int16_t* x, float* w, int cap)
+{
+  symop op;
+  const row* b = row_and_op(p, seg, view, ax, tang, &op);
+  if (b->n > cap) return -b->n;
+  for (int k = 0; k < b->n; ++k) {
+    int oz, oy, ox;
+    apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &oz, &oy, &ox);
+    z[k] = (int16_t)oz; y[k] = (int16_t)oy; x[k] = (int16_t)ox; w[k] = b->e[k].w;
+  }
+  return b->n;
+}
+
+int or_trace_row(const or_plan* p, int seg, int view, int ax, int tang,
+                 int16_t* z, int16_t* y, int16_t* x, float* w, int cap)
+{
+  row r = {0};
+  if (calc_row(p, seg, view, ax, tang, &r) != 0) { row_free(&r); return 0; }
+  const int n = export_row(&r, z, y, x, w, cap);
+  row_free(&r);
+  return n;
+}
+
+void or_bin_geometry(const or_plan* p, int seg, int view, int ax, int tang, float* out)
+{
+  out[0] = get_s(p, tang);
+  out[1] = get_phi(p, view);
+  out[2] = get_tantheta(p, seg, tang);
+  out[3] = get_costheta(p, seg, tang);
+  out[4] = get_t(p, seg, ax, tang);
+}
+
+/* ------------------------------------------------------------------ */
+/* subsets                                                             */
+/* ------------------------------------------------------------------ */
+/* list (view,seg) of a subset in the reference's processing order:
+ * basic vs (segment ascending, view = subset, subset+n, ...) then related */
+static int subset_vs(const or_plan* p, int subset, int nsub, int* views, int* segs, int cap)
+{
+  int n = 0;
+  for (int s = p->g.min_segment_num; s <= p->g.max_segment_num; ++s)
+    for (int v = subset; v < p->g.num_views; v += nsub) {
+      int bv = v, bs = s;
+      if (find_basic_vs(p, &bv, &bs)) continue;
+      int rv[16], rs[16];
+      const int nr = related_vs(p, v, s, rv, rs);
+      for (int k = 0; k < nr; ++k) {
+        if (n < cap) { views[n] = rv[k]; segs[n] = rs[k]; }
+        ++n;
+      }
+    }
+  return n;
+}
+
+int or_subset_view_segments(const or_plan* p, int subset, int num_subsets,
+                            int32_t* views, int32_t* segs, int cap)
+{
+  int* v = (int*)malloc(sizeof(int) * (size_t)(p->nseg * p->g.num_views + 16));
+  int* s = (int*)malloc(sizeof(int) * (size_t)(p->nseg * p->g.num_views + 16));
+  const int n = subset_vs(p, subset, num_subsets, v, s, p->nseg * p->g.num_views + 16);
+  for (int k = 0; k < n && k < cap; ++k) { views[k] = v[k]; segs[k] = s[k]; }
+  free(v); free(s);
+  return n;
+}
+
+/* subsets_are_approximately_balanced -- ...AndProjData.cxx:490-529: the
+ * number of segment-0 viewgrams per subset must be equal (0/1 of each
+ * related set counted per basic view). Returns 1 when balanced. */
+int or_subsets_balanced(const or_plan* p, int nsub)
+{
+  int first = -1;
+  for (int sub = 0; sub < nsub; ++sub) {
+    int cnt = 0;
+    for (int v = sub; v < p->g.num_views; v += nsub) {
+      int bv = v, bs = 0;
+      if (find_basic_vs(p, &bv, &bs)) continue;
+      int rv[16], rs[16];
+      cnt += related_vs(p, v, 0, rv, rs);
+    }
+    if (first < 0) first = cnt;
+    else if (cnt != first) return 0;
+  }
+  return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* projections                                                         */
+/* ------------------------------------------------------------------ */
+typedef struct { int seg, view; } vs_t;
+
+static int all_vs(const or_plan* p, int subset, int nsub, vs_t** out)
+{
+  const int cap = p->nseg * p->g.num_views + 16;
+  int* v = (int*)malloc(sizeof(int) * (size_t)cap);
+  int* s = (int*)malloc(sizeof(int) * (size_t)cap);
+  int n;
+  if (subset < 0) {
+    n = subset_vs(p, 0, 1, v, s, cap);
+  } else {
+    n = subset_vs(p, subset, nsub, v, s, cap);
+  }
+  *out = (vs_t*)malloc(sizeof(vs_t) * (size_t)(n > 0 ? n : 1));
+  for (int k = 0; k < n; ++k) { (*out)[k].seg = s[k]; (*out)[k].view = v[k]; }
+  free(v); free(s);
+  return n;
+}
+
+static size_t img_index(const or_plan* p, int z, int y, int x)
+{
+  return ((size_t)(z - p->g.min_z) * p->g.ny + (y - p->g.min_y)) * p->g.nx + (x - p->g.min_x);
+}
+
+/* one bin FP: ProjMatrixElemsForOneBin::forward_project -- .cxx:354-371 */
+static float fp_bin(const or_plan* p, const float* img, int seg, int view, int ax, int tang)
+{
+  symop op;
+  const row* b = row_and_op(p, seg, view, ax, tang, &op);
+  float acc = 0.f;
+  for (int k = 0; k < b->n; ++k) {
+    int z, y, x;
+    apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &z, &y, &x);
+    if (z >= p->g.min_z && z <= p->max_z) acc += img[img_index(p, z, y, x)] * b->e[k].w;
+  }
+  return acc;
+}
+static double fp_bin64(const or_plan* p, const double* img, int seg, int view, int ax, int tang)
+{
+  symop op;
+  const row* b = row_and_op(p, seg, view, ax, tang, &op);
+  double acc = 0.;
+  for (int k = 0; k < b->n; ++k) {
+    int z, y, x;
+    apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &z, &y, &x);
+    if (z >= p->g.min_z && z <= p->max_z) acc += img[img_index(p, z, y, x)] * (double)b->e[k].w;
+  }
+  return acc;
+}
+
+void or_forward_project(const or_plan* p, const float* img, float* sino, int subset, int nsub)
+{
+  vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+  for (int k = 0; k < n; ++k) {
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
+      for (int a = 0; a < nax; ++a) sino[or_bin_index(p, seg, view, a, t)] = fp_bin(p, img, seg, view, a, t);
+  }
+  free(vs);
+}
+
+void or_forward_project_omp(const or_plan* p, const float* img, float* sino, int subset, int nsub)
+{
+  vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < n; ++k) {
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
+      for (int a = 0; a < nax; ++a) sino[or_bin_index(p, seg, view, a, t)] = fp_bin(p, img, seg, view, a, t);
+  }
+  free(vs);
+}
+
+void or_forward_project_f64(const or_plan* p, const double* img, double* sino, int subset, int nsub)
+{
+  vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < n; ++k) {
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
+      for (int a = 0; a < nax; ++a) sino[or_bin_index(p, seg, view, a, t)] = fp_bin64(p, img, seg, view, a, t);
+  }
+  free(vs);
+}
+
+/* ProjMatrixElemsForOneBin::back_project -- .cxx:333-352 (skips zero data) */
+static void bp_bin(const or_plan* p, float* img, float data, int seg, int view, int ax, int tang)
+{
+  if (data == 0) return;
+  symop op;
+  const row* b = row_and_op(p, seg, view, ax, tang, &op);
+  for (int k = 0; k < b->n; ++k) {
+    int z, y, x;
+    apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &z, &y, &x);
+    if (z >= p->g.min_z && z <= p->max_z) img[img_index(p, z, y, x)] += b->e[k].w * data;
+  }
+}
+static void bp_bin64(const or_plan* p, double* img, double data, int seg, int view, int ax, int tang)
+{
+  if (data == 0) return;
+  symop op;
+  const row* b = row_and_op(p, seg, view, ax, tang, &op);
+  for (int k = 0; k < b->n; ++k) {
+    int z, y, x;
+    apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &z, &y, &x);
+    if (z >= p->g.min_z && z <= p->max_z) img[img_index(p, z, y, x)] += (double)b->e[k].w * data;
+  }
+}
+
+/* serial back projection in the reference's bin order
+ * (BackProjectorByBinUsingProjMatrixByBin.cxx:107-144: tang outer, axial inner) */
+void or_back_project(const or_plan* p, const float* sino, float* img, int subset, int nsub)
+{
+  memset(img, 0, sizeof(float) * (size_t)p->g.nz * p->g.ny * p->g.nx);
+  vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+  for (int k = 0; k < n; ++k) {
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
+      for (int a = 0; a < nax; ++a) bp_bin(p, img, sino[or_bin_index(p, seg, view, a, t)], seg, view, a, t);
+  }
+  free(vs);
+}
+
+void or_back_project_f64(const or_plan* p, const double* sino, double* img, int subset, int nsub)
+{
+  const size_t nv = (size_t)p->g.nz * p->g.ny * p->g.nx;
+  memset(img, 0, sizeof(double) * nv);
+  vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  double* part = (double*)calloc((size_t)nt * nv, sizeof(double));
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < n; ++k) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
+      for (int a = 0; a < nax; ++a)
+        bp_bin64(p, part + (size_t)tid * nv, sino[or_bin_index(p, seg, view, a, t)], seg, view, a, t);
+  }
+  for (int t = 0; t < nt; ++t)
+    for (size_t i = 0; i < nv; ++i) img[i] += part[(size_t)t * nv + i];
+  free(part);
+  free(vs);
+}
+
+/* OpenMP back projection: per-thread images reduced serially
+ * (BackProjectorByBin.cxx:316-346) -- the timed CPU baseline */
+void or_back_project_omp(const or_plan* p, const float* sino, float* img, int subset, int nsub)
+{
+  const size_t nv = (size_t)p->g.nz * p->g.ny * p->g.nx;
+  memset(img, 0, sizeof(float) * nv);
+  vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  float* part = (float*)calloc((size_t)nt * nv, sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < n; ++k) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
+      for (int a = 0; a < nax; ++a)
+        bp_bin(p, part + (size_t)tid * nv, sino[or_bin_index(p, seg, view, a, t)], seg, view, a, t);
+  }
+  for (int t = 0; t < nt; ++t)
+    for (size_t i = 0; i < nv; ++i) img[i] += part[(size_t)t * nv + i];
+  free(part);
+  free(vs);
+}
+
+int64_t or_nnz(const or_plan* p, int subset, int nsub)
+{
+  vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+  int64_t total = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : total)
+  for (int k = 0; k < n; ++k) {
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
+      for (int a = 0; a < nax; ++a) {
+        symop op;
+        const row* b = row_and_op(p, seg, view, a, t, &op);
+        for (int e = 0; e < b->n; ++e) {
+          const int z = op.zsign * b->e[e].z + op.zadd;
+          if (z >= p->g.min_z && z <= p->max_z) total++;
+        }
+      }
+  }
+  free(vs);
+  return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* objective function and OSEM                                          */
+/* ------------------------------------------------------------------ */
+static int is_end_plane(const or_plan* p, int seg, int ax)
+{
+  return seg == 0 && (ax == 0 || ax == p->g.num_axial_poss[SEGI(p, 0)] - 1);
+}
+
+/* RPC_process_related_viewgrams_sensitivity_computation -- ...AndProjData.cxx:1517-1539,
+ * with mult viewgrams = ones and zeroed seg-0 end planes when requested
+ * (distributable.cxx:200-223). */
+void or_subset_sensitivity(const or_plan* p, int subset, int nsub, int zero_end_planes, float* img)
+{
+  const int64_t nb = p->num_bins;
+  float* ones = (float*)malloc(sizeof(float) * (size_t)nb);
+  for (int64_t i = 0; i < nb; ++i) ones[i] = 1.F;
+  if (zero_end_planes) {
+    for (int v = 0; v < p->g.num_views; ++v)
+      for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+        ones[or_bin_index(p, 0, v, 0, t)] = 0;
+        ones[or_bin_index(p, 0, v, p->g.num_axial_poss[SEGI(p, 0)] - 1, t)] = 0;
+      }
+  }
+  or_back_project(p, ones, img, subset, nsub);
+  free(ones);
+}
+
+/* divide_and_truncate for one viewgram -- recon_array_functions.cxx:172-266
+ * (rim_truncation_sino = 0). Writes the ratio into q. */
+static void divide_and_truncate_vg(const or_plan* p, const float* y, const float* ybar, float* q,
+                                   int seg, int view, int zero_end_planes)
+{
+  const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+  float mx = -FLT_MAX;
+  for (int a = 0; a < nax; ++a)
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+      float yv = y[or_bin_index(p, seg, view, a, t)];
+      if (zero_end_planes && is_end_plane(p, seg, a)) yv = 0;
+      if (yv > mx) mx = yv;
+    }
+  const float small_value = maxf_(mx * 0.000001F, 0.F);
+  for (int a = 0; a < nax; ++a)
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+      const int64_t i = or_bin_index(p, seg, view, a, t);
+      float num = y[i];
+      if (zero_end_planes && is_end_plane(p, seg, a)) num = 0;
+      if (num <= small_value) num = 0;
+      else {
+        const float max_quotient = 10000.F;
+        const float denom = ybar[i];
+        if (num > max_quotient * denom) num = max_quotient;
+        else num = num / denom;
+      }
+      q[i] = num;
+    }
+}
+
+/* compute_sub_gradient_without_penalty[_plus_sensitivity]
+ * -- ...AndProjData.cxx:694-735, RPC at :1423-1481 */
+void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, const float* x,
+                        int subset, int nsub, int zero_end_planes, int add_sensitivity, float* grad)
+{
+  const int64_t nb = p->num_bins;
+  float* ybar = (float*)calloc((size_t)nb, sizeof(float));
+  float* q = (float*)calloc((size_t)nb, sizeof(float));
+  or_forward_project_omp(p, x, ybar, subset, nsub);
+  vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+  for (int k = 0; k < n; ++k) {
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    if (bkg)
+      for (int a = 0; a < nax; ++a)
+        for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+          const int64_t i = or_bin_index(p, seg, view, a, t);
+          float b = bkg[i];
+          if (zero_end_planes && is_end_plane(p, seg, a)) b = 0;
+          ybar[i] += b;
+        }
+    divide_and_truncate_vg(p, y, ybar, q, seg, view, zero_end_planes);
+    if (!add_sensitivity)
+      for (int a = 0; a < nax; ++a)
+        for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+          const int64_t i = or_bin_index(p, seg, view, a, t);
+          const float m = (zero_end_planes && is_end_plane(p, seg, a)) ? 0.F : 1.F;
+          q[i] -= m;
+        }
+  }
+  free(vs);
+  or_back_project(p, q, grad, subset, nsub);
+  free(ybar); free(q);
+}
+
+/* accumulate_loglikelihood -- recon_array_functions.cxx:336-377 via
+ * RPC_process_related_viewgrams_accumulate_loglikelihood (...AndProjData.cxx:1484-1515) */
+double or_log_likelihood(const or_plan* p, const float* y, const float* bkg, const float* x,
+                         int subset, int nsub, int zero_end_planes)
+{
+  const int64_t nb = p->num_bins;
+  float* ybar = (float*)calloc((size_t)nb, sizeof(float));
+  or_forward_project_omp(p, x, ybar, subset, nsub);
+  vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+  double total = 0;
+  for (int k = 0; k < n; ++k) {
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    float mx = -FLT_MAX;
+    for (int a = 0; a < nax; ++a)
+      for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+        float yv = y[or_bin_index(p, seg, view, a, t)];
+        if (zero_end_planes && is_end_plane(p, seg, a)) yv = 0;
+        if (yv > mx) mx = yv;
+      }
+    const float small_value = maxf_(mx * 0.000001F, 0.F);
+    double result = 0;
+    for (int a = 0; a < nax; ++a) {
+      double sub = 0;
+      const int ep = zero_end_planes && is_end_plane(p, seg, a);
+      for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+        const int64_t i = or_bin_index(p, seg, view, a, t);
+        float est = ybar[i] + (bkg ? (ep ? 0.F : bkg[i]) : 0.F);
+        if (ep) est = est * 0.F;
+        const float yv = ep ? 0.F : y[i];
+        const float ne = maxf_(est, yv / 10000.F);
+        if (yv <= small_value) sub += -(double)ne;
+        else sub += yv * log((double)ne) - (double)ne;
+      }
+      result += sub;
+    }
+    total += result;
+  }
+  free(vs);
+  free(ybar);
+  return total;
+}
+
+/* threshold_min_to_small_positive_value -- thresholding.h:101-109 */
+static void threshold_min_to_small_positive(float* x, size_t n, float small_number)
+{
+  float minpos = 0; int found = 0;
+  for (size_t i = 0; i < n; ++i)
+    if (x[i] > 0 && (!found || x[i] < minpos)) { minpos = x[i]; found = 1; }
+  if (found) {
+    const float thr = minpos * small_number;
+    for (size_t i = 0; i < n; ++i) if (thr > x[i]) x[i] = thr;
+  } else {
+    for (size_t i = 0; i < n; ++i) x[i] = small_number;
+  }
+}
+
+void or_osem(const or_plan* p, const float* y, const float* bkg, float* x, int nsub, int nsubiter,
+             int start_subset, int zero_end_planes, int use_omp)
+{
+  const size_t nv = (size_t)p->g.nz * p->g.ny * p->g.nx;
+  float** sens = (float**)malloc(sizeof(float*) * (size_t)nsub);
+  for (int s = 0; s < nsub; ++s) {
+    sens[s] = (float*)malloc(sizeof(float) * nv);
+    or_subset_sensitivity(p, s, nsub, zero_end_planes, sens[s]);
+  }
+  float* u = (float*)malloc(sizeof(float) * nv);
+  threshold_min_to_small_positive(x, nv, 0.000001F);
+  for (int k = 1; k <= nsubiter; ++k) {
+    const int subset = (k + start_subset - 1) % nsub;
+    if (use_omp) {
+      /* same maths, OpenMP back projection */
+      const int64_t nb = p->num_bins;
+      float* ybar = (float*)calloc((size_t)nb, sizeof(float));
+      float* q = (float*)calloc((size_t)nb, sizeof(float));
+      or_forward_project_omp(p, x, ybar, subset, nsub);
+      vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+      for (int j = 0; j < n; ++j) {
+        const int seg = vs[j].seg, view = vs[j].view;
+        const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+        if (bkg)
+          for (int a = 0; a < nax; ++a)
+            for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+              const int64_t i = or_bin_index(p, seg, view, a, t);
+              ybar[i] += (zero_end_planes && is_end_plane(p, seg, a)) ? 0.F : bkg[i];
+            }
+        divide_and_truncate_vg(p, y, ybar, q, seg, view, zero_end_planes);
+      }
+      free(vs);
+      or_back_project_omp(p, q, u, subset, nsub);
+      free(ybar); free(q);
+    } else {
+      or_subset_gradient(p, y, bkg, x, subset, nsub, zero_end_planes, 1, u);
+    }
+    /* divide(update, subsens, 0.F) -- numerics/divide.inl:26-47 */
+    {
+      float mx = -FLT_MAX;
+      for (size_t i = 0; i < nv; ++i) if (u[i] > mx) mx = u[i];
+      float small_value = mx * 0.F;
+      small_value = small_value > 0 ? small_value : 0;
+      for (size_t i = 0; i < nv; ++i) {
+        if (fabsf(sens[subset][i]) <= small_value && fabsf(u[i]) <= small_value) u[i] = 0;
+        else u[i] /= sens[subset][i];
+      }
+    }
+    /* threshold_upper_lower(0, FLT_MAX) except subiteration 1 (OSMAPOSLReconstruction.cxx:491-507) */
+    if (k != 1)
+      for (size_t i = 0; i < nv; ++i) {
+        if (u[i] > FLT_MAX) u[i] = FLT_MAX;
+        else if (0.F > u[i]) u[i] = 0.F;
+      }
+    for (size_t i = 0; i < nv; ++i) x[i] *= u[i];
+  }
+  free(u);
+  for (int s = 0; s < nsub; ++s) free(sens[s]);
+  free(sens);
+}

@@ -1,0 +1,147 @@
+/*
+ * CPU oracle for the ray-tracing-matrix projector pair and OSEM.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of STIR's
+ * ProjMatrixByBinUsingRayTracing (PMRT) projector pair, its PET
+ * Cartesian-grid symmetries, the Poisson log-likelihood sub-gradient and
+ * the OSMAPOSL multiplicative update.  It is the checker that the HIP
+ * path is compared against; it is never called by the product library.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it.
+ *
+ * Every function cites the reference file:line it restates (paths are
+ * relative to the STIR source tree, STIR 6.5.0).
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   - RPTsens_seg3_PM (reference-held golden sensitivity image): pins the
+ *     Siddon tracer, FOV clipping, multi-tangential merging, the axial
+ *     TOR handling for span-1 data, the symmetries and the back projector.
+ *   - closed forms: axis-aligned rays, total row length, z-TOR overlaps.
+ *   - brute force: an FP64 exact-interval tracer on tiny grids.
+ *   - symmetric rows equal directly traced rows (reference invariant
+ *     test_DataSymmetriesForBins_PET_CartesianGrid.cxx:112-122).
+ *   - adjoint identity, finite-difference gradient, one-voxel EM closed
+ *     form, OSEM likelihood monotonicity.
+ * Parity unpinned: the choice between float and double libm overloads in
+ * get_s/get_tantheta/get_costheta (depends on header inclusion order in
+ * the reference build, which cannot be compiled here) and the OpenMP
+ * reduction order of the reference back projector.
+ */
+#ifndef PMRT_ORACLE_H
+#define PMRT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_SEGMENTS 256
+
+/* Geometry of one reconstruction problem. Segment arrays are indexed by
+ * (segment_num - min_segment_num). Image index ranges follow
+ * VoxelsOnCartesianGrid (z in [min_z, min_z+nz), y in [min_y, ...), x ...). */
+typedef struct {
+  /* scanner / ProjDataInfoCylindrical */
+  int32_t num_rings;
+  int32_t num_detectors_per_ring;
+  float effective_ring_radius; /* mm */
+  float ring_spacing;          /* mm */
+  float intrinsic_tilt;        /* rad */
+  int32_t arc_corrected;       /* 1: ProjDataInfoCylindricalArcCorr */
+  float bin_size;              /* mm, arc-corrected tangential sampling */
+  int32_t num_views;
+  int32_t num_tangential_poss;
+  int32_t min_tangential_pos_num;
+  int32_t min_segment_num;
+  int32_t max_segment_num;
+  int32_t min_ring_diff[OR_MAX_SEGMENTS];
+  int32_t max_ring_diff[OR_MAX_SEGMENTS];
+  int32_t num_axial_poss[OR_MAX_SEGMENTS];
+  /* image / VoxelsOnCartesianGrid */
+  int32_t nz, ny, nx;
+  int32_t min_z, min_y, min_x;
+  float vox_z, vox_y, vox_x;
+  float origin_z, origin_y, origin_x;
+  /* Ray Tracing Matrix Parameters (ProjMatrixByBinUsingRayTracing.cxx:76-89) */
+  int32_t restrict_to_cylindrical_FOV;
+  int32_t num_tangential_LORs;
+  int32_t do_symmetry_90degrees_min_phi;
+  int32_t do_symmetry_180degrees_min_phi;
+  int32_t do_symmetry_swap_segment;
+  int32_t do_symmetry_swap_s;
+  int32_t do_symmetry_shift_z;
+} or_geometry;
+
+typedef struct or_plan or_plan;
+
+/* Build the plan: geometry, symmetries and all basic rows (the reference's
+ * basic-bin cache, ProjMatrixByBin.inl:48-113). Returns NULL on error. */
+or_plan* or_plan_create(const or_geometry* g, char* err, int errlen);
+void or_plan_destroy(or_plan* p);
+
+/* Effective symmetry flags after the reference's auto-disable rules
+ * (DataSymmetriesForBins_PET_CartesianGrid.cxx:241-378). out[5]. */
+void or_symmetry_flags(const or_plan* p, int32_t* out);
+
+/* Total number of bins, and the STIR ProjDataInMemory index of a bin
+ * (ProjDataInMemory.cxx:190-235, ProjData.cxx:641-661). */
+int64_t or_num_bins(const or_plan* p);
+int64_t or_bin_index(const or_plan* p, int seg, int view, int ax, int tang);
+
+/* Row of one bin through basic bin + symmetry operation
+ * (ProjMatrixByBin.inl:48-113). Returns the number of elements, or
+ * -(needed) when cap is too small. */
+int or_get_row(const or_plan* p, int seg, int view, int ax, int tang,
+               int16_t* z, int16_t* y, int16_t* x, float* w, int cap);
+/* Row computed directly for the given bin (no symmetry), as
+ * calculate_proj_matrix_elems_for_one_bin (PMRT.cxx:582-829). */
+int or_trace_row(const or_plan* p, int seg, int view, int ax, int tang,
+                 int16_t* z, int16_t* y, int16_t* x, float* w, int cap);
+/* Per-bin geometry (s, phi, tantheta, costheta, t) as float[5]. */
+void or_bin_geometry(const or_plan* p, int seg, int view, int ax, int tang, float* out5);
+
+/* Subsets: number of (view,segment) pairs in subset and the list
+ * (find_basic_vs_nums_in_subset.cxx:31-69 + related viewgrams
+ * DataSymmetriesForBins_PET_CartesianGrid.inl:801-849). */
+int or_subset_view_segments(const or_plan* p, int subset, int num_subsets,
+                            int32_t* views, int32_t* segs, int cap);
+/* Subset balance test (PoissonLogLikelihood...AndProjData.cxx:490-529). */
+int or_subsets_balanced(const or_plan* p, int num_subsets);
+
+/* Projections in STIR layouts: image [z][y][x], projection data
+ * ProjDataInMemory order. subset < 0 means all bins. FP writes only the
+ * bins of the subset. */
+void or_forward_project(const or_plan* p, const float* img, float* sino, int subset, int num_subsets);
+void or_forward_project_f64(const or_plan* p, const double* img, double* sino, int subset, int num_subsets);
+void or_back_project(const or_plan* p, const float* sino, float* img, int subset, int num_subsets);
+void or_back_project_f64(const or_plan* p, const double* sino, double* img, int subset, int num_subsets);
+/* OpenMP variants: per-thread images reduced serially, as the reference
+ * (BackProjectorByBin.cxx:316-346, distributable.cxx:414-434). */
+void or_forward_project_omp(const or_plan* p, const float* img, float* sino, int subset, int num_subsets);
+void or_back_project_omp(const or_plan* p, const float* sino, float* img, int subset, int num_subsets);
+
+/* Number of matrix elements with z inside the image (nnz after clipping). */
+int64_t or_nnz(const or_plan* p, int subset, int num_subsets);
+
+/* Objective function pieces (PoissonLogLikelihoodWithLinearModelForMeanAndProjData).
+ * bkg may be NULL (no additive term) and is a full sinogram otherwise.
+ * zero_end_planes: "zero end planes of segment 0". */
+void or_subset_sensitivity(const or_plan* p, int subset, int num_subsets, int zero_end_planes, float* img);
+/* compute_sub_gradient_without_penalty(_plus_sensitivity):
+ * grad = BP_subset(y/ybar) (+sens) or BP_subset(y/ybar - 1). */
+void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, const float* x,
+                        int subset, int num_subsets, int zero_end_planes, int add_sensitivity,
+                        float* grad);
+double or_log_likelihood(const or_plan* p, const float* y, const float* bkg, const float* x,
+                         int subset, int num_subsets, int zero_end_planes);
+/* OSMAPOSL (OSMAPOSLReconstruction.cxx:266-516, IterativeReconstruction.cxx:384-428,630-639),
+ * prior-free, subset sensitivities, initial positivity enforced on x. */
+void or_osem(const or_plan* p, const float* y, const float* bkg, float* x,
+             int num_subsets, int num_subiterations, int start_subset, int zero_end_planes,
+             int use_omp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,418 @@
+"""Projection-data and image geometry descriptions.
+
+Host-side mirror of the STIR objects that define a reconstruction problem
+for the ray-tracing matrix projector pair:
+
+* ``Scanner`` -- cylindrical scanner parameters (``Scanner.cxx:357-383,511-535``).
+* ``ProjDataInfoCylindrical`` -- segments / ring differences / views /
+  tangential positions, arc-corrected or not (``ProjDataInfo.cxx:487-601``
+  for the span construction, ``ProjDataInfoCylindrical.cxx:56-117``).
+* ``VoxelsOnCartesianGrid`` -- image index ranges, voxel size, origin
+  (``VoxelsOnCartesianGrid.cxx:230-288``).
+* ``RayTracingMatrixParameters`` -- the ``Ray Tracing Matrix Parameters``
+  keys of ``ProjMatrixByBinUsingRayTracing.cxx:76-104``.
+
+All floating-point parameters are held as float32, as the reference holds
+them, so that the values handed to the native code are the reference's.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field, replace
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+MAX_SEGMENTS = 256
+
+
+def f32(x: float) -> float:
+    """Round a Python float to float32 and back."""
+    return float(np.float32(x))
+
+
+@dataclass(frozen=True)
+class Scanner:
+    """Cylindrical PET scanner (subset of ``stir::Scanner`` used on this path)."""
+
+    name: str
+    num_rings: int
+    num_detectors_per_ring: int
+    inner_ring_radius: float  # mm
+    average_depth_of_interaction: float  # mm
+    ring_spacing: float  # mm
+    default_bin_size: float  # mm
+    intrinsic_tilt: float = 0.0  # rad
+    max_num_non_arccorrected_bins: int = 0
+    default_num_arccorrected_bins: int = 0
+    # TOF description (None for non-TOF scanners)
+    timing_resolution_ps: Optional[float] = None
+
+    @property
+    def effective_ring_radius(self) -> float:
+        # Scanner.inl:92-95 (float + float)
+        return float(np.float32(self.inner_ring_radius) + np.float32(self.average_depth_of_interaction))
+
+    # -- scanners used by the reference's tests and the workloads -----------
+    @staticmethod
+    def RPT() -> "Scanner":
+        """``Scanner::RPT`` (``Scanner.cxx:511-535``)."""
+        return Scanner("RPT", 16, 384, f32(380.0 - 7.0), 7.0, 6.75, f32(3.1088), 0.0, 128, 128)
+
+    @staticmethod
+    def Siemens_mMR(radius_mm: float = 328.0, doi_mm: float = 0.0) -> "Scanner":
+        """mMR geometry (``Scanner.cxx:361-383``: 64 rings, 504 dets, 4.0625 mm).
+
+        The reference's built-in entry uses inner radius 328 mm plus 7 mm DOI
+        (effective 335 mm); the workload description states 328 mm, which is the
+        default here (DOI 0)."""
+        return Scanner("mMR", 64, 504, radius_mm, doi_mm, 4.0625, f32(2.08626), 0.0, 344, 344)
+
+    @staticmethod
+    def cylindrical(num_rings: int, num_detectors_per_ring: int, radius_mm: float,
+                    ring_spacing_mm: float, bin_size_mm: Optional[float] = None,
+                    name: str = "cylindrical") -> "Scanner":
+        if bin_size_mm is None:
+            bin_size_mm = math.pi * radius_mm / num_detectors_per_ring
+        return Scanner(name, num_rings, num_detectors_per_ring, radius_mm, 0.0, ring_spacing_mm,
+                       f32(bin_size_mm), 0.0, num_detectors_per_ring // 2, num_detectors_per_ring // 2)
+
+
+@dataclass(frozen=True)
+class ProjDataInfoCylindrical:
+    """Cylindrical projection data layout (``ProjDataInfoCylindrical{NoArcCorr,ArcCorr}``)."""
+
+    scanner: Scanner
+    min_ring_diff: Tuple[int, ...]  # indexed by segment - min_segment_num
+    max_ring_diff: Tuple[int, ...]
+    num_axial_poss: Tuple[int, ...]
+    min_segment_num: int
+    num_views: int
+    num_tangential_poss: int
+    arc_corrected: bool = False
+    bin_size: float = 0.0
+    num_tof_bins: int = 1
+
+    # ------------------------------------------------------------------
+    @staticmethod
+    def construct(scanner: Scanner, span: int, max_delta: int, num_views: int,
+                  num_tangential_poss: int, arc_corrected: bool = False) -> "ProjDataInfoCylindrical":
+        """``ProjDataInfo::ProjDataInfoCTI`` (``ProjDataInfo.cxx:487-601``)."""
+        num_ring = scanner.num_rings
+        if max_delta > num_ring - 1:
+            raise ValueError(f"max_ring_difference {max_delta} too large for {num_ring} rings")
+        if span < 1 or span > 2 * num_ring - 1:
+            raise ValueError(f"span {span} out of range")
+        if max_delta < (span - 1) // 2:
+            raise ValueError("max_ring_difference has to be at least (span-1)/2")
+        rdmin: List[int] = []
+        rdmax: List[int] = []
+        if span % 2 == 1:
+            rdmin.append(-((span - 1) // 2))
+            rdmax.append(rdmin[0] + span - 1)
+        else:
+            rdmin.append(-(span // 2))
+            rdmax.append(rdmin[0] + span)
+        while rdmax[-1] < max_delta:
+            rdmin.append(rdmax[-1] + 1)
+            rdmax.append(rdmin[-1] + span - 1)
+        if rdmax[-1] > max_delta:
+            rdmax[-1] = max_delta
+        max_seg = len(rdmin) - 1
+        mins, maxs, naxs = {}, {}, {}
+        mins[0], maxs[0] = rdmin[0], rdmax[0]
+        for i in range(1, max_seg + 1):
+            maxs[-i] = -rdmin[i]
+            maxs[i] = rdmax[i]
+            mins[-i] = -rdmax[i]
+            mins[i] = rdmin[i]
+        if span == 1:
+            naxs[0] = num_ring
+            for i in range(1, max_seg + 1):
+                naxs[i] = naxs[-i] = num_ring - i
+        else:
+            naxs[0] = 2 * num_ring - 1
+            for i in range(1, max_seg + 1):
+                naxs[i] = naxs[-i] = 2 * num_ring - 1 - 2 * rdmin[i]
+        segs = range(-max_seg, max_seg + 1)
+        return ProjDataInfoCylindrical(
+            scanner=scanner,
+            min_ring_diff=tuple(mins[s] for s in segs),
+            max_ring_diff=tuple(maxs[s] for s in segs),
+            num_axial_poss=tuple(naxs[s] for s in segs),
+            min_segment_num=-max_seg,
+            num_views=num_views,
+            num_tangential_poss=num_tangential_poss,
+            arc_corrected=arc_corrected,
+            bin_size=scanner.default_bin_size,
+        )
+
+    def restrict_segments(self, max_abs_segment: int) -> "ProjDataInfoCylindrical":
+        """Keep segments in [-max_abs_segment, max_abs_segment]
+        ("maximum absolute segment number to process")."""
+        lo = max(self.min_segment_num, -max_abs_segment)
+        hi = min(self.max_segment_num, max_abs_segment)
+        sl = slice(lo - self.min_segment_num, hi - self.min_segment_num + 1)
+        return replace(self, min_ring_diff=self.min_ring_diff[sl], max_ring_diff=self.max_ring_diff[sl],
+                       num_axial_poss=self.num_axial_poss[sl], min_segment_num=lo)
+
+    # ------------------------------------------------------------------
+    @property
+    def max_segment_num(self) -> int:
+        return self.min_segment_num + len(self.num_axial_poss) - 1
+
+    @property
+    def segments(self) -> range:
+        return range(self.min_segment_num, self.max_segment_num + 1)
+
+    @property
+    def min_tangential_pos_num(self) -> int:
+        # ProjDataInfo.cxx:125-130
+        return -(self.num_tangential_poss // 2)
+
+    @property
+    def max_tangential_pos_num(self) -> int:
+        return self.min_tangential_pos_num + self.num_tangential_poss - 1
+
+    def num_axial_poss_of(self, seg: int) -> int:
+        return self.num_axial_poss[seg - self.min_segment_num]
+
+    def segment_sequence(self) -> List[int]:
+        """Storage order 0,+1,-1,+2,-2,... (``ProjData.cxx:641-661``)."""
+        seq = [0]
+        s = 1
+        while len(seq) < len(self.num_axial_poss):
+            if s <= self.max_segment_num:
+                seq.append(s)
+            if -s >= self.min_segment_num:
+                seq.append(-s)
+            s += 1
+        return seq
+
+    @property
+    def num_sinograms(self) -> int:
+        return int(sum(self.num_axial_poss))
+
+    @property
+    def num_bins(self) -> int:
+        return self.num_sinograms * self.num_views * self.num_tangential_poss * self.num_tof_bins
+
+    def shape_stir(self) -> Tuple[int, int, int]:
+        """Flat STIR storage as (sinograms, views, tangential)."""
+        return (self.num_sinograms, self.num_views, self.num_tangential_poss)
+
+    def segment_offsets(self) -> Dict[int, int]:
+        off, out = 0, {}
+        for s in self.segment_sequence():
+            out[s] = off
+            off += self.num_axial_poss_of(s) * self.num_views * self.num_tangential_poss
+        return out
+
+    def bin_index(self, seg: int, view: int, ax: int, tang: int) -> int:
+        """Index in ProjDataInMemory order (``ProjDataInMemory.cxx:190-235``)."""
+        off = self.segment_offsets()[seg]
+        return off + (ax * self.num_views + view) * self.num_tangential_poss + (tang - self.min_tangential_pos_num)
+
+    def segment_view(self, data: np.ndarray, seg: int) -> np.ndarray:
+        """View of one segment of a flat STIR-ordered array as [axial, view, tang]."""
+        off = self.segment_offsets()[seg]
+        n = self.num_axial_poss_of(seg) * self.num_views * self.num_tangential_poss
+        return data[off:off + n].reshape(self.num_axial_poss_of(seg), self.num_views, self.num_tangential_poss)
+
+    def end_plane_mask(self) -> np.ndarray:
+        """Bins of the first and last sinogram of segment 0 ("zero end planes of segment 0")."""
+        m = np.zeros(self.num_bins // self.num_tof_bins, dtype=bool)
+        v = self.segment_view(m, 0)
+        v[0] = True
+        v[-1] = True
+        return m
+
+
+@dataclass(frozen=True)
+class VoxelsOnCartesianGrid:
+    """Image geometry (``VoxelsOnCartesianGrid``): index ranges follow the reference,
+    z in [0, nz), y in [-(ny/2), ...], x in [-(nx/2), ...]."""
+
+    nz: int
+    ny: int
+    nx: int
+    vox_z: float
+    vox_y: float
+    vox_x: float
+    origin_z: float = 0.0
+    origin_y: float = 0.0
+    origin_x: float = 0.0
+
+    @property
+    def shape(self) -> Tuple[int, int, int]:
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def num_voxels(self) -> int:
+        return self.nz * self.ny * self.nx
+
+    @property
+    def min_z(self) -> int:
+        return 0
+
+    @property
+    def min_y(self) -> int:
+        return -(self.ny // 2)
+
+    @property
+    def min_x(self) -> int:
+        return -(self.nx // 2)
+
+    @staticmethod
+    def from_proj_data_info(pdi: ProjDataInfoCylindrical, zoom: float = 1.0, xy_size: int = -1,
+                            z_size: int = -1) -> "VoxelsOnCartesianGrid":
+        """``VoxelsOnCartesianGrid(proj_data_info, zoom, origin, sizes)``
+        (``VoxelsOnCartesianGrid.cxx:55-134,230-288``)."""
+        rs = np.float32(pdi.scanner.ring_spacing)
+        z_sampling = rs / np.float32(2)
+        seg0 = -pdi.min_segment_num
+        if z_size < 0:
+            z_size = (pdi.num_axial_poss[seg0] if pdi.max_ring_diff[seg0] > pdi.min_ring_diff[seg0]
+                      else 2 * pdi.num_axial_poss[seg0] - 1)
+        s_sampling = np.float32(pdi.scanner.default_bin_size)
+        vxy = s_sampling / np.float32(zoom)
+        if xy_size < 0:
+            raise ValueError("explicit xy size required")
+        return VoxelsOnCartesianGrid(z_size, xy_size, xy_size, float(z_sampling), float(vxy), float(vxy))
+
+
+@dataclass(frozen=True)
+class RayTracingMatrixParameters:
+    """``Ray Tracing Matrix Parameters`` (``ProjMatrixByBinUsingRayTracing.cxx:76-104``)."""
+
+    restrict_to_cylindrical_FOV: bool = True
+    num_tangential_LORs: int = 1
+    use_actual_detector_boundaries: bool = False
+    do_symmetry_90degrees_min_phi: bool = True
+    do_symmetry_180degrees_min_phi: bool = True
+    do_symmetry_swap_segment: bool = True
+    do_symmetry_swap_s: bool = True
+    do_symmetry_shift_z: bool = True
+
+    def __post_init__(self):
+        if self.num_tangential_LORs < 1:
+            raise ValueError("num_tangential_LORs should be at least 1")
+        if self.use_actual_detector_boundaries:
+            raise NotImplementedError("use_actual_detector_boundaries is not supported")
+
+
+class GeometryStruct(ctypes.Structure):
+    """Plain-C geometry record shared by the native library and the oracle
+    (same field order as ``mi_geometry`` in include/stir_amd_pmrt.h)."""
+
+    _fields_ = [
+        ("num_rings", ctypes.c_int32),
+        ("num_detectors_per_ring", ctypes.c_int32),
+        ("effective_ring_radius", ctypes.c_float),
+        ("ring_spacing", ctypes.c_float),
+        ("intrinsic_tilt", ctypes.c_float),
+        ("arc_corrected", ctypes.c_int32),
+        ("bin_size", ctypes.c_float),
+        ("num_views", ctypes.c_int32),
+        ("num_tangential_poss", ctypes.c_int32),
+        ("min_tangential_pos_num", ctypes.c_int32),
+        ("min_segment_num", ctypes.c_int32),
+        ("max_segment_num", ctypes.c_int32),
+        ("min_ring_diff", ctypes.c_int32 * MAX_SEGMENTS),
+        ("max_ring_diff", ctypes.c_int32 * MAX_SEGMENTS),
+        ("num_axial_poss", ctypes.c_int32 * MAX_SEGMENTS),
+        ("nz", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nx", ctypes.c_int32),
+        ("min_z", ctypes.c_int32),
+        ("min_y", ctypes.c_int32),
+        ("min_x", ctypes.c_int32),
+        ("vox_z", ctypes.c_float),
+        ("vox_y", ctypes.c_float),
+        ("vox_x", ctypes.c_float),
+        ("origin_z", ctypes.c_float),
+        ("origin_y", ctypes.c_float),
+        ("origin_x", ctypes.c_float),
+        ("restrict_to_cylindrical_FOV", ctypes.c_int32),
+        ("num_tangential_LORs", ctypes.c_int32),
+        ("do_symmetry_90degrees_min_phi", ctypes.c_int32),
+        ("do_symmetry_180degrees_min_phi", ctypes.c_int32),
+        ("do_symmetry_swap_segment", ctypes.c_int32),
+        ("do_symmetry_swap_s", ctypes.c_int32),
+        ("do_symmetry_shift_z", ctypes.c_int32),
+    ]
+
+
+def make_geometry_struct(pdi: ProjDataInfoCylindrical, img: VoxelsOnCartesianGrid,
+                         pm: RayTracingMatrixParameters) -> GeometryStruct:
+    g = GeometryStruct()
+    sc = pdi.scanner
+    g.num_rings = sc.num_rings
+    g.num_detectors_per_ring = sc.num_detectors_per_ring
+    g.effective_ring_radius = sc.effective_ring_radius
+    g.ring_spacing = sc.ring_spacing
+    g.intrinsic_tilt = sc.intrinsic_tilt
+    g.arc_corrected = int(pdi.arc_corrected)
+    g.bin_size = pdi.bin_size
+    g.num_views = pdi.num_views
+    g.num_tangential_poss = pdi.num_tangential_poss
+    g.min_tangential_pos_num = pdi.min_tangential_pos_num
+    g.min_segment_num = pdi.min_segment_num
+    g.max_segment_num = pdi.max_segment_num
+    nseg = len(pdi.num_axial_poss)
+    if nseg > MAX_SEGMENTS:
+        raise ValueError("too many segments")
+    for i in range(nseg):
+        g.min_ring_diff[i] = pdi.min_ring_diff[i]
+        g.max_ring_diff[i] = pdi.max_ring_diff[i]
+        g.num_axial_poss[i] = pdi.num_axial_poss[i]
+    g.nz, g.ny, g.nx = img.nz, img.ny, img.nx
+    g.min_z, g.min_y, g.min_x = img.min_z, img.min_y, img.min_x
+    g.vox_z, g.vox_y, g.vox_x = img.vox_z, img.vox_y, img.vox_x
+    g.origin_z, g.origin_y, g.origin_x = img.origin_z, img.origin_y, img.origin_x
+    g.restrict_to_cylindrical_FOV = int(pm.restrict_to_cylindrical_FOV)
+    g.num_tangential_LORs = pm.num_tangential_LORs
+    g.do_symmetry_90degrees_min_phi = int(pm.do_symmetry_90degrees_min_phi)
+    g.do_symmetry_180degrees_min_phi = int(pm.do_symmetry_180degrees_min_phi)
+    g.do_symmetry_swap_segment = int(pm.do_symmetry_swap_segment)
+    g.do_symmetry_swap_s = int(pm.do_symmetry_swap_s)
+    g.do_symmetry_shift_z = int(pm.do_symmetry_shift_z)
+    return g
+
+
+# ----------------------------------------------------------------------
+# the workloads
+# ----------------------------------------------------------------------
+def config0() -> Tuple[ProjDataInfoCylindrical, VoxelsOnCartesianGrid]:
+    """8 rings x 192 crystals, R=300 mm, ring spacing 4 mm, span 1, MRD 3,
+    96 views x 128 tangential; image 128x128x15 at 2 mm."""
+    sc = Scanner.cylindrical(8, 192, 300.0, 4.0, name="cyl8x192")
+    pdi = ProjDataInfoCylindrical.construct(sc, span=1, max_delta=3, num_views=96, num_tangential_poss=128)
+    img = VoxelsOnCartesianGrid(15, 128, 128, 2.0, 2.0, 2.0)
+    return pdi, img
+
+
+def config1(radius_mm: float = 328.0) -> Tuple[ProjDataInfoCylindrical, VoxelsOnCartesianGrid]:
+    """mMR-like: 64 rings x 504 crystals, span 11, MRD 60, 252 views x 344
+    tangential (837 sinograms); image 344x344x127 at 2.08626 x 2.08626 x 2.03125 mm."""
+    sc = Scanner.Siemens_mMR(radius_mm)
+    pdi = ProjDataInfoCylindrical.construct(sc, span=11, max_delta=60, num_views=252, num_tangential_poss=344)
+    img = VoxelsOnCartesianGrid(127, 344, 344, 2.03125, f32(2.08626), f32(2.08626))
+    return pdi, img
+
+
+def rpt_sensitivity_case() -> Tuple[ProjDataInfoCylindrical, VoxelsOnCartesianGrid, RayTracingMatrixParameters]:
+    """Geometry of the reference's RPTsens_seg3_PM fixture
+    (recon_test_pack/OSMAPOSL_test_PM_for_sensitivity.par on Utahscat600k_ca_seg4.hs):
+    RPT, arc-corrected, span 1, segments -3..3, 96 views x 128 tangential,
+    view offset -0.46875 deg, zoom 0.7, 60x60 pixels, 2 tangential LORs."""
+    base = Scanner.RPT()
+    tilt = float(np.float32(-0.46875 * math.pi / 180))  # InterfileHeader.cxx:1381
+    sc = replace(base, ring_spacing=float(np.float32(0.675 * 10.0)), intrinsic_tilt=tilt)
+    pdi = ProjDataInfoCylindrical.construct(sc, span=1, max_delta=4, num_views=96, num_tangential_poss=128,
+                                            arc_corrected=True)
+    pdi = pdi.restrict_segments(3)
+    img = VoxelsOnCartesianGrid.from_proj_data_info(pdi, zoom=0.7, xy_size=60)
+    pm = RayTracingMatrixParameters(num_tangential_LORs=2)
+    return pdi, img, pm
