@@ -1,0 +1,180 @@
+/*
+ * stir_amd_pmrt -- C ABI of the MI355X ray-tracing-matrix projector pair
+ * and OSEM update.
+ *
+ * This is the boundary a STIR ForwardProjectorByBin / BackProjectorByBin
+ * plugin (and a GeneralisedObjectiveFunction plugin for the fused OSEM step)
+ * binds. All entry points take plain pointers and sizes. Arrays use STIR's
+ * contiguous layouts:
+ *   image           [z][y][x]  float32 (VoxelsOnCartesianGrid, x fastest)
+ *   projection data ProjDataInMemory order: segment sequence 0,+1,-1,+2,-2,...,
+ *                   then axial position, view, tangential position (fastest)
+ *                   (ProjDataInMemory.cxx:190-235, ProjData.cxx:641-661).
+ * Every call returns 0 on success and a negative MI_ERR_* code otherwise;
+ * mi_last_error() returns the message of the last failure on the calling
+ * thread. No C++ exception crosses this boundary.
+ *
+ * Threading: a context is used by one host thread at a time. All GPU work of
+ * a context is issued on one HIP stream (the context's own stream, or the
+ * stream passed to the *_dev entry points).
+ */
+#ifndef STIR_AMD_PMRT_H
+#define STIR_AMD_PMRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_MAX_SEGMENTS 256
+
+enum {
+  MI_OK = 0,
+  MI_ERR_ARGUMENT = -1,  /* bad argument or geometry the projector does not support */
+  MI_ERR_STATE = -2,     /* call order violated (e.g. projection before mi_set_up) */
+  MI_ERR_DEVICE = -3,    /* HIP runtime error */
+  MI_ERR_MEMORY = -4,    /* device or host allocation failed */
+  MI_ERR_INTERNAL = -5
+};
+
+/* Geometry of one problem: ProjDataInfoCylindrical{NoArcCorr,ArcCorr} +
+ * VoxelsOnCartesianGrid + "Ray Tracing Matrix Parameters"
+ * (ProjMatrixByBinUsingRayTracing.cxx:76-104). Segment arrays are indexed by
+ * segment_num - min_segment_num. Image index ranges: z in [min_z, min_z+nz),
+ * y in [min_y, min_y+ny), x in [min_x, min_x+nx). */
+typedef struct {
+  int32_t num_rings;
+  int32_t num_detectors_per_ring;
+  float effective_ring_radius; /* mm, Scanner::get_effective_ring_radius */
+  float ring_spacing;          /* mm */
+  float intrinsic_tilt;        /* rad */
+  int32_t arc_corrected;
+  float bin_size; /* mm, tangential sampling of arc-corrected data */
+  int32_t num_views;
+  int32_t num_tangential_poss;
+  int32_t min_tangential_pos_num;
+  int32_t min_segment_num;
+  int32_t max_segment_num;
+  int32_t min_ring_diff[MI_MAX_SEGMENTS];
+  int32_t max_ring_diff[MI_MAX_SEGMENTS];
+  int32_t num_axial_poss[MI_MAX_SEGMENTS];
+  int32_t nz, ny, nx;
+  int32_t min_z, min_y, min_x;
+  float vox_z, vox_y, vox_x;
+  float origin_z, origin_y, origin_x;
+  int32_t restrict_to_cylindrical_FOV;
+  int32_t num_tangential_LORs;
+  int32_t do_symmetry_90degrees_min_phi;
+  int32_t do_symmetry_180degrees_min_phi;
+  int32_t do_symmetry_swap_segment;
+  int32_t do_symmetry_swap_s;
+  int32_t do_symmetry_shift_z;
+} mi_geometry;
+
+/* Sizes of a set-up context. */
+typedef struct {
+  int64_t num_bins;
+  int64_t num_voxels;
+  int64_t num_basic_rows;
+  int64_t num_basic_elements;   /* Siddon elements stored for the basic rows */
+  int64_t num_families;         /* (segment, view, tangential) groups sharing a z-shifted row */
+  int64_t num_matrix_elements;  /* sum of row lengths over all bins, before z clipping */
+  int64_t num_table_entries;    /* entries of the voxel-driven back-projection table */
+  int64_t device_bytes;         /* device memory held by the context */
+  int32_t num_subsets;
+  int32_t symmetry_flags[5];    /* effective 90,180,swap_segment,swap_s,shift_z */
+  int32_t nz_padded;
+} mi_info;
+
+typedef struct mi_ctx mi_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates a context on HIP device `device`. */
+int mi_ctx_create(int device, mi_ctx** out);
+void mi_ctx_destroy(mi_ctx* ctx);
+/* Message of the last failed call on this thread ("" if none). */
+const char* mi_last_error(void);
+
+/* ---- set-up ------------------------------------------------------------- */
+/* Replaces ProjMatrixByBinUsingRayTracing::set_up
+ * (ProjMatrixByBinUsingRayTracing.cxx:239-407) together with
+ * ForwardProjectorByBin::set_up / BackProjectorByBin::set_up
+ * (ForwardProjectorByBin.h:70-73, BackProjectorByBin.h:68-71): builds the
+ * symmetry tables, traces the basic rows on the device (the reference's
+ * basic-bin cache, ProjMatrixByBin.inl:48-113) and the voxel-driven
+ * back-projection table for `num_subsets` OSEM subsets
+ * (find_basic_vs_nums_in_subset.cxx:31-69). May be called again. */
+int mi_set_up(mi_ctx* ctx, const mi_geometry* geometry, int num_subsets);
+int mi_get_info(mi_ctx* ctx, mi_info* out);
+/* Effective symmetries and subset balance, as
+ * ForwardProjectorByBin::get_symmetries_used (ForwardProjectorByBin.h:80) and
+ * subsets_are_approximately_balanced (...AndProjData.cxx:490-529). */
+int mi_subsets_balanced(mi_ctx* ctx, int* balanced);
+/* (view, segment) pairs of a subset in processing order. *count is set to the
+ * number of pairs; at most cap are written. */
+int mi_subset_view_segments(mi_ctx* ctx, int subset, int32_t* views, int32_t* segments, int cap, int* count);
+
+/* Matrix row of one bin as the device uses it (basic row + symmetry
+ * operation), ProjMatrixByBin::get_proj_matrix_elems_for_one_bin
+ * (ProjMatrixByBin.inl:48-113). *n receives the length; at most cap written. */
+int mi_get_row(mi_ctx* ctx, int segment, int view, int axial_pos, int tang_pos,
+               int16_t* z, int16_t* y, int16_t* x, float* w, int cap, int* n);
+
+/* ---- projections, host buffers (STIR layouts) ---------------------------- */
+/* ForwardProjectorByBin::forward_project(ProjData&, image)
+ * (ForwardProjectorByBin.cxx:97-103,168-230). subset < 0: all bins; otherwise
+ * only the bins of that subset are written (others untouched). */
+int mi_forward_project(mi_ctx* ctx, const float* image, float* proj_data, int subset);
+/* BackProjectorByBin::back_project(image, ProjData)
+ * (BackProjectorByBin.cxx:111-117,183-229,316-346); overwrites image. */
+int mi_back_project(mi_ctx* ctx, const float* proj_data, float* image, int subset);
+
+/* ---- projections, device buffers (STIR layouts, device pointers) --------- */
+/* Same operations on device-resident arrays, issued on `stream` (a
+ * hipStream_t; NULL = the context's stream). Used by the PyTorch layer. */
+int mi_forward_project_dev(mi_ctx* ctx, const float* d_image, float* d_proj_data, int subset, void* stream);
+int mi_back_project_dev(mi_ctx* ctx, const float* d_proj_data, float* d_image, int subset, void* stream);
+
+/* ---- objective function / OSEM, data resident on the device -------------- */
+/* Measured data and additive term of
+ * PoissonLogLikelihoodWithLinearModelForMeanAndProjData (set_up_before_sensitivity,
+ * ...AndProjData.cxx:580-686). additive may be NULL (then additive_constant is
+ * used for every bin). zero_end_planes: "zero end planes of segment 0"
+ * (distributable.cxx:145-225). */
+int mi_obj_set_data(mi_ctx* ctx, const float* measured, const float* additive, float additive_constant,
+                    int zero_end_planes);
+/* Subset sensitivities BP_subset(1) (compute_sensitivities,
+ * PoissonLogLikelihoodWithLinearModelForMean.cxx:374-455, use subset sensitivities). */
+int mi_obj_compute_sensitivities(mi_ctx* ctx);
+int mi_obj_get_sensitivity(mi_ctx* ctx, int subset, float* image);
+/* Current estimate (host, STIR layout). mi_obj_set_estimate applies
+ * threshold_min_to_small_positive_value(x, 1e-6) when enforce_positivity
+ * (OSMAPOSLReconstruction.cxx:291-292). */
+int mi_obj_set_estimate(mi_ctx* ctx, const float* image, int enforce_positivity);
+int mi_obj_get_estimate(mi_ctx* ctx, float* image);
+/* One OSMAPOSL subiteration, prior-free (OSMAPOSLReconstruction::update_estimate,
+ * OSMAPOSLReconstruction.cxx:373-516): x *= clamp(BP_s(y/(FP_s(x)+b))/sens_s).
+ * subiteration_num is 1-based (thresholding is skipped for subiteration 1). */
+int mi_osem_subset_step(mi_ctx* ctx, int subset, int subiteration_num);
+/* num_subiterations subiterations starting at start_subiteration_num,
+ * subset = (k + start_subset - 1) % num_subsets
+ * (IterativeReconstruction.cxx:384-428,630-639). */
+int mi_osem_run(mi_ctx* ctx, int start_subiteration_num, int num_subiterations, int start_subset);
+/* compute_sub_gradient_without_penalty[_plus_sensitivity]
+ * (...AndProjData.cxx:694-735, RPC at :1423-1481) at the current estimate;
+ * subset < 0: all subsets. Output image in STIR layout (host). */
+int mi_obj_subset_gradient(mi_ctx* ctx, int subset, int add_sensitivity, float* gradient);
+/* actual_compute_objective_function_without_penalty
+ * (...AndProjData.cxx:739-781, recon_array_functions.cxx:336-377) at the
+ * current estimate, accumulated in FP64. subset < 0: all subsets. */
+int mi_obj_log_likelihood(mi_ctx* ctx, int subset, double* value);
+
+/* Waits for all work of the context. */
+int mi_synchronize(mi_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -137,7 +137,10 @@ struct or_plan {
   int ppa[OR_MAX_SEGMENTS];
   int ppr;
   float ap2z[OR_MAX_SEGMENTS];
-  /* basic row cache: index (seg-min)*nviews*ntang + view*ntang + (tang-min_tang) */
+  /* basic row cache: index (seg-min)*nviews*(2T+1) + view*(2T+1) + (tang+T), where
+   * T = max(-min_tang, max_tang): the basic bin of a tangential position
+   * may lie outside the data's range (e.g. -n/2 -> +n/2) and is still traced */
+  int tmax;
   row* basic;
 };
 
@@ -525,8 +528,7 @@ static int related_vs(const or_plan* p, int view, int seg, int* rv, int* rs)
 /* ------------------------------------------------------------------ */
 static size_t basic_index(const or_plan* p, int seg, int view, int tang)
 {
-  return ((size_t)SEGI(p, seg) * p->g.num_views + view) * p->g.num_tangential_poss +
-         (tang - p->g.min_tangential_pos_num);
+  return ((size_t)SEGI(p, seg) * p->g.num_views + view) * (size_t)(2 * p->tmax + 1) + (size_t)(tang + p->tmax);
 }
 
 or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
@@ -550,6 +552,12 @@ or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
   p->ring_spacing = g->ring_spacing;
   p->dalpha = (float)(OR_PI / g->num_detectors_per_ring); /* NoArcCorr.cxx:77-93 */
   p->max_tang = g->min_tangential_pos_num + g->num_tangential_poss - 1;
+  p->tmax = p->max_tang > -g->min_tangential_pos_num ? p->max_tang : -g->min_tangential_pos_num;
+  if (g->do_symmetry_swap_segment && g->min_segment_num != -g->max_segment_num) {
+    snprintf(err, errlen, "segment symmetry needs a symmetric segment range");
+    free(p);
+    return NULL;
+  }
   for (int s = g->min_segment_num; s <= g->max_segment_num; ++s) {
     const int i = SEGI(p, s);
     p->ax_samp[i] = get_axial_sampling(p, s);
@@ -617,14 +625,14 @@ or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
     return NULL;
   }
   /* basic rows (the reference's basic-bin cache) */
-  const size_t nb = (size_t)p->nseg * g->num_views * g->num_tangential_poss;
+  const size_t nb = (size_t)p->nseg * g->num_views * (size_t)(2 * p->tmax + 1);
   p->basic = (row*)calloc(nb, sizeof(row));
   int fail = 0;
 #pragma omp parallel for schedule(dynamic) collapse(2) reduction(| : fail)
   for (int si = 0; si < p->nseg; ++si)
     for (int v = 0; v < g->num_views; ++v) {
       const int s = si + g->min_segment_num;
-      for (int t = g->min_tangential_pos_num; t <= p->max_tang; ++t) {
+      for (int t = -p->tmax; t <= p->tmax; ++t) {
         int bs = s, bv = v, ba = 0, bt = t;
         find_basic_bin(p, &bs, &bv, &ba, &bt);
         if (bs != s || bv != v || bt != t || ba != 0) continue;
@@ -643,7 +651,7 @@ void or_plan_destroy(or_plan* p)
 {
   if (!p) return;
   if (p->basic) {
-    const size_t nb = (size_t)p->nseg * p->g.num_views * p->g.num_tangential_poss;
+    const size_t nb = (size_t)p->nseg * p->g.num_views * (size_t)(2 * p->tmax + 1);
     for (size_t i = 0; i < nb; ++i) row_free(&p->basic[i]);
     free(p->basic);
   }

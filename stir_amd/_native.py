@@ -1,0 +1,139 @@
+"""ctypes bindings of libstir_amd_pmrt.so (the C ABI in include/stir_amd_pmrt.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C stir_amd/csrc``). There is no fallback: if the library is missing
+or a call fails, a ``NativeError`` is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from .geometry import GeometryStruct
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstir_amd_pmrt.so")
+
+# every symbol declared in include/stir_amd_pmrt.h
+EXPORTED_SYMBOLS = (
+    "mi_ctx_create", "mi_ctx_destroy", "mi_last_error", "mi_set_up", "mi_get_info", "mi_subsets_balanced",
+    "mi_subset_view_segments", "mi_get_row", "mi_forward_project", "mi_back_project", "mi_forward_project_dev",
+    "mi_back_project_dev", "mi_obj_set_data", "mi_obj_compute_sensitivities", "mi_obj_get_sensitivity",
+    "mi_obj_set_estimate", "mi_obj_get_estimate", "mi_osem_subset_step", "mi_osem_run", "mi_obj_subset_gradient",
+    "mi_obj_log_likelihood", "mi_synchronize",
+)
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+class InfoStruct(ctypes.Structure):
+    _fields_ = [
+        ("num_bins", ctypes.c_int64),
+        ("num_voxels", ctypes.c_int64),
+        ("num_basic_rows", ctypes.c_int64),
+        ("num_basic_elements", ctypes.c_int64),
+        ("num_families", ctypes.c_int64),
+        ("num_matrix_elements", ctypes.c_int64),
+        ("num_table_entries", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+        ("num_subsets", ctypes.c_int32),
+        ("symmetry_flags", ctypes.c_int32 * 5),
+        ("nz_padded", ctypes.c_int32),
+    ]
+
+
+_lib = None
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i16p = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def load():
+    """Load the native library (raises NativeError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C stir_amd/csrc")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    c_int = ctypes.c_int
+    L.mi_last_error.restype = ctypes.c_char_p
+    L.mi_last_error.argtypes = []
+    L.mi_ctx_create.argtypes = [c_int, ctypes.POINTER(vp)]
+    L.mi_ctx_destroy.argtypes = [vp]
+    L.mi_ctx_destroy.restype = None
+    L.mi_set_up.argtypes = [vp, ctypes.POINTER(GeometryStruct), c_int]
+    L.mi_get_info.argtypes = [vp, ctypes.POINTER(InfoStruct)]
+    L.mi_subsets_balanced.argtypes = [vp, ctypes.POINTER(c_int)]
+    L.mi_subset_view_segments.argtypes = [vp, c_int, _i32p, _i32p, c_int, ctypes.POINTER(c_int)]
+    L.mi_get_row.argtypes = [vp, c_int, c_int, c_int, c_int, _i16p, _i16p, _i16p, _f32p, c_int, ctypes.POINTER(c_int)]
+    L.mi_forward_project.argtypes = [vp, _f32p, _f32p, c_int]
+    L.mi_back_project.argtypes = [vp, _f32p, _f32p, c_int]
+    L.mi_forward_project_dev.argtypes = [vp, vp, vp, c_int, vp]
+    L.mi_back_project_dev.argtypes = [vp, vp, vp, c_int, vp]
+    L.mi_obj_set_data.argtypes = [vp, _f32p, vp, ctypes.c_float, c_int]
+    L.mi_obj_compute_sensitivities.argtypes = [vp]
+    L.mi_obj_get_sensitivity.argtypes = [vp, c_int, _f32p]
+    L.mi_obj_set_estimate.argtypes = [vp, _f32p, c_int]
+    L.mi_obj_get_estimate.argtypes = [vp, _f32p]
+    L.mi_osem_subset_step.argtypes = [vp, c_int, c_int]
+    L.mi_osem_run.argtypes = [vp, c_int, c_int, c_int]
+    L.mi_obj_subset_gradient.argtypes = [vp, c_int, c_int, _f32p]
+    L.mi_obj_log_likelihood.argtypes = [vp, c_int, ctypes.POINTER(ctypes.c_double)]
+    L.mi_synchronize.argtypes = [vp]
+    for name in EXPORTED_SYMBOLS:
+        fn = getattr(L, name)
+        if name not in ("mi_last_error", "mi_ctx_destroy"):
+            fn.restype = c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().mi_last_error().decode(errors="replace")
+        raise NativeError(f"{what} failed ({rc}): {msg}")
+
+
+class Context:
+    """One device context (mi_ctx)."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        h = ctypes.c_void_p()
+        check(L.mi_ctx_create(device, ctypes.byref(h)), "mi_ctx_create")
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().mi_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> InfoStruct:
+        out = InfoStruct()
+        check(load().mi_get_info(self._h, ctypes.byref(out)), "mi_get_info")
+        return out
+
+
+def float_ptr_or_none(a: Optional[np.ndarray]):
+    if a is None:
+        return None, None
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    return a.ctypes.data_as(ctypes.c_void_p), a

@@ -1,0 +1,802 @@
+// C ABI of the MI355X ray-tracing-matrix projector pair (include/stir_amd_pmrt.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "pmrt_device.hpp"
+#include "pmrt_plan.hpp"
+#include "stir_amd_pmrt.h"
+
+namespace mi {
+hipError_t radix_sort_pairs(void* d_temp, size_t& temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                            const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit, hipStream_t stream);
+hipError_t exclusive_scan_counts(void* d_temp, size_t& temp_bytes, const int32_t* in, int64_t* out, int n,
+                                 hipStream_t stream);
+}  // namespace mi
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct ApiError : std::runtime_error {
+  int code;
+  ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    const int code = (e == hipErrorOutOfMemory) ? MI_ERR_MEMORY : MI_ERR_DEVICE;
+    throw ApiError(code, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+#define HIPCHECK(x) hip_check((x), #x)
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return MI_OK;
+  } catch (const ApiError& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return MI_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return MI_ERR_ARGUMENT;
+  }
+}
+
+// Owning device buffer.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    HIPCHECK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    n = count;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+}  // namespace
+
+struct mi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<mi::Plan> plan;
+  mi::ImageDims dims{};
+  // matrix
+  DBuf<int2> pool;
+  DBuf<mi::Family> fams;
+  DBuf<mi::FpTask> tasks;
+  std::vector<int> task_off;  // per subset, size nsub+1
+  DBuf<int64_t> list_off;
+  DBuf<mi::BpEntry> entries;
+  std::vector<mi::Family> h_fams;
+  std::unordered_map<int64_t, int> fam_lookup;  // (seg,view,tang,a0) -> family
+  int64_t num_basic_elements = 0;
+  int64_t num_matrix_elements = 0;
+  // images / projection data (device layout) and staging (STIR layout)
+  DBuf<float> img, img2, sino, ratio, stage_img, stage_sino;
+  // objective function
+  DBuf<float> y, add, mult, ones, vg_small, x;
+  std::vector<std::unique_ptr<DBuf<float>>> sens;
+  DBuf<int64_t> vg_off;
+  DBuf<int32_t> vg_len;
+  DBuf<double> partial;
+  bool have_data = false, have_sens = false, have_x = false;
+  float add_const = 0.F;
+  int zero_end_planes = 0;
+
+  void require_setup() const {
+    if (!plan) throw ApiError(MI_ERR_STATE, "mi_set_up has not been called");
+  }
+  int subset_begin(int subset) const { return subset < 0 ? 0 : subset; }
+  int subset_end(int subset) const { return subset < 0 ? plan->num_subsets() : subset + 1; }
+  void check_subset(int subset) const {
+    if (subset >= plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
+  }
+  hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
+
+  // ---- layout conversions
+  void image_to_device(const float* src_stir, float* dst, hipStream_t st) const {
+    mi::launch_transpose(src_stir, dst, dims.nz, dims.ncols, dims.ncols, dims.nzp, st);
+  }
+  void image_from_device(const float* src, float* dst_stir, hipStream_t st) const {
+    mi::launch_transpose(src, dst_stir, dims.ncols, dims.nz, dims.nzp, dims.ncols, st);
+  }
+  void proj_to_device(const float* src_stir, float* dst, hipStream_t st) const {
+    const mi_geometry& g = plan->geometry();
+    const int cols = g.num_views * g.num_tangential_poss;
+    for (int seg : plan->segment_sequence()) {
+      const int64_t off = plan->segment_offset(seg);
+      const int nax = plan->num_axial(seg);
+      mi::launch_transpose(src_stir + off, dst + off, nax, cols, cols, nax, st);
+    }
+  }
+  void proj_from_device(const float* src, float* dst_stir, hipStream_t st) const {
+    const mi_geometry& g = plan->geometry();
+    const int cols = g.num_views * g.num_tangential_poss;
+    for (int seg : plan->segment_sequence()) {
+      const int64_t off = plan->segment_offset(seg);
+      const int nax = plan->num_axial(seg);
+      mi::launch_transpose(src + off, dst_stir + off, cols, nax, nax, cols, st);
+    }
+  }
+
+  // ---- projections on device-layout buffers
+  void fp(mi::FpMode mode, const float* d_img, float* d_out, int subset, const mi::FpData& data, hipStream_t st) const {
+    const int tb = task_off[subset_begin(subset)], te = task_off[subset_end(subset)];
+    mi::launch_forward_project(mode, tasks.p + tb, te - tb, fams.p, pool.p, d_img, d_out, dims, data, st);
+  }
+  void bp(mi::BpMode mode, const float* d_proj, float* d_img, int subset, const mi::BpData& data, hipStream_t st) const {
+    mi::launch_back_project(mode, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj, d_img, dims, data,
+                            st);
+  }
+
+  void set_up(const mi_geometry& g, int nsub);
+  void build_rows();
+  void build_tasks();
+  void build_table();
+  mi::FpData fp_data() const {
+    mi::FpData d{};
+    d.measured = y.p;
+    d.additive = add.p;
+    d.additive_constant = add_const;
+    d.vg_small = vg_small.p;
+    d.mult = mult.p;
+    d.partial = partial.p;
+    return d;
+  }
+};
+
+namespace {
+int64_t fam_key(int seg, int view, int tang, int a0) {
+  return (((static_cast<int64_t>(seg + 512) * 4096 + view) * 8192) + (tang + 4096)) * 8192 + a0;
+}
+}  // namespace
+
+void mi_ctx::build_rows() {
+  const auto& rays = plan->basic_rays();
+  const int nb = static_cast<int>(rays.size());
+  DBuf<mi::BasicRay> d_rays;
+  d_rays.alloc(nb);
+  HIPCHECK(hipMemcpyAsync(d_rays.p, rays.data(), nb * sizeof(mi::BasicRay), hipMemcpyHostToDevice, stream));
+  DBuf<int> d_len, d_buf, d_flag;
+  d_len.alloc(nb);
+  d_buf.alloc(nb);
+  d_flag.alloc(2);
+  std::vector<int> h_len(nb), h_buf(nb);
+  int cap = plan->row_length_bound();
+  // allowed in-plane coordinates of basic-row elements: with any in-plane
+  // symmetry the element's images (x,y negated or swapped) must stay inside
+  // the image, so the range is the largest centred square that fits
+  int4 lim;
+  {
+    const mi_geometry& g = plan->geometry();
+    int32_t fl[5];
+    plan->symmetry_flags(fl);
+    const int max_x = g.min_x + g.nx - 1, max_y = g.min_y + g.ny - 1;
+    if (fl[0] || fl[1] || fl[3]) {
+      const int m = std::min(std::min(max_x, -g.min_x), std::min(max_y, -g.min_y));
+      lim = make_int4(-m, m, -m, m);
+    } else {
+      lim = make_int4(g.min_x, max_x, g.min_y, max_y);
+    }
+    if (lim.x < -512 || lim.y > 511 || lim.z < -512 || lim.w > 511)
+      throw ApiError(MI_ERR_ARGUMENT, "image x/y index range exceeds the packed element range");
+  }
+  const size_t budget = size_t(2) << 30;  // scratch bytes per batch
+  struct Chunk {
+    int64_t global_start;
+    int64_t size;
+    std::unique_ptr<DBuf<int2>> data;
+  };
+  std::vector<Chunk> chunks;
+  std::vector<int64_t> row_start(nb);
+  int64_t total = 0;
+  DBuf<uint64_t> sk;
+  DBuf<float> sw;
+  DBuf<int64_t> d_start;
+  d_start.alloc(nb);
+  int first = 0;
+  while (first < nb) {
+    int batch = static_cast<int>(std::max<size_t>(64, budget / (size_t(3) * cap * (sizeof(uint64_t) + sizeof(float)))));
+    batch = std::min(batch, nb - first);
+    if (sk.n < size_t(3) * cap * batch) {
+      sk.alloc(size_t(3) * cap * batch);
+      sw.alloc(size_t(3) * cap * batch);
+    }
+    HIPCHECK(hipMemsetAsync(d_flag.p, 0, 2 * sizeof(int), stream));
+    mi::launch_trace_basic_rows(d_rays.p, first, batch, plan->trace_consts(), sk.p, sw.p, cap, d_len.p, d_buf.p, d_flag.p,
+                                stream);
+    HIPCHECK(hipGetLastError());
+    int flag[2];
+    HIPCHECK(hipMemcpyAsync(flag, d_flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(h_len.data() + first, d_len.p + first, batch * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (flag[0]) {  // a row exceeded the scratch bound: retry the batch with more room
+      if (cap > (1 << 22)) throw ApiError(MI_ERR_INTERNAL, "row length bound exceeded");
+      cap *= 4;
+      continue;
+    }
+    int64_t batch_total = 0;
+    for (int i = first; i < first + batch; ++i) {
+      row_start[i] = batch_total;
+      batch_total += h_len[i];
+    }
+    Chunk ch{total, batch_total, std::make_unique<DBuf<int2>>()};
+    ch.data->alloc(std::max<int64_t>(batch_total, 1));
+    HIPCHECK(hipMemcpyAsync(d_start.p + first, row_start.data() + first, batch * sizeof(int64_t), hipMemcpyHostToDevice,
+                            stream));
+    mi::launch_pack_rows(first, batch, sk.p, sw.p, cap, d_len.p, d_buf.p, d_start.p, ch.data->p, d_flag.p + 1, lim, stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(flag, d_flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (flag[1]) throw ApiError(MI_ERR_ARGUMENT, "matrix element coordinates exceed the packed range");
+    for (int i = first; i < first + batch; ++i) row_start[i] += total;
+    total += batch_total;
+    chunks.push_back(std::move(ch));
+    first += batch;
+  }
+  sk.release();
+  sw.release();
+  if (total >= (int64_t(1) << 31)) throw ApiError(MI_ERR_ARGUMENT, "more than 2^31 basic-row elements");
+  pool.alloc(std::max<int64_t>(total, 1));
+  for (auto& ch : chunks)
+    if (ch.size > 0)
+      HIPCHECK(hipMemcpyAsync(pool.p + ch.global_start, ch.data->p, ch.size * sizeof(int2), hipMemcpyDeviceToDevice, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  num_basic_elements = total;
+  // families
+  h_fams = plan->families();
+  const auto& keys = plan->family_keys();
+  num_matrix_elements = 0;
+  fam_lookup.clear();
+  fam_lookup.reserve(h_fams.size() * 2);
+  for (size_t i = 0; i < h_fams.size(); ++i) {
+    const int bi = keys[i].basic_index;
+    h_fams[i].row_start = static_cast<int32_t>(row_start[bi]);
+    h_fams[i].row_len = h_len[bi];
+    num_matrix_elements += static_cast<int64_t>(h_len[bi]) * h_fams[i].nax;
+    fam_lookup[fam_key(keys[i].seg, keys[i].view, keys[i].tang, keys[i].a0)] = static_cast<int>(i);
+  }
+  fams.alloc(h_fams.size());
+  HIPCHECK(hipMemcpyAsync(fams.p, h_fams.data(), h_fams.size() * sizeof(mi::Family), hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+}
+
+void mi_ctx::build_tasks() {
+  const int nsub = plan->num_subsets();
+  std::vector<mi::FpTask> h;
+  task_off.assign(nsub + 1, 0);
+  for (int s = 0; s < nsub; ++s) {
+    task_off[s] = static_cast<int>(h.size());
+    for (int f = plan->subset_family_begin(s); f < plan->subset_family_begin(s + 1); ++f)
+      for (int a0 = 0; a0 < h_fams[f].nax; a0 += 64) h.push_back({f, a0});
+  }
+  task_off[nsub] = static_cast<int>(h.size());
+  tasks.alloc(h.size());
+  HIPCHECK(hipMemcpyAsync(tasks.p, h.data(), h.size() * sizeof(mi::FpTask), hipMemcpyHostToDevice, stream));
+  partial.alloc(std::max<size_t>(h.size(), 1));
+  HIPCHECK(hipStreamSynchronize(stream));
+}
+
+void mi_ctx::build_table() {
+  const int nsub = plan->num_subsets();
+  const int ncols = dims.ncols;
+  // entry offsets per family (family order == subset order)
+  std::vector<int64_t> fam_off(h_fams.size() + 1, 0);
+  for (size_t i = 0; i < h_fams.size(); ++i) fam_off[i + 1] = fam_off[i] + h_fams[i].row_len;
+  const int64_t n = fam_off.back();
+  if (n >= (int64_t(1) << 31)) throw ApiError(MI_ERR_ARGUMENT, "back-projection table above 2^31 entries");
+  const int64_t nkeys = static_cast<int64_t>(nsub) * ncols;
+  DBuf<int32_t> counts;
+  counts.alloc(nkeys);
+  HIPCHECK(hipMemsetAsync(counts.p, 0, nkeys * sizeof(int32_t), stream));
+  DBuf<int64_t> d_famoff;
+  d_famoff.alloc(fam_off.size());
+  HIPCHECK(hipMemcpyAsync(d_famoff.p, fam_off.data(), fam_off.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+  DBuf<uint32_t> keys_in, keys_out, vals_in, vals_out;
+  DBuf<mi::BpEntry> raw;
+  keys_in.alloc(std::max<int64_t>(n, 1));
+  raw.alloc(std::max<int64_t>(n, 1));
+  for (int s = 0; s < nsub; ++s) {
+    const int fb = plan->subset_family_begin(s), fe = plan->subset_family_begin(s + 1);
+    mi::launch_count_entries(fams.p, fb, fe, pool.p, s, dims, counts.p, stream);
+    mi::launch_make_entries(fams.p, fb, fe, d_famoff.p + fb, pool.p, s, dims, keys_in.p, raw.p, stream);
+  }
+  HIPCHECK(hipGetLastError());
+  // list offsets = exclusive scan of counts, plus the total at the end
+  list_off.alloc(nkeys + 1);
+  {
+    size_t tb = 0;
+    HIPCHECK(mi::exclusive_scan_counts(nullptr, tb, counts.p, list_off.p, static_cast<int>(nkeys), stream));
+    DBuf<char> tmp;
+    tmp.alloc(std::max<size_t>(tb, 1));
+    HIPCHECK(mi::exclusive_scan_counts(tmp.p, tb, counts.p, list_off.p, static_cast<int>(nkeys), stream));
+    HIPCHECK(hipMemcpyAsync(list_off.p + nkeys, &n, sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  counts.release();
+  entries.alloc(std::max<int64_t>(n, 1));
+  if (n > 0) {
+    keys_out.alloc(n);
+    vals_in.alloc(n);
+    vals_out.alloc(n);
+    mi::launch_iota(vals_in.p, n, stream);
+    int end_bit = 1;
+    while ((int64_t(1) << end_bit) < nkeys) ++end_bit;
+    size_t tb = 0;
+    HIPCHECK(mi::radix_sort_pairs(nullptr, tb, keys_in.p, keys_out.p, vals_in.p, vals_out.p, static_cast<int>(n), end_bit,
+                                  stream));
+    DBuf<char> tmp;
+    tmp.alloc(std::max<size_t>(tb, 1));
+    HIPCHECK(mi::radix_sort_pairs(tmp.p, tb, keys_in.p, keys_out.p, vals_in.p, vals_out.p, static_cast<int>(n), end_bit,
+                                  stream));
+    mi::launch_gather_entries(raw.p, vals_out.p, entries.p, n, stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+}
+
+void mi_ctx::set_up(const mi_geometry& g, int nsub) {
+  HIPCHECK(hipSetDevice(device));
+  // release everything of a previous set-up
+  pool.release(); fams.release(); tasks.release(); list_off.release(); entries.release();
+  img.release(); img2.release(); sino.release(); ratio.release(); stage_img.release(); stage_sino.release();
+  y.release(); add.release(); mult.release(); ones.release(); vg_small.release(); x.release(); sens.clear();
+  vg_off.release(); vg_len.release(); partial.release();
+  have_data = have_sens = have_x = false;
+  plan.reset();
+  plan = std::make_unique<mi::Plan>(g, nsub);
+  dims.nz = g.nz; dims.ny = g.ny; dims.nx = g.nx;
+  dims.min_z = g.min_z; dims.min_y = g.min_y; dims.min_x = g.min_x;
+  dims.nzp = plan->nz_padded();
+  dims.ncols = plan->num_columns();
+  build_rows();
+  build_tasks();
+  build_table();
+  const size_t nvox_dev = static_cast<size_t>(dims.ncols) * dims.nzp;
+  img.alloc(nvox_dev);
+  img2.alloc(nvox_dev);
+  HIPCHECK(hipMemsetAsync(img.p, 0, img.bytes(), stream));
+  HIPCHECK(hipMemsetAsync(img2.p, 0, img2.bytes(), stream));
+  sino.alloc(plan->num_bins());
+  ratio.alloc(plan->num_bins());
+  stage_img.alloc(plan->num_voxels());
+  stage_sino.alloc(plan->num_bins());
+  // viewgram extents in the device layout
+  {
+    const int nvg = plan->num_viewgrams();
+    std::vector<int64_t> off(nvg, 0);
+    std::vector<int32_t> len(nvg, 0);
+    for (int seg = g.min_segment_num; seg <= g.max_segment_num; ++seg)
+      for (int v = 0; v < g.num_views; ++v) {
+        const int i = plan->viewgram_index(seg, v);
+        off[i] = plan->device_bin_offset(seg, v, 0, g.min_tangential_pos_num);
+        len[i] = g.num_tangential_poss * plan->num_axial(seg);
+      }
+    vg_off.alloc(nvg);
+    vg_len.alloc(nvg);
+    HIPCHECK(hipMemcpyAsync(vg_off.p, off.data(), nvg * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(vg_len.p, len.data(), nvg * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  }
+  HIPCHECK(hipStreamSynchronize(stream));
+}
+
+// ===========================================================================
+extern "C" {
+
+const char* mi_last_error(void) { return g_last_error.c_str(); }
+
+int mi_ctx_create(int device, mi_ctx** out) {
+  return guarded([&] {
+    if (!out) throw ApiError(MI_ERR_ARGUMENT, "out is NULL");
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw ApiError(MI_ERR_ARGUMENT, "no such HIP device");
+    auto ctx = std::make_unique<mi_ctx>();
+    ctx->device = device;
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    *out = ctx.release();
+  });
+}
+
+void mi_ctx_destroy(mi_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  hipStream_t s = ctx->stream;
+  delete ctx;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+int mi_set_up(mi_ctx* ctx, const mi_geometry* geometry, int num_subsets) {
+  return guarded([&] {
+    if (!ctx || !geometry) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->set_up(*geometry, num_subsets);
+  });
+}
+
+int mi_get_info(mi_ctx* ctx, mi_info* out) {
+  return guarded([&] {
+    if (!ctx || !out) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    std::memset(out, 0, sizeof(*out));
+    out->num_bins = ctx->plan->num_bins();
+    out->num_voxels = ctx->plan->num_voxels();
+    out->num_basic_rows = static_cast<int64_t>(ctx->plan->basic_rays().size());
+    out->num_basic_elements = ctx->num_basic_elements;
+    out->num_families = static_cast<int64_t>(ctx->h_fams.size());
+    out->num_matrix_elements = ctx->num_matrix_elements;
+    out->num_table_entries = static_cast<int64_t>(ctx->entries.n);
+    out->device_bytes = static_cast<int64_t>(ctx->pool.bytes() + ctx->fams.bytes() + ctx->tasks.bytes() +
+                                             ctx->list_off.bytes() + ctx->entries.bytes() + ctx->img.bytes() +
+                                             ctx->img2.bytes() + ctx->sino.bytes() + ctx->ratio.bytes() +
+                                             ctx->stage_img.bytes() + ctx->stage_sino.bytes() + ctx->y.bytes() +
+                                             ctx->add.bytes() + ctx->mult.bytes() + ctx->ones.bytes() + ctx->x.bytes());
+    for (auto& s : ctx->sens) out->device_bytes += static_cast<int64_t>(s->bytes());
+    out->num_subsets = ctx->plan->num_subsets();
+    ctx->plan->symmetry_flags(out->symmetry_flags);
+    out->nz_padded = ctx->dims.nzp;
+  });
+}
+
+int mi_subsets_balanced(mi_ctx* ctx, int* balanced) {
+  return guarded([&] {
+    if (!ctx || !balanced) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    *balanced = ctx->plan->subsets_balanced() ? 1 : 0;
+  });
+}
+
+int mi_subset_view_segments(mi_ctx* ctx, int subset, int32_t* views, int32_t* segments, int cap, int* count) {
+  return guarded([&] {
+    if (!ctx || !count) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    if (subset < 0 || subset >= ctx->plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
+    const auto vs = ctx->plan->subset_view_segments(subset);
+    *count = static_cast<int>(vs.size());
+    for (int i = 0; i < cap && i < static_cast<int>(vs.size()); ++i) {
+      if (views) views[i] = vs[i].first;
+      if (segments) segments[i] = vs[i].second;
+    }
+  });
+}
+
+int mi_get_row(mi_ctx* ctx, int segment, int view, int axial_pos, int tang_pos, int16_t* z, int16_t* y, int16_t* x,
+               float* w, int cap, int* n) {
+  return guarded([&] {
+    if (!ctx || !n) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    const mi_geometry& g = ctx->plan->geometry();
+    if (segment < g.min_segment_num || segment > g.max_segment_num || view < 0 || view >= g.num_views ||
+        tang_pos < g.min_tangential_pos_num || tang_pos > ctx->plan->max_tang() || axial_pos < 0 ||
+        axial_pos >= ctx->plan->num_axial(segment))
+      throw ApiError(MI_ERR_ARGUMENT, "bin out of range");
+    int32_t flags[5];
+    ctx->plan->symmetry_flags(flags);
+    const int a0 = flags[4] ? 0 : axial_pos;
+    auto it = ctx->fam_lookup.find(fam_key(segment, view, tang_pos, a0));
+    if (it == ctx->fam_lookup.end()) throw ApiError(MI_ERR_INTERNAL, "family not found");
+    const mi::Family& f = ctx->h_fams[it->second];
+    *n = f.row_len;
+    if (f.row_len == 0) return;
+    std::vector<int2> row(f.row_len);
+    HIPCHECK(hipMemcpyAsync(row.data(), ctx->pool.p + f.row_start, f.row_len * sizeof(int2), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    const int a = axial_pos - a0;
+    for (int i = 0; i < f.row_len && i < cap; ++i) {
+      const int c = row[i].x;
+      const int bx = (c & 1023) - 512, by = ((c >> 10) & 1023) - 512;
+      const int bz = static_cast<int>(static_cast<unsigned>(c) >> 20) - 2048;
+      int xo = bx, yo = by;
+      switch (f.inplane) {
+        case mi::IP_ID: break;
+        case mi::IP_XMX: xo = -bx; break;
+        case mi::IP_YMY: yo = -by; break;
+        case mi::IP_XMX_YMY: xo = -bx; yo = -by; break;
+        case mi::IP_XY_YX: xo = by; yo = bx; break;
+        case mi::IP_XMY_YX: xo = -by; yo = bx; break;
+        case mi::IP_XY_YMX: xo = by; yo = -bx; break;
+        default: xo = -by; yo = -bx; break;
+      }
+      if (z) z[i] = static_cast<int16_t>(f.zsign * bz + f.zadd0 + f.ppa * a);
+      if (y) y[i] = static_cast<int16_t>(yo);
+      if (x) x[i] = static_cast<int16_t>(xo);
+      float wv;
+      std::memcpy(&wv, &row[i].y, sizeof(float));
+      if (w) w[i] = wv;
+    }
+  });
+}
+
+int mi_forward_project_dev(mi_ctx* ctx, const float* d_image, float* d_proj_data, int subset, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_image || !d_proj_data) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    ctx->check_subset(subset);
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->pick(stream);
+    ctx->image_to_device(d_image, ctx->img.p, st);
+    if (subset >= 0) HIPCHECK(hipMemsetAsync(ctx->sino.p, 0, ctx->sino.bytes(), st));
+    ctx->fp(mi::FP_PLAIN, ctx->img.p, ctx->sino.p, subset, mi::FpData{}, st);
+    ctx->proj_from_device(ctx->sino.p, d_proj_data, st);
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_back_project_dev(mi_ctx* ctx, const float* d_proj_data, float* d_image, int subset, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_image || !d_proj_data) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    ctx->check_subset(subset);
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->pick(stream);
+    ctx->proj_to_device(d_proj_data, ctx->sino.p, st);
+    ctx->bp(mi::BP_PLAIN, ctx->sino.p, ctx->img.p, subset, mi::BpData{}, st);
+    ctx->image_from_device(ctx->img.p, d_image, st);
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_forward_project(mi_ctx* ctx, const float* image, float* proj_data, int subset) {
+  return guarded([&] {
+    if (!ctx || !image || !proj_data) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    ctx->check_subset(subset);
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(hipMemcpyAsync(ctx->stage_img.p, image, ctx->stage_img.bytes(), hipMemcpyHostToDevice, ctx->stream));
+    if (mi_forward_project_dev(ctx, ctx->stage_img.p, ctx->stage_sino.p, subset, nullptr) != MI_OK)
+      throw ApiError(MI_ERR_DEVICE, g_last_error);
+    HIPCHECK(hipMemcpyAsync(proj_data, ctx->stage_sino.p, ctx->stage_sino.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mi_back_project(mi_ctx* ctx, const float* proj_data, float* image, int subset) {
+  return guarded([&] {
+    if (!ctx || !image || !proj_data) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    ctx->check_subset(subset);
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(hipMemcpyAsync(ctx->stage_sino.p, proj_data, ctx->stage_sino.bytes(), hipMemcpyHostToDevice, ctx->stream));
+    if (mi_back_project_dev(ctx, ctx->stage_sino.p, ctx->stage_img.p, subset, nullptr) != MI_OK)
+      throw ApiError(MI_ERR_DEVICE, g_last_error);
+    HIPCHECK(hipMemcpyAsync(image, ctx->stage_img.p, ctx->stage_img.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mi_obj_set_data(mi_ctx* ctx, const float* measured, const float* additive, float additive_constant,
+                    int zero_end_planes) {
+  return guarded([&] {
+    if (!ctx || !measured) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    const mi_geometry& g = ctx->plan->geometry();
+    hipStream_t st = ctx->stream;
+    const int64_t nb = ctx->plan->num_bins();
+    ctx->y.alloc(nb);
+    HIPCHECK(hipMemcpyAsync(ctx->stage_sino.p, measured, nb * sizeof(float), hipMemcpyHostToDevice, st));
+    ctx->proj_to_device(ctx->stage_sino.p, ctx->y.p, st);
+    if (additive) {
+      ctx->add.alloc(nb);
+      HIPCHECK(hipMemcpyAsync(ctx->stage_sino.p, additive, nb * sizeof(float), hipMemcpyHostToDevice, st));
+      ctx->proj_to_device(ctx->stage_sino.p, ctx->add.p, st);
+    } else {
+      ctx->add.release();
+    }
+    ctx->add_const = additive_constant;
+    ctx->zero_end_planes = zero_end_planes;
+    ctx->ones.alloc(nb);
+    mi::launch_fill(ctx->ones.p, nb, 1.F, st);
+    if (zero_end_planes) {
+      // "zero end planes of segment 0" (distributable.cxx:218-223): y, b and the
+      // multiplicative viewgrams get their first and last sinogram zeroed
+      DBuf<int64_t> off;
+      off.alloc(1);
+      const int64_t o = ctx->plan->segment_offset(0);
+      HIPCHECK(hipMemcpyAsync(off.p, &o, sizeof(int64_t), hipMemcpyHostToDevice, st));
+      const int runs = g.num_views * g.num_tangential_poss, run = ctx->plan->num_axial(0);
+      mi::launch_zero_end_planes(ctx->y.p, off.p, runs, run, 0, st);
+      if (ctx->add.p) mi::launch_zero_end_planes(ctx->add.p, off.p, runs, run, 0, st);
+      ctx->mult.alloc(nb);
+      mi::launch_fill(ctx->mult.p, nb, 1.F, st);
+      mi::launch_zero_end_planes(ctx->mult.p, off.p, runs, run, 0, st);
+      HIPCHECK(hipStreamSynchronize(st));
+    } else {
+      ctx->mult.release();
+    }
+    ctx->vg_small.alloc(ctx->plan->num_viewgrams());
+    mi::launch_viewgram_max(ctx->y.p, ctx->vg_off.p, ctx->vg_len.p, ctx->plan->num_viewgrams(), ctx->vg_small.p, st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    ctx->have_data = true;
+    ctx->have_sens = false;
+  });
+}
+
+int mi_obj_compute_sensitivities(mi_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    if (!ctx->have_data) throw ApiError(MI_ERR_STATE, "mi_obj_set_data has not been called");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const int nsub = ctx->plan->num_subsets();
+    ctx->sens.clear();
+    const float* in = ctx->mult.p ? ctx->mult.p : ctx->ones.p;
+    for (int s = 0; s < nsub; ++s) {
+      auto b = std::make_unique<DBuf<float>>();
+      b->alloc(ctx->img.n);
+      ctx->bp(mi::BP_PLAIN, in, b->p, s, mi::BpData{}, ctx->stream);
+      ctx->sens.push_back(std::move(b));
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->have_sens = true;
+  });
+}
+
+int mi_obj_get_sensitivity(mi_ctx* ctx, int subset, float* image) {
+  return guarded([&] {
+    if (!ctx || !image) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    if (!ctx->have_sens) throw ApiError(MI_ERR_STATE, "sensitivities have not been computed");
+    if (subset < 0 || subset >= ctx->plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->image_from_device(ctx->sens[subset]->p, ctx->stage_img.p, ctx->stream);
+    HIPCHECK(hipMemcpyAsync(image, ctx->stage_img.p, ctx->stage_img.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mi_obj_set_estimate(mi_ctx* ctx, const float* image, int enforce_positivity) {
+  return guarded([&] {
+    if (!ctx || !image) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    const int64_t nv = ctx->plan->num_voxels();
+    std::vector<float> h(image, image + nv);
+    if (enforce_positivity) {
+      // threshold_min_to_small_positive_value(x, 1e-6) (thresholding.h:101-109)
+      const float small_number = 0.000001F;
+      bool found = false;
+      float minpos = 0;
+      for (float v : h)
+        if (v > 0 && (!found || v < minpos)) { minpos = v; found = true; }
+      if (found) {
+        const float thr = minpos * small_number;
+        for (float& v : h)
+          if (thr > v) v = thr;
+      } else {
+        std::fill(h.begin(), h.end(), small_number);
+      }
+    }
+    ctx->x.alloc(ctx->img.n);
+    HIPCHECK(hipMemsetAsync(ctx->x.p, 0, ctx->x.bytes(), ctx->stream));
+    HIPCHECK(hipMemcpyAsync(ctx->stage_img.p, h.data(), nv * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    ctx->image_to_device(ctx->stage_img.p, ctx->x.p, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->have_x = true;
+  });
+}
+
+int mi_obj_get_estimate(mi_ctx* ctx, float* image) {
+  return guarded([&] {
+    if (!ctx || !image) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    if (!ctx->have_x) throw ApiError(MI_ERR_STATE, "no estimate");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->image_from_device(ctx->x.p, ctx->stage_img.p, ctx->stream);
+    HIPCHECK(hipMemcpyAsync(image, ctx->stage_img.p, ctx->stage_img.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+static void osem_step(mi_ctx* ctx, int subset, int subiteration_num) {
+  ctx->fp(mi::FP_RATIO, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
+  mi::BpData bd{};
+  bd.sens = ctx->sens[subset]->p;
+  bd.estimate = ctx->x.p;
+  bd.threshold = subiteration_num != 1;
+  ctx->bp(mi::BP_OSEM_UPDATE, ctx->ratio.p, nullptr, subset, bd, ctx->stream);
+}
+
+int mi_osem_subset_step(mi_ctx* ctx, int subset, int subiteration_num) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    if (!ctx->have_data || !ctx->have_sens || !ctx->have_x)
+      throw ApiError(MI_ERR_STATE, "data, sensitivities and estimate must be set first");
+    if (subset < 0 || subset >= ctx->plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
+    HIPCHECK(hipSetDevice(ctx->device));
+    osem_step(ctx, subset, subiteration_num);
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_osem_run(mi_ctx* ctx, int start_subiteration_num, int num_subiterations, int start_subset) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    if (!ctx->have_data || !ctx->have_sens || !ctx->have_x)
+      throw ApiError(MI_ERR_STATE, "data, sensitivities and estimate must be set first");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const int nsub = ctx->plan->num_subsets();
+    for (int k = start_subiteration_num; k < start_subiteration_num + num_subiterations; ++k) {
+      const int subset = (k + start_subset - 1) % nsub;  // IterativeReconstruction.cxx:630-639
+      osem_step(ctx, subset, k);
+    }
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_obj_subset_gradient(mi_ctx* ctx, int subset, int add_sensitivity, float* gradient) {
+  return guarded([&] {
+    if (!ctx || !gradient) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    ctx->check_subset(subset);
+    if (!ctx->have_data || !ctx->have_x) throw ApiError(MI_ERR_STATE, "data and estimate must be set first");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->fp(add_sensitivity ? mi::FP_RATIO : mi::FP_GRADIENT, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
+    ctx->bp(mi::BP_PLAIN, ctx->ratio.p, ctx->img2.p, subset, mi::BpData{}, ctx->stream);
+    ctx->image_from_device(ctx->img2.p, ctx->stage_img.p, ctx->stream);
+    HIPCHECK(hipMemcpyAsync(gradient, ctx->stage_img.p, ctx->stage_img.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mi_obj_log_likelihood(mi_ctx* ctx, int subset, double* value) {
+  return guarded([&] {
+    if (!ctx || !value) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    ctx->check_subset(subset);
+    if (!ctx->have_data || !ctx->have_x) throw ApiError(MI_ERR_STATE, "data and estimate must be set first");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const int tb = ctx->task_off[ctx->subset_begin(subset)], te = ctx->task_off[ctx->subset_end(subset)];
+    ctx->fp(mi::FP_LOGLIK, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
+    std::vector<double> h(te - tb);
+    HIPCHECK(hipMemcpyAsync(h.data(), ctx->partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    double acc = 0;
+    for (double v : h) acc += v;
+    *value = acc;
+  });
+}
+
+int mi_synchronize(mi_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"

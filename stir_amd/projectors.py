@@ -1,0 +1,300 @@
+"""Projector pair and objective function on the MI355X, with STIR's interface.
+
+Mirrors the reference's projector plugin surface
+(``ForwardProjectorByBin.h:53-185``, ``BackProjectorByBin.h:50-210``,
+``ProjectorByBinPairUsingProjMatrixByBin.cxx:31-107``) and the objective
+function / OSMAPOSL entry points used on this path
+(``PoissonLogLikelihoodWithLinearModelForMeanAndProjData``,
+``OSMAPOSLReconstruction``). All computation happens in the native library
+(``libstir_amd_pmrt.so``) on the GPU; arrays cross as numpy (host) or torch
+CUDA tensors (device).
+
+Layouts are STIR's: images ``[z][y][x]`` float32 and projection data flat in
+``ProjDataInMemory`` order (segments 0,+1,-1,..., then axial, view, tangential).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import _native
+from .geometry import (ProjDataInfoCylindrical, RayTracingMatrixParameters, VoxelsOnCartesianGrid,
+                       make_geometry_struct)
+
+
+class ProjMatrixByBinUsingRayTracing:
+    """Parameters of the ray-tracing matrix (registered name "Ray Tracing"
+    in ``ProjMatrixByBinUsingRayTracing.cxx:67``). The matrix itself is
+    built on the device by :class:`ProjectorByBinPairMI355X.set_up`."""
+
+    registered_name = "Ray Tracing"
+
+    def __init__(self, parameters: RayTracingMatrixParameters = RayTracingMatrixParameters(), **kw):
+        if kw:
+            from dataclasses import replace
+            parameters = replace(parameters, **kw)
+        self.parameters = parameters
+
+    def set_num_tangential_LORs(self, n: int) -> None:
+        from dataclasses import replace
+        self.parameters = replace(self.parameters, num_tangential_LORs=n)
+
+    def get_num_tangential_LORs(self) -> int:
+        return self.parameters.num_tangential_LORs
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+class ProjectorByBinPairMI355X:
+    """Forward/back projector pair sharing one device-resident matrix.
+
+    ``set_up(proj_data_info, image_info, num_subsets)`` replaces
+    ``ProjectorByBinPair::set_up`` (``ProjectorByBinPair.cxx:36-47``): it traces
+    the basic rows on the device and builds the back-projection table for the
+    given number of OSEM subsets (1 for plain projections)."""
+
+    registered_name = "MI355X Ray Tracing"
+
+    def __init__(self, proj_matrix: Optional[ProjMatrixByBinUsingRayTracing] = None, device: int = 0):
+        self.proj_matrix = proj_matrix or ProjMatrixByBinUsingRayTracing()
+        self._ctx = _native.Context(device)
+        self.pdi: Optional[ProjDataInfoCylindrical] = None
+        self.img: Optional[VoxelsOnCartesianGrid] = None
+        self.num_subsets = 0
+
+    # ------------------------------------------------------------------
+    def set_up(self, proj_data_info: ProjDataInfoCylindrical, image_info: VoxelsOnCartesianGrid,
+               num_subsets: int = 1) -> None:
+        g = make_geometry_struct(proj_data_info, image_info, self.proj_matrix.parameters)
+        self._geometry = g
+        _native.check(_native.load().mi_set_up(self._ctx.handle, ctypes.byref(g), int(num_subsets)), "mi_set_up")
+        self.pdi, self.img, self.num_subsets = proj_data_info, image_info, num_subsets
+
+    @property
+    def context(self) -> _native.Context:
+        return self._ctx
+
+    def info(self) -> _native.InfoStruct:
+        return self._ctx.info()
+
+    def get_symmetry_flags(self) -> Tuple[int, ...]:
+        return tuple(self.info().symmetry_flags)
+
+    def subsets_are_approximately_balanced(self) -> bool:
+        b = ctypes.c_int()
+        _native.check(_native.load().mi_subsets_balanced(self._ctx.handle, ctypes.byref(b)), "mi_subsets_balanced")
+        return bool(b.value)
+
+    def subset_view_segments(self, subset: int) -> List[Tuple[int, int]]:
+        cap = 4 * self.pdi.num_views * len(self.pdi.num_axial_poss) + 16
+        v = np.zeros(cap, np.int32)
+        s = np.zeros(cap, np.int32)
+        n = ctypes.c_int()
+        _native.check(_native.load().mi_subset_view_segments(self._ctx.handle, subset, v, s, cap, ctypes.byref(n)),
+                      "mi_subset_view_segments")
+        return list(zip(s[:n.value].tolist(), v[:n.value].tolist()))
+
+    def get_row(self, segment: int, view: int, axial_pos: int, tang_pos: int):
+        """Row of one bin as the device uses it: (z, y, x, w) arrays."""
+        cap = 1 << 14
+        while True:
+            z = np.empty(cap, np.int16); y = np.empty(cap, np.int16); x = np.empty(cap, np.int16)
+            w = np.empty(cap, np.float32)
+            n = ctypes.c_int()
+            _native.check(_native.load().mi_get_row(self._ctx.handle, segment, view, axial_pos, tang_pos, z, y, x, w, cap,
+                                                    ctypes.byref(n)), "mi_get_row")
+            if n.value <= cap:
+                m = n.value
+                return z[:m].copy(), y[:m].copy(), x[:m].copy(), w[:m].copy()
+            cap = n.value
+
+    # ------------------------------------------------------------------
+    def _check_setup(self):
+        if self.pdi is None:
+            raise RuntimeError("set_up has not been called")
+
+    def forward_project(self, image, subset: int = -1, out=None, stream=None):
+        """A x. numpy in -> numpy out (host path); torch CUDA tensor in ->
+        torch CUDA tensor out (device path, no host copies). For subset >= 0
+        bins outside the subset are zero."""
+        self._check_setup()
+        if _is_torch(image):
+            import torch
+            if not image.is_cuda:
+                raise ValueError("torch tensors must be on the GPU")
+            image = image.contiguous()
+            if image.dtype != torch.float32 or image.numel() != self.img.num_voxels:
+                raise ValueError("image must be float32 with the image's number of voxels")
+            if out is None:
+                out = torch.empty(self.pdi.num_bins, dtype=torch.float32, device=image.device)
+            st = stream if stream is not None else torch.cuda.current_stream(image.device).cuda_stream
+            _native.check(_native.load().mi_forward_project_dev(self._ctx.handle, ctypes.c_void_p(image.data_ptr()),
+                                                                ctypes.c_void_p(out.data_ptr()), subset,
+                                                                ctypes.c_void_p(st)), "mi_forward_project_dev")
+            return out
+        image = np.ascontiguousarray(image, dtype=np.float32).reshape(-1)
+        if image.size != self.img.num_voxels:
+            raise ValueError("image has the wrong number of voxels")
+        if out is None:
+            out = np.empty(self.pdi.num_bins, np.float32)
+        _native.check(_native.load().mi_forward_project(self._ctx.handle, image, out, subset), "mi_forward_project")
+        return out
+
+    def back_project(self, proj_data, subset: int = -1, out=None, stream=None):
+        """A^T p (overwrites the output image)."""
+        self._check_setup()
+        if _is_torch(proj_data):
+            import torch
+            if not proj_data.is_cuda:
+                raise ValueError("torch tensors must be on the GPU")
+            proj_data = proj_data.contiguous()
+            if proj_data.dtype != torch.float32 or proj_data.numel() != self.pdi.num_bins:
+                raise ValueError("projection data must be float32 with the data's number of bins")
+            if out is None:
+                out = torch.empty(self.img.shape, dtype=torch.float32, device=proj_data.device)
+            st = stream if stream is not None else torch.cuda.current_stream(proj_data.device).cuda_stream
+            _native.check(_native.load().mi_back_project_dev(self._ctx.handle, ctypes.c_void_p(proj_data.data_ptr()),
+                                                             ctypes.c_void_p(out.data_ptr()), subset,
+                                                             ctypes.c_void_p(st)), "mi_back_project_dev")
+            return out
+        proj_data = np.ascontiguousarray(proj_data, dtype=np.float32).reshape(-1)
+        if proj_data.size != self.pdi.num_bins:
+            raise ValueError("projection data has the wrong number of bins")
+        if out is None:
+            out = np.empty(self.img.shape, np.float32)
+        _native.check(_native.load().mi_back_project(self._ctx.handle, proj_data, out.reshape(-1), subset),
+                      "mi_back_project")
+        return out
+
+    def synchronize(self):
+        _native.check(_native.load().mi_synchronize(self._ctx.handle), "mi_synchronize")
+
+    def close(self):
+        self._ctx.close()
+
+
+class ForwardProjectorByBinMI355X:
+    """``ForwardProjectorByBin`` view of a :class:`ProjectorByBinPairMI355X`."""
+
+    registered_name = "MI355X Ray Tracing"
+
+    def __init__(self, pair: ProjectorByBinPairMI355X):
+        self.pair = pair
+        self._input = None
+
+    def set_input(self, image):
+        self._input = image
+
+    def forward_project(self, proj_data=None, image=None, subset: int = -1):
+        img = image if image is not None else self._input
+        return self.pair.forward_project(img, subset=subset, out=proj_data)
+
+
+class BackProjectorByBinMI355X:
+    """``BackProjectorByBin`` view of a :class:`ProjectorByBinPairMI355X`:
+    ``start_accumulating_in_new_target`` / ``back_project`` / ``get_output``."""
+
+    registered_name = "MI355X Ray Tracing"
+
+    def __init__(self, pair: ProjectorByBinPairMI355X):
+        self.pair = pair
+        self._acc = None
+
+    def start_accumulating_in_new_target(self):
+        self._acc = None
+
+    def back_project(self, proj_data, subset: int = -1):
+        img = self.pair.back_project(proj_data, subset=subset)
+        self._acc = img if self._acc is None else self._acc + img
+
+    def get_output(self):
+        if self._acc is None:
+            return np.zeros(self.pair.img.shape, np.float32)
+        return self._acc
+
+
+class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
+    """Prior-free Poisson log-likelihood with data resident on the device.
+
+    ``set_up`` takes the measured data y, an additive term b (array or
+    constant) and "zero end planes of segment 0"; sensitivities are
+    per-subset (``use_subset_sensitivities`` defaults to true in the reference,
+    ``PoissonLogLikelihoodWithLinearModelForMean.cxx:48``)."""
+
+    def __init__(self, projector: ProjectorByBinPairMI355X):
+        self.projector = projector
+        self._ctx = projector.context
+
+    def set_up(self, measured: np.ndarray, additive=0.0, zero_end_planes: bool = False,
+               compute_sensitivities: bool = True) -> None:
+        L = _native.load()
+        y = np.ascontiguousarray(measured, dtype=np.float32).reshape(-1)
+        if isinstance(additive, np.ndarray):
+            ap, keep = _native.float_ptr_or_none(additive)
+            self._keep = keep
+            _native.check(L.mi_obj_set_data(self._ctx.handle, y, ap, ctypes.c_float(0.0), int(zero_end_planes)),
+                          "mi_obj_set_data")
+        else:
+            _native.check(L.mi_obj_set_data(self._ctx.handle, y, None, ctypes.c_float(float(additive)),
+                                            int(zero_end_planes)), "mi_obj_set_data")
+        if compute_sensitivities:
+            _native.check(L.mi_obj_compute_sensitivities(self._ctx.handle), "mi_obj_compute_sensitivities")
+
+    def get_subset_sensitivity(self, subset: int) -> np.ndarray:
+        out = np.empty(self.projector.img.shape, np.float32)
+        _native.check(_native.load().mi_obj_get_sensitivity(self._ctx.handle, subset, out.reshape(-1)),
+                      "mi_obj_get_sensitivity")
+        return out
+
+    def set_estimate(self, image: np.ndarray, enforce_positivity: bool = False) -> None:
+        x = np.ascontiguousarray(image, dtype=np.float32).reshape(-1)
+        _native.check(_native.load().mi_obj_set_estimate(self._ctx.handle, x, int(enforce_positivity)),
+                      "mi_obj_set_estimate")
+
+    def get_estimate(self) -> np.ndarray:
+        out = np.empty(self.projector.img.shape, np.float32)
+        _native.check(_native.load().mi_obj_get_estimate(self._ctx.handle, out.reshape(-1)), "mi_obj_get_estimate")
+        return out
+
+    def compute_sub_gradient_without_penalty(self, current_estimate: np.ndarray, subset: int = -1,
+                                             plus_sensitivity: bool = False) -> np.ndarray:
+        self.set_estimate(current_estimate)
+        out = np.empty(self.projector.img.shape, np.float32)
+        _native.check(_native.load().mi_obj_subset_gradient(self._ctx.handle, subset, int(plus_sensitivity),
+                                                            out.reshape(-1)), "mi_obj_subset_gradient")
+        return out
+
+    def compute_objective_function(self, current_estimate: np.ndarray, subset: int = -1) -> float:
+        self.set_estimate(current_estimate)
+        v = ctypes.c_double()
+        _native.check(_native.load().mi_obj_log_likelihood(self._ctx.handle, subset, ctypes.byref(v)),
+                      "mi_obj_log_likelihood")
+        return float(v.value)
+
+
+class OSMAPOSLReconstruction:
+    """Prior-free OSMAPOSL (``OSMAPOSLReconstruction.cxx:266-516``) on the device."""
+
+    def __init__(self, objective: PoissonLogLikelihoodWithLinearModelForMeanAndProjData,
+                 num_subiterations: int, start_subset: int = 0, enforce_initial_positivity: bool = True):
+        self.objective = objective
+        self.num_subiterations = num_subiterations
+        self.start_subset = start_subset
+        self.enforce_initial_positivity = enforce_initial_positivity
+
+    def reconstruct(self, initial_image: np.ndarray) -> np.ndarray:
+        proj = self.objective.projector
+        if not proj.subsets_are_approximately_balanced():
+            raise RuntimeError("OSMAPOSL cannot handle unbalanced subsets")
+        self.objective.set_estimate(initial_image, enforce_positivity=self.enforce_initial_positivity)
+        _native.check(_native.load().mi_osem_run(self.objective._ctx.handle, 1, self.num_subiterations,
+                                                 self.start_subset), "mi_osem_run")
+        return self.objective.get_estimate()
+
+    def update_estimate(self, subset: int, subiteration_num: int) -> None:
+        _native.check(_native.load().mi_osem_subset_step(self.objective._ctx.handle, subset, subiteration_num),
+                      "mi_osem_subset_step")
