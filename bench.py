@@ -1,0 +1,224 @@
+"""Benchmark of the MI355X ray-tracing-matrix projector pair at mMR size.
+
+A step is one forward projection of the whole image into all 72.6M bins plus
+one back projection of all bins (a projector pair), with image and
+projection data resident in HBM (STIR layouts, torch device tensors) and the
+matrix built beforehand. Prints one JSON line.
+
+Also measured (extra fields): OSEM iterations/s (21 subsets), per-kernel
+times from HIP events on the kernels' stream, the HBM roofline of the
+dominant kernel, and the CPU oracle (OpenMP) on a bounded sample of the same
+workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "mMR-size fwd+back proj pairs/s, OSEM iter/s, autograd grad/s; HBM GB/s vs peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--osem-iterations", type=int, default=1)
+    ap.add_argument("--skip-osem", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-subsets", type=int, default=1, help="subsets of 21 in the CPU baseline sample")
+    ap.add_argument("--radius", type=float, default=328.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "detail": "multi-GPU sharding is described in DESIGN.md but not launched"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from stir_amd import phantoms
+    from stir_amd.geometry import RayTracingMatrixParameters, config1
+    from stir_amd.projectors import (OSMAPOSLReconstruction, PoissonLogLikelihoodWithLinearModelForMeanAndProjData,
+                                     ProjectorByBinPairMI355X, ProjMatrixByBinUsingRayTracing)
+
+    dev = torch.device("cuda:0")
+    pdi, img = config1(args.radius)
+    pm = RayTracingMatrixParameters()
+    nsub = 21
+    log(f"set-up: {pdi.num_bins} bins, image {img.shape}, {nsub} subsets")
+    t0 = time.time()
+    pair = ProjectorByBinPairMI355X(ProjMatrixByBinUsingRayTracing(pm))
+    pair.set_up(pdi, img, num_subsets=nsub)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    info = pair.info()
+    log(f"set-up {setup_s:.2f}s basic rows {info.num_basic_rows} elements {info.num_basic_elements} "
+        f"nnz {info.num_nonzeros} table {info.num_table_entries} device {info.device_bytes / 2**30:.2f} GiB")
+
+    # synthetic, seeded inputs resident in HBM
+    x_true = phantoms.config1_phantom(img)
+    x = torch.from_numpy(x_true).to(dev)
+    sino = torch.empty(pdi.num_bins, dtype=torch.float32, device=dev)
+    back = torch.empty(img.shape, dtype=torch.float32, device=dev)
+
+    def step():
+        pair.forward_project(x, out=sino)
+        pair.back_project(sino, out=back)
+
+    log(f"warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx = pair.context
+    ctx.kernel_timing()  # reset
+    ctx.set_kernel_timing(True)
+    log(f"timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    fp_ms, nfp, bp_ms, nbp = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    fp_avg = fp_ms / max(nfp, 1)
+    bp_avg = bp_ms / max(nbp, 1)
+    log(f"pair {ms_per_step:.3f} ms (fp kernel {fp_avg:.3f} ms, bp kernel {bp_avg:.3f} ms)")
+
+    nnz = int(info.num_nonzeros)
+    nbins = int(pdi.num_bins)
+    nvox = int(img.num_voxels)
+    # algorithmic bytes per launch: one 4-byte operand gather per non-zero,
+    # each output written once, each input streamed once
+    b_fp = 4 * nnz + 4 * nbins + 4 * nvox
+    b_bp = 4 * nnz + 4 * nvox + 4 * nbins
+    fp_gbs = b_fp / (fp_avg * 1e-3) / 1e9
+    bp_gbs = b_bp / (bp_avg * 1e-3) / 1e9
+    dominant = "bp" if bp_avg >= fp_avg else "fp"
+    dom_ms = bp_avg if dominant == "bp" else fp_avg
+    dom_bytes = b_bp if dominant == "bp" else b_fp
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": METRIC,
+        "value": round(1000.0 / ms_per_step, 4),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded ellipsoidal torso phantom (10 hot spheres, lung 0.2), Poisson data mean 2/bin + 0.05",
+        "config": {
+            "workload": "mMR-size projector pair: 64 rings x 504 crystals, R=%.0f mm, ring spacing 4.0625 mm, "
+                        "span 11, MRD 60, 837 sinograms x 252 views x 344 tangential; image 344x344x127 at "
+                        "2.08626x2.08626x2.03125 mm; ray-tracing matrix, all symmetries" % args.radius,
+            "num_bins": nbins,
+            "num_voxels": nvox,
+            "nnz": nnz,
+            "subsets": nsub,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "bp_kernel" if dominant == "bp" else "fp_kernel",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "kernels": {
+            "fp_kernel_ms": round(fp_avg, 4),
+            "bp_kernel_ms": round(bp_avg, 4),
+            "fp_algorithmic_GBps": round(fp_gbs, 1),
+            "bp_algorithmic_GBps": round(bp_gbs, 1),
+            "compulsory_bytes_per_projection": 4 * (nbins + nvox),
+        },
+        "setup_s": round(setup_s, 3),
+    }
+
+    if not args.skip_osem:
+        fwd = pair.forward_project(x).cpu().numpy()
+        y, scale = phantoms.poisson_data(fwd, 2.0, 0.05, seed=101)
+        obj = PoissonLogLikelihoodWithLinearModelForMeanAndProjData(pair)
+        t0 = time.time()
+        obj.set_up(y, 0.05)
+        torch.cuda.synchronize()
+        sens_s = time.time() - t0
+        recon = OSMAPOSLReconstruction(obj, num_subiterations=nsub)
+        obj.set_estimate(np.ones(img.shape, np.float32), enforce_positivity=True)
+        # one warm-up subiteration, then timed full iterations
+        recon.update_estimate(0, 1)
+        torch.cuda.synchronize()
+        ctx.kernel_timing()
+        t0 = time.perf_counter()
+        k = 2
+        for _ in range(args.osem_iterations):
+            for s in range(nsub):
+                recon.update_estimate((k - 1) % nsub, k)
+                k += 1
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        it_s = (t1 - t0) / args.osem_iterations
+        result["osem"] = {"iter_per_s": round(1.0 / it_s, 4), "ms_per_iteration": round(it_s * 1000, 3),
+                          "subsets": nsub, "sensitivity_s": round(sens_s, 3)}
+        log(f"OSEM iteration {it_s * 1000:.2f} ms")
+
+    if not args.skip_cpu:
+        result["cpu_baseline"] = cpu_baseline(pdi, img, pm, x_true, nsub, args.cpu_subsets)
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(pdi, img, pm, x_true, nsub, nsample):
+    """The oracle's OpenMP forward + back projection (per-thread images
+    reduced serially, as the reference) on `nsample` of the 21 subsets,
+    scaled to whole pairs/s."""
+    import numpy as np
+
+    from oracle.oracle import OraclePlan
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"cpu baseline: oracle plan ({threads} threads)")
+    t0 = time.time()
+    plan = OraclePlan(pdi, img, pm)
+    plan_s = time.time() - t0
+    sino = np.zeros(pdi.num_bins, np.float32)
+    t0 = time.time()
+    for s in range(nsample):
+        plan.forward_project(x_true, subset=s, num_subsets=nsub, omp=True, out=sino)
+        plan.back_project(sino, subset=s, num_subsets=nsub, omp=True)
+    dt = time.time() - t0
+    frac = nsample / nsub
+    pairs_per_s = frac / dt
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    log(f"cpu baseline {dt:.2f}s for {nsample}/{nsub} of a pair -> {pairs_per_s:.5f} pairs/s")
+    return {"value": round(pairs_per_s, 6), "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"oracle OpenMP forward+back projection of {nsample} of {nsub} subsets "
+                      f"({frac * 100:.1f}% of the bins), scaled to whole pairs; cache fill {plan_s:.1f}s excluded; "
+                      f"CPU {cpu}"}
+
+
+if __name__ == "__main__":
+    main()

@@ -82,6 +82,7 @@ typedef struct {
   int64_t num_families;         /* (segment, view, tangential) groups sharing a z-shifted row */
   int64_t num_matrix_elements;  /* sum of row lengths over all bins, before z clipping */
   int64_t num_table_entries;    /* entries of the voxel-driven back-projection table */
+  int64_t num_nonzeros;         /* matrix elements whose voxel lies inside the image (nnz after z clipping) */
   int64_t device_bytes;         /* device memory held by the context */
   int32_t num_subsets;
   int32_t symmetry_flags[5];    /* effective 90,180,swap_segment,swap_s,shift_z */
@@ -123,9 +124,10 @@ int mi_get_row(mi_ctx* ctx, int segment, int view, int axial_pos, int tang_pos,
                int16_t* z, int16_t* y, int16_t* x, float* w, int cap, int* n);
 
 /* ---- projections, host buffers (STIR layouts) ---------------------------- */
-/* ForwardProjectorByBin::forward_project(ProjData&, image)
- * (ForwardProjectorByBin.cxx:97-103,168-230). subset < 0: all bins; otherwise
- * only the bins of that subset are written (others untouched). */
+/* ForwardProjectorByBin::forward_project(ProjData&, image, subset_num,
+ * num_subsets, zero=true) (ForwardProjectorByBin.cxx:97-103,168-230).
+ * subset < 0: all bins; otherwise the bins of that subset are projected and
+ * all other bins are set to zero. */
 int mi_forward_project(mi_ctx* ctx, const float* image, float* proj_data, int subset);
 /* BackProjectorByBin::back_project(image, ProjData)
  * (BackProjectorByBin.cxx:111-117,183-229,316-346); overwrites image. */
@@ -173,6 +175,15 @@ int mi_obj_log_likelihood(mi_ctx* ctx, int subset, double* value);
 
 /* Waits for all work of the context. */
 int mi_synchronize(mi_ctx* ctx);
+
+/* ---- diagnostics ----------------------------------------------------------- */
+/* No counterpart in the reference (its TimedObject timers are CPU/wall clocks,
+ * TimedObject.h:45-57). When enabled, every forward- and back-projection kernel
+ * launch of the context is bracketed by HIP events recorded on the stream it
+ * runs on; mi_get_kernel_timing returns the summed kernel time (ms) and the
+ * launch counts since the last call, and resets them. */
+int mi_set_kernel_timing(mi_ctx* ctx, int enable);
+int mi_get_kernel_timing(mi_ctx* ctx, double* fp_ms, int64_t* fp_launches, double* bp_ms, int64_t* bp_launches);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,7 @@ EXPORTED_SYMBOLS = (
     "mi_subset_view_segments", "mi_get_row", "mi_forward_project", "mi_back_project", "mi_forward_project_dev",
     "mi_back_project_dev", "mi_obj_set_data", "mi_obj_compute_sensitivities", "mi_obj_get_sensitivity",
     "mi_obj_set_estimate", "mi_obj_get_estimate", "mi_osem_subset_step", "mi_osem_run", "mi_obj_subset_gradient",
-    "mi_obj_log_likelihood", "mi_synchronize",
+    "mi_obj_log_likelihood", "mi_synchronize", "mi_set_kernel_timing", "mi_get_kernel_timing",
 )
 
 
@@ -40,6 +40,7 @@ class InfoStruct(ctypes.Structure):
         ("num_families", ctypes.c_int64),
         ("num_matrix_elements", ctypes.c_int64),
         ("num_table_entries", ctypes.c_int64),
+        ("num_nonzeros", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64),
         ("num_subsets", ctypes.c_int32),
         ("symmetry_flags", ctypes.c_int32 * 5),
@@ -87,6 +88,9 @@ def load():
     L.mi_obj_subset_gradient.argtypes = [vp, c_int, c_int, _f32p]
     L.mi_obj_log_likelihood.argtypes = [vp, c_int, ctypes.POINTER(ctypes.c_double)]
     L.mi_synchronize.argtypes = [vp]
+    L.mi_set_kernel_timing.argtypes = [vp, c_int]
+    L.mi_get_kernel_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
     for name in EXPORTED_SYMBOLS:
         fn = getattr(L, name)
         if name not in ("mi_last_error", "mi_ctx_destroy"):
@@ -130,6 +134,17 @@ class Context:
         out = InfoStruct()
         check(load().mi_get_info(self._h, ctypes.byref(out)), "mi_get_info")
         return out
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        check(load().mi_set_kernel_timing(self._h, int(enable)), "mi_set_kernel_timing")
+
+    def kernel_timing(self):
+        """(fp_ms, fp_launches, bp_ms, bp_launches) since the last call."""
+        f, b = ctypes.c_double(), ctypes.c_double()
+        nf, nb = ctypes.c_int64(), ctypes.c_int64()
+        check(load().mi_get_kernel_timing(self._h, ctypes.byref(f), ctypes.byref(nf), ctypes.byref(b), ctypes.byref(nb)),
+              "mi_get_kernel_timing")
+        return f.value, nf.value, b.value, nb.value
 
 
 def float_ptr_or_none(a: Optional[np.ndarray]):

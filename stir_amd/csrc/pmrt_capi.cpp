@@ -98,6 +98,31 @@ struct mi_ctx {
   std::unordered_map<int64_t, int> fam_lookup;  // (seg,view,tang,a0) -> family
   int64_t num_basic_elements = 0;
   int64_t num_matrix_elements = 0;
+  int64_t num_nonzeros = 0;
+  // kernel timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> fp_events, bp_events;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  void release_events() {
+    for (auto& p : fp_events) { event_pool.push_back(p.first); event_pool.push_back(p.second); }
+    for (auto& p : bp_events) { event_pool.push_back(p.first); event_pool.push_back(p.second); }
+    fp_events.clear();
+    bp_events.clear();
+  }
+  ~mi_ctx() {
+    release_events();
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+  }
   // images / projection data (device layout) and staging (STIR layout)
   DBuf<float> img, img2, sino, ratio, stage_img, stage_sino;
   // objective function
@@ -147,13 +172,19 @@ struct mi_ctx {
   }
 
   // ---- projections on device-layout buffers
-  void fp(mi::FpMode mode, const float* d_img, float* d_out, int subset, const mi::FpData& data, hipStream_t st) const {
+  void fp(mi::FpMode mode, const float* d_img, float* d_out, int subset, const mi::FpData& data, hipStream_t st) {
     const int tb = task_off[subset_begin(subset)], te = task_off[subset_end(subset)];
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
     mi::launch_forward_project(mode, tasks.p + tb, te - tb, fams.p, pool.p, d_img, d_out, dims, data, st);
+    if (timing) { HIPCHECK(hipEventRecord(e1, st)); fp_events.emplace_back(e0, e1); }
   }
-  void bp(mi::BpMode mode, const float* d_proj, float* d_img, int subset, const mi::BpData& data, hipStream_t st) const {
+  void bp(mi::BpMode mode, const float* d_proj, float* d_img, int subset, const mi::BpData& data, hipStream_t st) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
     mi::launch_back_project(mode, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj, d_img, dims, data,
                             st);
+    if (timing) { HIPCHECK(hipEventRecord(e1, st)); bp_events.emplace_back(e0, e1); }
   }
 
   void set_up(const mi_geometry& g, int nsub);
@@ -285,6 +316,17 @@ void mi_ctx::build_rows() {
   }
   fams.alloc(h_fams.size());
   HIPCHECK(hipMemcpyAsync(fams.p, h_fams.data(), h_fams.size() * sizeof(mi::Family), hipMemcpyHostToDevice, stream));
+  {
+    DBuf<unsigned long long> d_nnz;
+    d_nnz.alloc(1);
+    HIPCHECK(hipMemsetAsync(d_nnz.p, 0, sizeof(unsigned long long), stream));
+    mi::launch_count_nonzeros(fams.p, static_cast<int>(h_fams.size()), pool.p, dims, d_nnz.p, stream);
+    HIPCHECK(hipGetLastError());
+    unsigned long long h = 0;
+    HIPCHECK(hipMemcpyAsync(&h, d_nnz.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    num_nonzeros = static_cast<int64_t>(h);
+  }
   HIPCHECK(hipStreamSynchronize(stream));
 }
 
@@ -454,6 +496,7 @@ int mi_get_info(mi_ctx* ctx, mi_info* out) {
     out->num_families = static_cast<int64_t>(ctx->h_fams.size());
     out->num_matrix_elements = ctx->num_matrix_elements;
     out->num_table_entries = static_cast<int64_t>(ctx->entries.n);
+    out->num_nonzeros = ctx->num_nonzeros;
     out->device_bytes = static_cast<int64_t>(ctx->pool.bytes() + ctx->fams.bytes() + ctx->tasks.bytes() +
                                              ctx->list_off.bytes() + ctx->entries.bytes() + ctx->img.bytes() +
                                              ctx->img2.bytes() + ctx->sino.bytes() + ctx->ratio.bytes() +
@@ -788,6 +831,39 @@ int mi_obj_log_likelihood(mi_ctx* ctx, int subset, double* value) {
     double acc = 0;
     for (double v : h) acc += v;
     *value = acc;
+  });
+}
+
+int mi_set_kernel_timing(mi_ctx* ctx, int enable) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->timing = enable != 0;
+  });
+}
+
+int mi_get_kernel_timing(mi_ctx* ctx, double* fp_ms, int64_t* fp_launches, double* bp_ms, int64_t* bp_launches) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    HIPCHECK(hipSetDevice(ctx->device));
+    double f = 0, b = 0;
+    for (auto& p : ctx->fp_events) {
+      HIPCHECK(hipEventSynchronize(p.second));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, p.first, p.second));
+      f += ms;
+    }
+    for (auto& p : ctx->bp_events) {
+      HIPCHECK(hipEventSynchronize(p.second));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, p.first, p.second));
+      b += ms;
+    }
+    if (fp_ms) *fp_ms = f;
+    if (bp_ms) *bp_ms = b;
+    if (fp_launches) *fp_launches = static_cast<int64_t>(ctx->fp_events.size());
+    if (bp_launches) *bp_launches = static_cast<int64_t>(ctx->bp_events.size());
+    ctx->release_events();
   });
 }
 

@@ -89,6 +89,8 @@ void launch_make_entries(const Family* d_fams, int fam_begin, int fam_end, const
                          const int2* d_pool, int subset, const ImageDims& dims, uint32_t* d_keys, BpEntry* d_entries,
                          hipStream_t stream);
 void launch_iota(uint32_t* d_vals, int64_t n, hipStream_t stream);
+void launch_count_nonzeros(const Family* d_fams, int nfam, const int2* d_pool, const ImageDims& dims,
+                           unsigned long long* d_total, hipStream_t stream);
 void launch_gather_entries(const BpEntry* d_src, const uint32_t* d_idx, BpEntry* d_dst, int64_t n, hipStream_t stream);
 
 // ---- small utilities (pmrt_project.hip)

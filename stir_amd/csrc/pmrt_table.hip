@@ -75,6 +75,30 @@ __global__ void __launch_bounds__(64) make_entries_kernel(const Family* __restri
   }
 }
 
+// number of (bin, element) pairs whose voxel z lies inside the image:
+// a in [0, nax) with 0 <= zsign*z + zadd0 + ppa*a - min_z < nz
+__global__ void __launch_bounds__(64) count_nonzeros_kernel(const Family* __restrict__ fams, int nfam,
+                                                            const int2* __restrict__ pool, ImageDims d,
+                                                            unsigned long long* __restrict__ total) {
+  const int fi = blockIdx.x;
+  if (fi >= nfam) return;
+  const Family f = fams[fi];
+  unsigned long long cnt = 0;
+  for (int e = threadIdx.x; e < f.row_len; e += blockDim.x) {
+    int x, y, z;
+    decode(pool[f.row_start + e].x, x, y, z);
+    const int k = f.zsign * z + f.zadd0 - d.min_z;  // image z of axial position 0
+    // ppa*a in [-k, nz-1-k]
+    const int lo_num = -k, hi_num = d.nz - 1 - k;
+    int lo = lo_num <= 0 ? 0 : (lo_num + f.ppa - 1) / f.ppa;
+    int hi = hi_num < 0 ? -1 : hi_num / f.ppa;
+    if (hi > f.nax - 1) hi = f.nax - 1;
+    if (hi >= lo) cnt += static_cast<unsigned long long>(hi - lo + 1);
+  }
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+  if (threadIdx.x == 0) atomicAdd(total, cnt);
+}
+
 __global__ void iota_kernel(uint32_t* v, int64_t n) {
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -110,6 +134,12 @@ void launch_make_entries(const Family* d_fams, int fam_begin, int fam_end, const
   if (fam_end <= fam_begin) return;
   hipLaunchKernelGGL(make_entries_kernel, dim3(fam_end - fam_begin), dim3(64), 0, stream, d_fams, fam_begin, fam_end,
                      d_fam_entry_off, d_pool, subset, dims, d_keys, d_entries);
+}
+
+void launch_count_nonzeros(const Family* d_fams, int nfam, const int2* d_pool, const ImageDims& dims,
+                           unsigned long long* d_total, hipStream_t stream) {
+  if (nfam <= 0) return;
+  hipLaunchKernelGGL(count_nonzeros_kernel, dim3(nfam), dim3(64), 0, stream, d_fams, nfam, d_pool, dims, d_total);
 }
 
 void launch_iota(uint32_t* d_vals, int64_t n, hipStream_t stream) {
