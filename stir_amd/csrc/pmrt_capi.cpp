@@ -99,6 +99,7 @@ struct mi_ctx {
   int64_t num_basic_elements = 0;
   int64_t num_matrix_elements = 0;
   int64_t num_nonzeros = 0;
+  int uniform_ppa = 0;  // planes per axial position if equal for all families, else 0
   // kernel timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> fp_events, bp_events;
@@ -182,8 +183,8 @@ struct mi_ctx {
   void bp(mi::BpMode mode, const float* d_proj, float* d_img, int subset, const mi::BpData& data, hipStream_t st) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
-    mi::launch_back_project(mode, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj, d_img, dims, data,
-                            st);
+    mi::launch_back_project(mode, uniform_ppa, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj, d_img,
+                            dims, data, st);
     if (timing) { HIPCHECK(hipEventRecord(e1, st)); bp_events.emplace_back(e0, e1); }
   }
 
@@ -314,6 +315,9 @@ void mi_ctx::build_rows() {
     num_matrix_elements += static_cast<int64_t>(h_len[bi]) * h_fams[i].nax;
     fam_lookup[fam_key(keys[i].seg, keys[i].view, keys[i].tang, keys[i].a0)] = static_cast<int>(i);
   }
+  uniform_ppa = h_fams.empty() ? 0 : h_fams[0].ppa;
+  for (const auto& f : h_fams)
+    if (f.ppa != uniform_ppa) uniform_ppa = 0;
   fams.alloc(h_fams.size());
   HIPCHECK(hipMemcpyAsync(fams.p, h_fams.data(), h_fams.size() * sizeof(mi::Family), hipMemcpyHostToDevice, stream));
   {
@@ -331,13 +335,35 @@ void mi_ctx::build_rows() {
 }
 
 void mi_ctx::build_tasks() {
+  // One task per family, or per (+segment, -segment) pair of families that
+  // share the basic row and in-plane operation (the swap-segment symmetry);
+  // the pair's bins together fit the 4 x 64 lanes of a wave.
   const int nsub = plan->num_subsets();
+  const auto& keys = plan->family_keys();
   std::vector<mi::FpTask> h;
+  std::vector<char> taken(h_fams.size(), 0);
   task_off.assign(nsub + 1, 0);
   for (int s = 0; s < nsub; ++s) {
     task_off[s] = static_cast<int>(h.size());
-    for (int f = plan->subset_family_begin(s); f < plan->subset_family_begin(s + 1); ++f)
-      for (int a0 = 0; a0 < h_fams[f].nax; a0 += 64) h.push_back({f, a0});
+    for (int f = plan->subset_family_begin(s); f < plan->subset_family_begin(s + 1); ++f) {
+      if (taken[f]) continue;
+      taken[f] = 1;
+      if (h_fams[f].nax > 256) throw ApiError(MI_ERR_ARGUMENT, "more than 256 axial positions in a segment");
+      int partner = -1;
+      if (keys[f].seg > 0) {
+        auto it = fam_lookup.find(fam_key(-keys[f].seg, keys[f].view, keys[f].tang, keys[f].a0));
+        if (it != fam_lookup.end()) {
+          const int g = it->second;
+          const mi::Family& A = h_fams[f];
+          const mi::Family& B = h_fams[g];
+          if (!taken[g] && g >= plan->subset_family_begin(s) && g < plan->subset_family_begin(s + 1) &&
+              A.row_start == B.row_start && A.row_len == B.row_len && A.inplane == B.inplane && A.nax + B.nax <= 256)
+            partner = g;
+        }
+      }
+      if (partner >= 0) taken[partner] = 1;
+      h.push_back({f, partner});
+    }
   }
   task_off[nsub] = static_cast<int>(h.size());
   tasks.alloc(h.size());

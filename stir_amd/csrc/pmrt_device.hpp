@@ -9,11 +9,13 @@
 
 namespace mi {
 
-// One wavefront of forward-projection work: a family and the first of the
-// (up to 64) consecutive axial positions its lanes handle.
+// One wavefront of forward-projection work: one family, or the two families
+// of a (+segment, -segment) pair that share a basic row and in-plane
+// operation (second = -1 when single). Lane l of slot q handles combined
+// axial index g = l + 64*q: family `first` for g < nax(first), then `second`.
 struct FpTask {
-  int32_t family;
-  int32_t a0;
+  int32_t first;
+  int32_t second;
 };
 
 // One entry of the voxel-driven back-projection table: the family's
@@ -75,7 +77,9 @@ void launch_pack_rows(int first, int count, const uint64_t* scratch_k, const flo
 void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams, const int2* d_pool,
                             const float* d_img, float* d_out, const ImageDims& dims, const FpData& data,
                             hipStream_t stream);
-void launch_back_project(BpMode mode, const int64_t* d_list_off, const BpEntry* d_entries, int subset_begin,
+// ppa: planes per axial position shared by all families (1 or 2 select a
+// specialised kernel), 0 when it varies between segments.
+void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const BpEntry* d_entries, int subset_begin,
                          int subset_end, const float* d_proj, float* d_img, const ImageDims& dims, const BpData& data,
                          hipStream_t stream);
 // 2-D transpose used for the STIR <-> device layout conversions:

@@ -40,6 +40,12 @@ __device__ inline int xcd_remap(int bid, int nblocks) {
   return start + idx;
 }
 
+// base[idx] with a wave-uniform base and a 32-bit unsigned lane offset, so
+// the address is formed as scalar base + 32-bit vector offset
+__device__ inline float load_u32off(const float* base, unsigned idx) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (idx << 2));
+}
+
 template <int IP>
 __device__ inline void inplane_op(int x, int y, int& xo, int& yo) {
   if constexpr (IP == IP_ID) { xo = x; yo = y; }
@@ -58,60 +64,115 @@ __device__ inline void decode(int c, int& x, int& y, int& z) {
   z = static_cast<int>(static_cast<unsigned>(c) >> 20) - 2048;
 }
 
-template <int IP>
-__device__ inline float fp_row(const int2* __restrict__ row, int n, int zsign, int zl, bool valid,
-                               const float* __restrict__ img, const ImageDims& d) {
-  float acc = 0.F;
+constexpr int kUnroll = 8;      // matrix elements whose gathers are in flight together
+constexpr int kWavesPerBlock = 4;
+
+// Per-lane, per-slot bin of a forward-projection task.
+template <int NSLOT>
+struct LaneBins {
+  int zs[NSLOT];     // z sign of the symmetry operation
+  int zb[NSLOT];     // image z of the element z = 0 for this bin (zadd0 + ppa*a - min_z)
+  bool valid[NSLOT];
+  int64_t out[NSLOT];  // projection-data index
+  int vg[NSLOT];     // viewgram index
+};
+
+// Sum of img * w over one row in row order for every bin of the lane
+// (ProjMatrixElemsForOneBin.cxx:354-371). kUnroll elements have their
+// gathers issued before they are accumulated; lanes whose voxel is outside
+// the image read a clamped (valid) address and skip the add, so each sum is
+// the reference's, term by term.
+template <int IP, int NSLOT>
+__device__ inline void fp_row(const int2* __restrict__ row, int n, const LaneBins<NSLOT>& L,
+                              const float* __restrict__ img, const ImageDims& d, float (&acc)[NSLOT]) {
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
   int e = 0;
+  for (; e + kUnroll <= n; e += kUnroll) {
+    float v[kUnroll][NSLOT], w[kUnroll];
+    bool ok[kUnroll][NSLOT];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      const int2 el = row[e + j];
+      int x, y, z, xo, yo;
+      decode(el.x, x, y, z);
+      inplane_op<IP>(x, y, xo, yo);
+      const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
+      const float* colp = img + static_cast<int64_t>(col) * d.nzp;  // wave-uniform
+      w[j] = __int_as_float(el.y);
+#pragma unroll
+      for (int q = 0; q < NSLOT; ++q) {
+        const int zz = L.zs[q] * z + L.zb[q];
+        ok[j][q] = L.valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
+        const int zc = min(max(zz, 0), d.nz - 1);
+        v[j][q] = load_u32off(colp, static_cast<unsigned>(zc));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j)
+#pragma unroll
+      for (int q = 0; q < NSLOT; ++q)
+        if (ok[j][q]) acc[q] = acc[q] + v[j][q] * w[j];
+  }
   for (; e < n; ++e) {
     const int2 el = row[e];
     int x, y, z, xo, yo;
     decode(el.x, x, y, z);
     inplane_op<IP>(x, y, xo, yo);
     const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-    const int zz = zsign * z + zl;
-    if (valid && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz)) {
-      const float v = img[static_cast<int64_t>(col) * d.nzp + zz];
-      acc = acc + v * __int_as_float(el.y);
+    const float* colp = img + static_cast<int64_t>(col) * d.nzp;
+#pragma unroll
+    for (int q = 0; q < NSLOT; ++q) {
+      const int zz = L.zs[q] * z + L.zb[q];
+      if (L.valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz))
+        acc[q] = acc[q] + load_u32off(colp, static_cast<unsigned>(zz)) * __int_as_float(el.y);
     }
   }
-  return acc;
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(kWave) fp_kernel(const FpTask* __restrict__ tasks, int ntasks,
-                                                   const Family* __restrict__ fams, const int2* __restrict__ pool,
-                                                   const float* __restrict__ img, float* __restrict__ out, ImageDims d,
-                                                   FpData data) {
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  if (t >= ntasks) return;
-  const int lane = threadIdx.x;
-  const FpTask task = tasks[t];
-  const Family f = fams[task.family];
-  const int a = task.a0 + lane;
-  const bool valid = a < f.nax;
-  const int zl = f.zadd0 + f.ppa * a - d.min_z;
-  const int2* row = pool + f.row_start;
-  float acc;
-  switch (f.inplane) {
-    case IP_ID: acc = fp_row<IP_ID>(row, f.row_len, f.zsign, zl, valid, img, d); break;
-    case IP_XMX: acc = fp_row<IP_XMX>(row, f.row_len, f.zsign, zl, valid, img, d); break;
-    case IP_YMY: acc = fp_row<IP_YMY>(row, f.row_len, f.zsign, zl, valid, img, d); break;
-    case IP_XMX_YMY: acc = fp_row<IP_XMX_YMY>(row, f.row_len, f.zsign, zl, valid, img, d); break;
-    case IP_XY_YX: acc = fp_row<IP_XY_YX>(row, f.row_len, f.zsign, zl, valid, img, d); break;
-    case IP_XMY_YX: acc = fp_row<IP_XMY_YX>(row, f.row_len, f.zsign, zl, valid, img, d); break;
-    case IP_XY_YMX: acc = fp_row<IP_XY_YMX>(row, f.row_len, f.zsign, zl, valid, img, d); break;
-    default: acc = fp_row<IP_XMY_YMX>(row, f.row_len, f.zsign, zl, valid, img, d); break;
+template <int MODE, int NSLOT>
+__device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, const int2* __restrict__ pool,
+                               const float* __restrict__ img, float* __restrict__ out, const ImageDims& d,
+                               const FpData& data, int t, int lane) {
+  LaneBins<NSLOT> L;
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) {
+    const int g = lane + kWave * q;
+    const bool in1 = g < f1.nax;
+    const bool in2 = pair && !in1 && g < f1.nax + f2.nax;
+    const int a = in1 ? g : g - f1.nax;
+    L.valid[q] = in1 || in2;
+    const Family& f = in1 ? f1 : f2;
+    L.zs[q] = in1 ? f1.zsign : f2.zsign;
+    L.zb[q] = (in1 ? f1.zadd0 + f1.ppa * a : f2.zadd0 + f2.ppa * a) - d.min_z;
+    L.out[q] = static_cast<int64_t>(in1 ? f1.sino_base : f2.sino_base) + a;
+    L.vg[q] = in1 ? f1.viewgram : f2.viewgram;
+    (void)f;
   }
-  const int64_t i = static_cast<int64_t>(f.sino_base) + a;
-  if constexpr (MODE == FP_PLAIN) {
-    if (valid) out[i] = acc;
-  } else if constexpr (MODE == FP_RATIO || MODE == FP_GRADIENT) {
-    if (valid) {
+  const int2* row = pool + f1.row_start;
+  float acc[NSLOT];
+  switch (f1.inplane) {
+    case IP_ID: fp_row<IP_ID, NSLOT>(row, f1.row_len, L, img, d, acc); break;
+    case IP_XMX: fp_row<IP_XMX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
+    case IP_YMY: fp_row<IP_YMY, NSLOT>(row, f1.row_len, L, img, d, acc); break;
+    case IP_XMX_YMY: fp_row<IP_XMX_YMY, NSLOT>(row, f1.row_len, L, img, d, acc); break;
+    case IP_XY_YX: fp_row<IP_XY_YX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
+    case IP_XMY_YX: fp_row<IP_XMY_YX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
+    case IP_XY_YMX: fp_row<IP_XY_YMX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
+    default: fp_row<IP_XMY_YMX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
+  }
+  double ll = 0.0;
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) {
+    if (!L.valid[q]) continue;
+    const int64_t i = L.out[q];
+    if constexpr (MODE == FP_PLAIN) {
+      out[i] = acc[q];
+    } else if constexpr (MODE == FP_RATIO || MODE == FP_GRADIENT) {
       // estimated += additive; divide_and_truncate (recon_array_functions.cxx:172-266)
-      const float ybar = acc + (data.additive ? data.additive[i] : data.additive_constant);
+      const float ybar = acc[q] + (data.additive ? data.additive[i] : data.additive_constant);
       float num = data.measured[i];
-      const float small_value = data.vg_small[f.viewgram];
+      const float small_value = data.vg_small[L.vg[q]];
       if (num <= small_value) {
         num = 0;
       } else {
@@ -121,75 +182,143 @@ __global__ void __launch_bounds__(kWave) fp_kernel(const FpTask* __restrict__ ta
       }
       if constexpr (MODE == FP_GRADIENT) num = num - (data.mult ? data.mult[i] : 1.F);
       out[i] = num;
-    }
-  } else {  // FP_LOGLIK: accumulate_loglikelihood (recon_array_functions.cxx:336-377)
-    double v = 0.0;
-    if (valid) {
-      float est = acc + (data.additive ? data.additive[i] : data.additive_constant);
+    } else {  // FP_LOGLIK: accumulate_loglikelihood (recon_array_functions.cxx:336-377)
+      float est = acc[q] + (data.additive ? data.additive[i] : data.additive_constant);
       if (data.mult) est = est * data.mult[i];
       const float y = data.measured[i];
-      const float small_value = data.vg_small[f.viewgram];
+      const float small_value = data.vg_small[L.vg[q]];
       const float ne = (est < y / 10000.F) ? y / 10000.F : est;
-      if (y <= small_value) v = -static_cast<double>(ne);
-      else v = y * log(static_cast<double>(ne)) - static_cast<double>(ne);
+      if (y <= small_value) ll += -static_cast<double>(ne);
+      else ll += y * log(static_cast<double>(ne)) - static_cast<double>(ne);
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    if (lane == 0) data.partial[t] = v;
+  }
+  if constexpr (MODE == FP_LOGLIK) {
+    for (int off = kWave / 2; off > 0; off >>= 1) ll += __shfl_xor(ll, off, kWave);
+    if (lane == 0) data.partial[t] = ll;
   }
 }
 
-// Voxel-driven back projection: one wave per (image column (y,x), chunk of
-// 64 z values), lanes along z. The entries of the column's table are
-// wave-uniform and the projection data each entry reads are one contiguous run.
+// One wave per task; kWavesPerBlock tasks per workgroup. The number of axial
+// slots per lane (1..4) follows the task's total number of bins.
 template <int MODE>
-__global__ void __launch_bounds__(kWave) bp_kernel(const int64_t* __restrict__ list_off, const BpEntry* __restrict__ ent,
-                                                   int sb, int se, const float* __restrict__ proj,
-                                                   float* __restrict__ img, ImageDims d, int nzch, BpData data) {
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
-  const int col = w / nzch;
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask* __restrict__ tasks, int ntasks,
+                                                                   const Family* __restrict__ fams,
+                                                                   const int2* __restrict__ pool,
+                                                                   const float* __restrict__ img, float* __restrict__ out,
+                                                                   ImageDims d, FpData data) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int t = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (t >= ntasks) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  const FpTask task = tasks[t];
+  const Family f1 = fams[task.first];
+  const bool pair = task.second >= 0;
+  const Family f2 = fams[pair ? task.second : task.first];
+  const int total = f1.nax + (pair ? f2.nax : 0);
+  const int nslot = (total + kWave - 1) / kWave;
+  switch (nslot) {
+    case 1: fp_task<MODE, 1>(f1, f2, pair, pool, img, out, d, data, t, lane); break;
+    case 2: fp_task<MODE, 2>(f1, f2, pair, pool, img, out, d, data, t, lane); break;
+    case 3: fp_task<MODE, 3>(f1, f2, pair, pool, img, out, d, data, t, lane); break;
+    default: fp_task<MODE, 4>(f1, f2, pair, pool, img, out, d, data, t, lane); break;
+  }
+}
+
+// Axial index of voxel zi for a table entry; `ok` tells whether the entry
+// contributes, the returned index is always inside the entry's run. PPA is
+// the number of image planes per axial position (0: not known at compile time).
+template <int PPA>
+__device__ inline unsigned entry_axial(int k, int nax_ppa, int zi, bool& ok) {
+  const int nax = nax_ppa & 0xffff;
+  const int diff = zi - k;
+  int a;
+  if constexpr (PPA == 1) {
+    a = diff;
+    ok = static_cast<unsigned>(a) < static_cast<unsigned>(nax);
+  } else if constexpr (PPA == 2) {
+    a = diff >> 1;
+    ok = (diff & 1) == 0 && static_cast<unsigned>(a) < static_cast<unsigned>(nax);
+  } else {
+    const int ppa = nax_ppa >> 16;
+    a = diff / ppa;
+    ok = diff >= 0 && a * ppa == diff && a < nax;
+  }
+  return static_cast<unsigned>(min(max(a, 0), nax - 1));
+}
+
+// Voxel-driven back projection: one wave per image column (y,x), NSLOT
+// z values per lane (z = lane + 64*slot), kWavesPerBlock columns per
+// workgroup. Each table entry (wave-uniform, scalar loads) is decoded once
+// for all slots; kUnroll entries have their projection-data reads in flight
+// together and are accumulated in table order.
+template <int MODE, int PPA, int NSLOT>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t* __restrict__ list_off,
+                                                                   const BpEntry* __restrict__ ent, int sb, int se,
+                                                                   const float* __restrict__ proj,
+                                                                   float* __restrict__ img, ImageDims d, BpData data) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int col = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
   if (col >= d.ncols) return;
-  const int z = (w - col * nzch) * kWave + static_cast<int>(threadIdx.x);
-  const int zi = d.min_z + z;
-  float acc = 0.F;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  float acc[NSLOT];
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
   for (int s = sb; s < se; ++s) {
     const int64_t beg = list_off[static_cast<int64_t>(s) * d.ncols + col];
     const int64_t end = list_off[static_cast<int64_t>(s) * d.ncols + col + 1];
-    for (int64_t e = beg; e < end; ++e) {
-      const BpEntry E = ent[e];
-      const int nax = E.nax_ppa & 0xffff;
-      const int ppa = E.nax_ppa >> 16;
-      const int diff = zi - E.k;
-      int a;
-      bool ok;
-      if (ppa == 1) {
-        a = diff;
-        ok = static_cast<unsigned>(a) < static_cast<unsigned>(nax);
-      } else if (ppa == 2) {
-        a = diff >> 1;
-        ok = diff >= 0 && (diff & 1) == 0 && a < nax;
-      } else {
-        a = diff / ppa;
-        ok = diff >= 0 && a * ppa == diff && a < nax;
+    int64_t e = beg;
+    for (; e + kUnroll <= end; e += kUnroll) {
+      float p[kUnroll][NSLOT], wt[kUnroll];
+      bool ok[kUnroll][NSLOT];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) {
+        const BpEntry E = ent[e + j];
+        const float* run = proj + E.sino_base;  // wave-uniform
+        wt[j] = E.w;
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) {
+          const unsigned a = entry_axial<PPA>(E.k, E.nax_ppa, d.min_z + lane + kWave * q, ok[j][q]);
+          p[j][q] = load_u32off(run, a);
+        }
       }
-      if (ok) acc = acc + E.w * proj[static_cast<int64_t>(E.sino_base) + a];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j)
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q)
+          if (ok[j][q]) acc[q] = acc[q] + wt[j] * p[j][q];
+    }
+    for (; e < end; ++e) {
+      const BpEntry E = ent[e];
+      const float* run = proj + E.sino_base;
+#pragma unroll
+      for (int q = 0; q < NSLOT; ++q) {
+        bool ok;
+        const unsigned a = entry_axial<PPA>(E.k, E.nax_ppa, d.min_z + lane + kWave * q, ok);
+        if (ok) acc[q] = acc[q] + E.w * load_u32off(run, a);
+      }
     }
   }
-  if (z >= d.nzp) return;
-  const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
-  if constexpr (MODE == BP_PLAIN) {
-    img[idx] = z < d.nz ? acc : 0.F;
-  } else {
-    if (z < d.nz) {
-      // divide(update, sens, 0) (numerics/divide.inl:26-47), threshold, multiply
-      float u = acc;
-      const float sv = data.sens[idx];
-      if (fabsf(sv) <= 0.F && fabsf(u) <= 0.F) u = 0;
-      else u = u / sv;
-      if (data.threshold) {
-        if (u > FLT_MAX) u = FLT_MAX;
-        else if (0.F > u) u = 0.F;
+  float* out = img + static_cast<int64_t>(col) * d.nzp;
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) {
+    const int z = lane + kWave * q;
+    if (z >= d.nzp) continue;
+    if constexpr (MODE == BP_PLAIN) {
+      out[z] = z < d.nz ? acc[q] : 0.F;
+    } else {
+      if (z < d.nz) {
+        // divide(update, sens, 0) (numerics/divide.inl:26-47), threshold, multiply
+        const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
+        float u = acc[q];
+        const float sv = data.sens[idx];
+        if (fabsf(sv) <= 0.F && fabsf(u) <= 0.F) u = 0;
+        else u = u / sv;
+        if (data.threshold) {
+          if (u > FLT_MAX) u = FLT_MAX;
+          else if (0.F > u) u = 0.F;
+        }
+        data.estimate[idx] = data.estimate[idx] * u;
       }
-      data.estimate[idx] = data.estimate[idx] * u;
     }
   }
 }
@@ -251,7 +380,7 @@ void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, cons
                             const float* d_img, float* d_out, const ImageDims& dims, const FpData& data,
                             hipStream_t stream) {
   if (ntasks <= 0) return;
-  const dim3 grid(ntasks), block(kWave);
+  const dim3 grid((ntasks + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
   switch (mode) {
     case FP_PLAIN:
       hipLaunchKernelGGL(fp_kernel<FP_PLAIN>, grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, dims, data);
@@ -269,17 +398,32 @@ void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, cons
   }
 }
 
-void launch_back_project(BpMode mode, const int64_t* d_list_off, const BpEntry* d_entries, int subset_begin,
+void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const BpEntry* d_entries, int subset_begin,
                          int subset_end, const float* d_proj, float* d_img, const ImageDims& dims, const BpData& data,
                          hipStream_t stream) {
-  const int nzch = (dims.nzp + kWave - 1) / kWave;
-  const dim3 grid(static_cast<unsigned>(dims.ncols) * nzch), block(kWave);
-  if (mode == BP_PLAIN)
-    hipLaunchKernelGGL(bp_kernel<BP_PLAIN>, grid, block, 0, stream, d_list_off, d_entries, subset_begin, subset_end, d_proj,
-                       d_img, dims, nzch, data);
-  else
-    hipLaunchKernelGGL(bp_kernel<BP_OSEM_UPDATE>, grid, block, 0, stream, d_list_off, d_entries, subset_begin, subset_end,
-                       d_proj, d_img, dims, nzch, data);
+  const int nslot = (dims.nzp + kWave - 1) / kWave;
+  const dim3 grid((dims.ncols + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+#define MI_BP_LAUNCH(M, P, S)                                                                                       \
+  hipLaunchKernelGGL((bp_kernel<M, P, S>), grid, block, 0, stream, d_list_off, d_entries, subset_begin, subset_end, \
+                     d_proj, d_img, dims, data)
+#define MI_BP_SLOTS(M, P)                    \
+  switch (nslot) {                           \
+    case 1: MI_BP_LAUNCH(M, P, 1); break;    \
+    case 2: MI_BP_LAUNCH(M, P, 2); break;    \
+    case 3: MI_BP_LAUNCH(M, P, 3); break;    \
+    default: MI_BP_LAUNCH(M, P, 4); break;   \
+  }
+  if (mode == BP_PLAIN) {
+    if (ppa == 1) { MI_BP_SLOTS(BP_PLAIN, 1) }
+    else if (ppa == 2) { MI_BP_SLOTS(BP_PLAIN, 2) }
+    else { MI_BP_SLOTS(BP_PLAIN, 0) }
+  } else {
+    if (ppa == 1) { MI_BP_SLOTS(BP_OSEM_UPDATE, 1) }
+    else if (ppa == 2) { MI_BP_SLOTS(BP_OSEM_UPDATE, 2) }
+    else { MI_BP_SLOTS(BP_OSEM_UPDATE, 0) }
+  }
+#undef MI_BP_SLOTS
+#undef MI_BP_LAUNCH
 }
 
 void launch_transpose(const float* src, float* dst, int rows, int cols, int src_ld, int dst_ld, hipStream_t stream) {
