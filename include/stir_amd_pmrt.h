@@ -117,6 +117,14 @@ int mi_subsets_balanced(mi_ctx* ctx, int* balanced);
  * number of pairs; at most cap are written. */
 int mi_subset_view_segments(mi_ctx* ctx, int subset, int32_t* views, int32_t* segments, int cap, int* count);
 
+/* Basic bins and symmetry operations of n bins, evaluated on the host plan
+ * without a device: DataSymmetriesForBins_PET_CartesianGrid::
+ * find_symmetry_operation_from_basic_bin (.inl:664-674). bins, basic: n x
+ * (segment, view, axial, tangential); ops: n x (in-plane code, z sign, z add)
+ * with z' = zsign*z + zadd and the in-plane codes 0 (x,y), 1 (-x,y), 2 (x,-y),
+ * 3 (-x,-y), 4 (y,x), 5 (-y,x), 6 (y,-x), 7 (-y,-x). */
+int mi_symmetry_operations(const mi_geometry* geometry, int n, const int32_t* bins, int32_t* basic, int32_t* ops);
+
 /* Matrix row of one bin as the device uses it (basic row + symmetry
  * operation), ProjMatrixByBin::get_proj_matrix_elems_for_one_bin
  * (ProjMatrixByBin.inl:48-113). *n receives the length; at most cap written. */

@@ -48,6 +48,9 @@ def lib():
             fn.restype = ctypes.c_int
             fn.argtypes = [vp] + [ctypes.c_int] * 4 + [_i16p, _i16p, _i16p, _f32p, ctypes.c_int]
         L.or_bin_geometry.argtypes = [vp] + [ctypes.c_int] * 4 + [_f32p]
+        L.or_ray_trace.restype = ctypes.c_int
+        L.or_ray_trace.argtypes = [_f32p, _f32p, _f32p, ctypes.c_float, _i16p, _i16p, _i16p, _f32p, ctypes.c_int]
+        L.or_symmetry_op.argtypes = [vp] + [ctypes.c_int] * 4 + [_i32p, _i32p]
         L.or_subset_view_segments.restype = ctypes.c_int
         L.or_subset_view_segments.argtypes = [vp, ctypes.c_int, ctypes.c_int, _i32p, _i32p, ctypes.c_int]
         L.or_subsets_balanced.restype = ctypes.c_int
@@ -67,6 +70,20 @@ def lib():
                               ctypes.c_int]
         _lib = L
     return _lib
+
+
+def ray_trace(start_zyx, stop_zyx, vox_zyx, nc):
+    """RayTraceVoxelsOnCartesianGrid on one segment: (z, y, x, w)."""
+    st = np.asarray(start_zyx, np.float32); sp = np.asarray(stop_zyx, np.float32)
+    vx = np.asarray(vox_zyx, np.float32)
+    cap = 4096
+    while True:
+        z = np.empty(cap, np.int16); y = np.empty(cap, np.int16); x = np.empty(cap, np.int16)
+        w = np.empty(cap, np.float32)
+        n = lib().or_ray_trace(st, sp, vx, float(nc), z, y, x, w, cap)
+        if n >= 0:
+            return z[:n].copy(), y[:n].copy(), x[:n].copy(), w[:n].copy()
+        cap = -n
 
 
 def _opt_ptr(a: Optional[np.ndarray]):
@@ -121,6 +138,12 @@ class OraclePlan:
     def trace_row(self, seg, view, ax, tang):
         """Row traced directly for this bin (no symmetry)."""
         return self._row(lib().or_trace_row, seg, view, ax, tang)
+
+    def symmetry_op(self, seg, view, ax, tang):
+        """((basic seg, view, ax, tang), (in-plane code, z sign, z add))."""
+        b = np.zeros(4, np.int32); o = np.zeros(3, np.int32)
+        lib().or_symmetry_op(self._p, seg, view, ax, tang, b, o)
+        return tuple(int(v) for v in b), tuple(int(v) for v in o)
 
     def bin_geometry(self, seg, view, ax, tang):
         out = np.zeros(5, np.float32)

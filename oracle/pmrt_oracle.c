@@ -711,6 +711,24 @@ int or_trace_row(const or_plan* p, int seg, int view, int ax, int tang,
   return n;
 }
 
+int or_ray_trace(const float* st, const float* sp, const float* vox, float nc,
+                 int16_t* z, int16_t* y, int16_t* x, float* w, int cap)
+{
+  row r = {0};
+  ray_trace(&r, st[0], st[1], st[2], sp[0], sp[1], sp[2], vox[0], vox[1], vox[2], nc);
+  const int n = export_row(&r, z, y, x, w, cap);
+  row_free(&r);
+  return n;
+}
+
+void or_symmetry_op(const or_plan* p, int seg, int view, int ax, int tang, int32_t* basic4, int32_t* op3)
+{
+  int bs = seg, bv = view, ba = ax, bt = tang;
+  const symop o = sym_op_for_bin(p, &bs, &bv, &ba, &bt);
+  basic4[0] = bs; basic4[1] = bv; basic4[2] = ba; basic4[3] = bt;
+  op3[0] = o.inplane; op3[1] = o.zsign; op3[2] = o.zadd;
+}
+
 void or_bin_geometry(const or_plan* p, int seg, int view, int ax, int tang, float* out)
 {
   out[0] = get_s(p, tang);

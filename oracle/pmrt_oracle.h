@@ -98,6 +98,14 @@ int or_get_row(const or_plan* p, int seg, int view, int ax, int tang,
  * calculate_proj_matrix_elems_for_one_bin (PMRT.cxx:582-829). */
 int or_trace_row(const or_plan* p, int seg, int view, int ax, int tang,
                  int16_t* z, int16_t* y, int16_t* x, float* w, int cap);
+/* RayTraceVoxelsOnCartesianGrid on one segment (voxel units, coordinates
+ * z,y,x), for closed-form and brute-force checks. Same return convention. */
+int or_ray_trace(const float* start_zyx, const float* stop_zyx, const float* vox_zyx, float nc,
+                 int16_t* z, int16_t* y, int16_t* x, float* w, int cap);
+/* Basic bin (seg, view, ax, tang) and symmetry operation of a bin:
+ * op = (in-plane code, z sign, z add) with z' = zsign*z + zadd and the in-plane
+ * codes 0 id, 1 (-x,y), 2 (x,-y), 3 (-x,-y), 4 (y,x), 5 (-y,x), 6 (y,-x), 7 (-y,-x). */
+void or_symmetry_op(const or_plan* p, int seg, int view, int ax, int tang, int32_t* basic4, int32_t* op3);
 /* Per-bin geometry (s, phi, tantheta, costheta, t) as float[5]. */
 void or_bin_geometry(const or_plan* p, int seg, int view, int ax, int tang, float* out5);
 

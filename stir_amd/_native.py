@@ -24,6 +24,7 @@ EXPORTED_SYMBOLS = (
     "mi_back_project_dev", "mi_obj_set_data", "mi_obj_compute_sensitivities", "mi_obj_get_sensitivity",
     "mi_obj_set_estimate", "mi_obj_get_estimate", "mi_osem_subset_step", "mi_osem_run", "mi_obj_subset_gradient",
     "mi_obj_log_likelihood", "mi_synchronize", "mi_set_kernel_timing", "mi_get_kernel_timing",
+    "mi_symmetry_operations",
 )
 
 
@@ -88,6 +89,7 @@ def load():
     L.mi_obj_subset_gradient.argtypes = [vp, c_int, c_int, _f32p]
     L.mi_obj_log_likelihood.argtypes = [vp, c_int, ctypes.POINTER(ctypes.c_double)]
     L.mi_synchronize.argtypes = [vp]
+    L.mi_symmetry_operations.argtypes = [ctypes.POINTER(GeometryStruct), c_int, _i32p, _i32p, _i32p]
     L.mi_set_kernel_timing.argtypes = [vp, c_int]
     L.mi_get_kernel_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
@@ -152,3 +154,13 @@ def float_ptr_or_none(a: Optional[np.ndarray]):
         return None, None
     a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
     return a.ctypes.data_as(ctypes.c_void_p), a
+
+
+def symmetry_operations(geometry: GeometryStruct, bins: np.ndarray):
+    """Host-side (no device) basic bins and symmetry operations of bins (n x 4)."""
+    bins = np.ascontiguousarray(bins, dtype=np.int32).reshape(-1, 4)
+    basic = np.zeros_like(bins)
+    ops = np.zeros((bins.shape[0], 3), np.int32)
+    check(load().mi_symmetry_operations(ctypes.byref(geometry), bins.shape[0], bins.reshape(-1), basic.reshape(-1),
+                                        ops.reshape(-1)), "mi_symmetry_operations")
+    return basic, ops

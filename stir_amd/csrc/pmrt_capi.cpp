@@ -557,6 +557,23 @@ int mi_subset_view_segments(mi_ctx* ctx, int subset, int32_t* views, int32_t* se
   });
 }
 
+int mi_symmetry_operations(const mi_geometry* geometry, int n, const int32_t* bins, int32_t* basic, int32_t* ops) {
+  return guarded([&] {
+    if (!geometry || (n > 0 && (!bins || !basic || !ops))) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    mi::Plan plan(*geometry, 1);
+    const mi_geometry& g = plan.geometry();
+    for (int i = 0; i < n; ++i) {
+      int seg = bins[4 * i], view = bins[4 * i + 1], ax = bins[4 * i + 2], tang = bins[4 * i + 3];
+      if (seg < g.min_segment_num || seg > g.max_segment_num || view < 0 || view >= g.num_views)
+        throw ApiError(MI_ERR_ARGUMENT, "bin out of range");
+      const mi::SymOp op = plan.symmetry_op(seg, view, ax, tang);
+      plan.basic_bin(seg, view, ax, tang);
+      basic[4 * i] = seg; basic[4 * i + 1] = view; basic[4 * i + 2] = ax; basic[4 * i + 3] = tang;
+      ops[3 * i] = op.inplane; ops[3 * i + 1] = op.zsign; ops[3 * i + 2] = op.zadd;
+    }
+  });
+}
+
 int mi_get_row(mi_ctx* ctx, int segment, int view, int axial_pos, int tang_pos, int16_t* z, int16_t* y, int16_t* x,
                float* w, int cap, int* n) {
   return guarded([&] {
