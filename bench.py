@@ -35,8 +35,12 @@ def main():
     ap.add_argument("--osem-iterations", type=int, default=1)
     ap.add_argument("--skip-osem", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-autograd", action="store_true")
+    ap.add_argument("--autograd-steps", type=int, default=3)
     ap.add_argument("--cpu-subsets", type=int, default=1, help="subsets of 21 in the CPU baseline sample")
     ap.add_argument("--radius", type=float, default=328.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device copy first (a kernel of known traffic for rocprof counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -52,6 +56,12 @@ def main():
                                      ProjectorByBinPairMI355X, ProjMatrixByBinUsingRayTracing)
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(64 * 2**20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     pdi, img = config1(args.radius)
     pm = RayTracingMatrixParameters()
     nsub = 21
@@ -151,14 +161,18 @@ def main():
         "setup_s": round(setup_s, 3),
     }
 
-    if not args.skip_osem:
+    obj = None
+    if not (args.skip_osem and args.skip_autograd):
         fwd = pair.forward_project(x).cpu().numpy()
         y, scale = phantoms.poisson_data(fwd, 2.0, 0.05, seed=101)
+        del fwd
         obj = PoissonLogLikelihoodWithLinearModelForMeanAndProjData(pair)
         t0 = time.time()
         obj.set_up(y, 0.05)
         torch.cuda.synchronize()
         sens_s = time.time() - t0
+
+    if not args.skip_osem:
         recon = OSMAPOSLReconstruction(obj, num_subiterations=nsub)
         obj.set_estimate(np.ones(img.shape, np.float32), enforce_positivity=True)
         # one warm-up subiteration, then timed full iterations
@@ -178,9 +192,51 @@ def main():
                           "subsets": nsub, "sensitivity_s": round(sens_s, 3)}
         log(f"OSEM iteration {it_s * 1000:.2f} ms")
 
+    if not args.skip_autograd:
+        result["autograd"] = autograd_bench(pair, obj, img, pdi, dev, args.autograd_steps, ms_per_step)
+
     if not args.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(pdi, img, pm, x_true, nsub, args.cpu_subsets)
     print(json.dumps(result), flush=True)
+
+
+def autograd_bench(pair, obj, img, pdi, dev, steps, pair_ms):
+    """Gradient of NLL + 0.01 * QuadraticPrior for a batch of 2 log-normal(0, 0.5)
+    images through the autograd layer, plus the FP64 adjoint identity."""
+    import numpy as np
+    import torch
+
+    from stir_amd.autograd import PoissonLogLikelihoodLoss, QuadraticPriorLoss, back_project, forward_project
+
+    rng = np.random.default_rng(202)
+    xs = torch.from_numpy(rng.lognormal(0.0, 0.5, (2,) + tuple(img.shape)).astype(np.float32)).to(dev)
+    nll = PoissonLogLikelihoodLoss(obj)
+    prior = QuadraticPriorLoss(pair, 0.01)
+
+    def grad_step():
+        xg = xs.detach().requires_grad_(True)
+        (nll(xg) + prior(xg)).sum().backward()
+        return xg.grad
+
+    grad_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        grad_step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    per_image_ms = dt * 1000 / xs.shape[0]
+    log(f"autograd: {per_image_ms:.3f} ms per image gradient ({per_image_ms / pair_ms:.3f} x a pair)")
+    g = torch.Generator(device=dev).manual_seed(7)
+    xa = torch.rand(tuple(img.shape), generator=g, device=dev)
+    pa = torch.rand(pdi.num_bins, generator=g, device=dev)
+    with torch.no_grad():
+        lhs = torch.dot(forward_project(xa, pair).double(), pa.double()).item()
+        rhs = torch.dot(xa.reshape(-1).double(), back_project(pa, pair).reshape(-1).double()).item()
+    return {"grad_per_s": round(1000.0 / per_image_ms, 4), "ms_per_gradient": round(per_image_ms, 3),
+            "gradient_vs_pair": round(per_image_ms / pair_ms, 4), "batch": int(xs.shape[0]),
+            "loss": "Poisson NLL (mean 2/bin, b=0.05) + 0.01 * QuadraticPrior, log-normal(0, 0.5) images",
+            "adjoint_rel_err_fp64": abs(lhs - rhs) / abs(lhs)}
 
 
 def cpu_baseline(pdi, img, pm, x_true, nsub, nsample):

@@ -14,9 +14,12 @@
  * mi_last_error() returns the message of the last failure on the calling
  * thread. No C++ exception crosses this boundary.
  *
- * Threading: a context is used by one host thread at a time. All GPU work of
- * a context is issued on one HIP stream (the context's own stream, or the
- * stream passed to the *_dev entry points).
+ * Threading: a context is used by one host thread at a time. The entry
+ * points with host buffers use the context's own stream and return when
+ * their results are on the host; the *_dev entry points run on the stream
+ * the caller passes (NULL = the HIP null stream) and share the context's
+ * device scratch buffers, so their calls must be ordered (one stream, or
+ * synchronised).
  */
 #ifndef STIR_AMD_PMRT_H
 #define STIR_AMD_PMRT_H
@@ -29,6 +32,7 @@ extern "C" {
 #endif
 
 #define MI_MAX_SEGMENTS 256
+#define MI_MAX_TOF_BINS 64
 
 enum {
   MI_OK = 0,
@@ -71,6 +75,16 @@ typedef struct {
   int32_t do_symmetry_swap_segment;
   int32_t do_symmetry_swap_s;
   int32_t do_symmetry_shift_z;
+  /* Time of flight (ProjDataInfo::set_tof_mash_factor, ProjDataInfo.cxx:174-258;
+   * ProjMatrixByBin::enable_tof, ProjMatrixByBin.cxx:66-75). num_tof_bins <= 1
+   * means non-TOF data. TOF bin k (0-based: timing_pos_num - min_tof_pos_num)
+   * covers [tof_low[k], tof_high[k]] mm along the LOR; tof_r_sqrt2_gauss_sigma
+   * = 1 / (sigma * sqrt(2)), sigma in mm. TOF projection data are
+   * num_tof_bins non-TOF blocks, TOF bin outermost. */
+  int32_t num_tof_bins;
+  float tof_r_sqrt2_gauss_sigma;
+  float tof_low[MI_MAX_TOF_BINS];
+  float tof_high[MI_MAX_TOF_BINS];
 } mi_geometry;
 
 /* Sizes of a set-up context. */
@@ -143,7 +157,9 @@ int mi_back_project(mi_ctx* ctx, const float* proj_data, float* image, int subse
 
 /* ---- projections, device buffers (STIR layouts, device pointers) --------- */
 /* Same operations on device-resident arrays, issued on `stream` (a
- * hipStream_t; NULL = the context's stream). Used by the PyTorch layer. */
+ * hipStream_t; NULL is the HIP null stream, as for any HIP library call).
+ * They are stream-ordered with the caller's other work on that stream and
+ * return without waiting. Used by the PyTorch layer. */
 int mi_forward_project_dev(mi_ctx* ctx, const float* d_image, float* d_proj_data, int subset, void* stream);
 int mi_back_project_dev(mi_ctx* ctx, const float* d_proj_data, float* d_image, int subset, void* stream);
 
@@ -181,8 +197,58 @@ int mi_obj_subset_gradient(mi_ctx* ctx, int subset, int add_sensitivity, float* 
  * current estimate, accumulated in FP64. subset < 0: all subsets. */
 int mi_obj_log_likelihood(mi_ctx* ctx, int subset, double* value);
 
+/* ---- differentiable pieces, device buffers (STIR layouts) ----------------- */
+/* Calls on one context share its device scratch buffers: issue them on one
+ * stream (or serialise them); calls on different streams that overlap race. */
+
+/* Log-likelihood and its sub-gradient at a device image in one forward
+ * projection (FP64 partial sums) plus one back projection:
+ *   *d_value (device double, if not NULL) =
+ *     actual_compute_objective_function_without_penalty (...AndProjData.cxx:739-781,
+ *     recon_array_functions.cxx:336-377),
+ *   d_gradient (device image, if not NULL) =
+ *     compute_sub_gradient_without_penalty[_plus_sensitivity] (...AndProjData.cxx:694-735,
+ *     RPC at :1423-1481).
+ * Uses the data of mi_obj_set_data; subset < 0: all subsets. Nothing is
+ * copied to the host. */
+int mi_obj_value_and_gradient_dev(mi_ctx* ctx, const float* d_image, int subset, int add_sensitivity,
+                                  float* d_gradient, double* d_value, void* stream);
+
+/* QuadraticPrior::compute_value and compute_gradient (QuadraticPrior.cxx:229-367)
+ * of a device image with the context's image geometry. weights: 27 host
+ * floats [dz+1][dy+1][dx+1], or NULL for the default vox_x / distance
+ * (compute_weights, :202-226); d_kappa: device image or NULL; only_2D: no
+ * neighbours in other planes. *d_value (device double) and d_gradient
+ * (device image) are each optional. */
+int mi_quadratic_prior_dev(mi_ctx* ctx, const float* d_image, const float* d_kappa, const float* weights, float beta,
+                           int only_2D, float* d_gradient, double* d_value, void* stream);
+
+/* One OSMAPOSL subiteration on device images, out of place:
+ * d_x_out = d_x_in * u, u = clamp(BP_s(y/(FP_s(x)+b)) / sens_s)
+ * (OSMAPOSLReconstruction::update_estimate, OSMAPOSLReconstruction.cxx:373-516,
+ * as mi_osem_subset_step); d_update (if not NULL) receives u. Needs
+ * mi_obj_set_data and mi_obj_compute_sensitivities. d_x_out may equal d_x_in. */
+int mi_osem_subset_step_dev(mi_ctx* ctx, const float* d_x_in, float* d_x_out, float* d_update, int subset,
+                            int subiteration_num, void* stream);
+/* No counterpart in the reference (it has no derivatives of the OSEM map):
+ * vector-Jacobian product of mi_osem_subset_step_dev with respect to x_in,
+ *   d_grad_in = g * u + BP_s( r'(FP_s(x_in)+b) * FP_s(g * x_in / sens_s) ),
+ * r' the derivative of the truncated quotient y/ybar, g = d_grad_out, u the
+ * update written by the forward step. Two forward and one back projection. */
+int mi_osem_subset_step_vjp_dev(mi_ctx* ctx, const float* d_x_in, const float* d_update, const float* d_grad_out,
+                                float* d_grad_in, int subset, void* stream);
+
 /* Waits for all work of the context. */
 int mi_synchronize(mi_ctx* ctx);
+
+/* ---- launch tuning ---------------------------------------------------------- */
+/* No counterpart in the reference. Launch shape of the back projection: image
+ * columns are visited in tiles of MI_TUNE_BP_TILE_X x MI_TUNE_BP_TILE_Y
+ * (default 1 x 1, raster order), with or without the XCD-contiguous block
+ * remap (default on), and with or without skipping z slots an entry cannot
+ * reach (default off). Results do not depend on these settings. */
+enum { MI_TUNE_BP_TILE_X = 1, MI_TUNE_BP_TILE_Y = 2, MI_TUNE_BP_XCD_REMAP = 3, MI_TUNE_BP_SLOT_SKIP = 4 };
+int mi_set_tuning(mi_ctx* ctx, int param, int value);
 
 /* ---- diagnostics ----------------------------------------------------------- */
 /* No counterpart in the reference (its TimedObject timers are CPU/wall clocks,

@@ -47,6 +47,8 @@ def lib():
         for fn in (L.or_get_row, L.or_trace_row):
             fn.restype = ctypes.c_int
             fn.argtypes = [vp] + [ctypes.c_int] * 4 + [_i16p, _i16p, _i16p, _f32p, ctypes.c_int]
+        L.or_get_row_tof.restype = ctypes.c_int
+        L.or_get_row_tof.argtypes = [vp] + [ctypes.c_int] * 5 + [_i16p, _i16p, _i16p, _f32p, ctypes.c_int]
         L.or_bin_geometry.argtypes = [vp] + [ctypes.c_int] * 4 + [_f32p]
         L.or_ray_trace.restype = ctypes.c_int
         L.or_ray_trace.argtypes = [_f32p, _f32p, _f32p, ctypes.c_float, _i16p, _i16p, _i16p, _f32p, ctypes.c_int]
@@ -68,8 +70,31 @@ def lib():
         L.or_log_likelihood.argtypes = [vp, _f32p, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_osem.argtypes = [vp, _f32p, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int]
+        L.or_quadratic_prior_weights.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, _f32p]
+        L.or_quadratic_prior.restype = ctypes.c_double
+        L.or_quadratic_prior.argtypes = [_f32p, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int,
+                                         ctypes.c_float, vp]
         _lib = L
     return _lib
+
+
+def quadratic_prior_weights(vox_zyx) -> np.ndarray:
+    """QuadraticPrior weights [3][3][3] = vox_x/|offset| (0 at the centre)."""
+    w = np.empty(27, np.float32)
+    lib().or_quadratic_prior_weights(float(vox_zyx[0]), float(vox_zyx[1]), float(vox_zyx[2]), w)
+    return w.reshape(3, 3, 3)
+
+
+def quadratic_prior(image, vox_zyx, beta, kappa=None, only_2D=False, gradient=True):
+    """QuadraticPrior value (float) and gradient (float32 image, or None)."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    nz, ny, nx = img.shape
+    w = quadratic_prior_weights(vox_zyx).reshape(-1).copy()
+    kp = _opt_ptr(None if kappa is None else np.ascontiguousarray(kappa, np.float32).reshape(img.shape))
+    g = np.zeros(img.shape, np.float32) if gradient else None
+    v = lib().or_quadratic_prior(img.reshape(-1), kp[0] if kp else None, nz, ny, nx, w, int(only_2D), float(beta),
+                                 g.ctypes.data_as(ctypes.c_void_p) if g is not None else None)
+    return float(v), g
 
 
 def ray_trace(start_zyx, stop_zyx, vox_zyx, nc):
@@ -134,6 +159,17 @@ class OraclePlan:
     def row(self, seg, view, ax, tang):
         """Row of a bin via basic bin + symmetry operation: (z, y, x, w)."""
         return self._row(lib().or_get_row, seg, view, ax, tang)
+
+    def row_tof(self, seg, view, ax, tang, tof):
+        """TOF row of a bin for TOF bin `tof` (0-based): (z, y, x, w)."""
+        cap = 4096
+        while True:
+            z = np.empty(cap, np.int16); y = np.empty(cap, np.int16); x = np.empty(cap, np.int16)
+            w = np.empty(cap, np.float32)
+            n = lib().or_get_row_tof(self._p, seg, view, ax, tang, tof, z, y, x, w, cap)
+            if n >= 0:
+                return z[:n].copy(), y[:n].copy(), x[:n].copy(), w[:n].copy()
+            cap = -n
 
     def trace_row(self, seg, view, ax, tang):
         """Row traced directly for this bin (no symmetry)."""

@@ -56,7 +56,9 @@ static int is_half_integer(float a) { return fabsf(floorf(a) + .5F - a) < .0001F
 /* row container (ProjMatrixElemsForOneBin)                           */
 /* ------------------------------------------------------------------ */
 typedef struct { int16_t z, y, x; float w; } elem;
-typedef struct { elem* e; int n, cap; } row;
+/* tw: TOF weights of a basic row, n x num_tof_bins (element value times TOF
+ * kernel value, 0 where the reference drops the element); NULL for non-TOF */
+typedef struct { elem* e; int n, cap; float* tw; } row;
 
 static void row_reserve(row* r, int cap)
 {
@@ -72,7 +74,7 @@ static void row_push(row* r, int z, int y, int x, float w)
   r->e[r->n].z = (int16_t)z; r->e[r->n].y = (int16_t)y; r->e[r->n].x = (int16_t)x; r->e[r->n].w = w;
   r->n++;
 }
-static void row_free(row* r) { free(r->e); r->e = NULL; r->n = r->cap = 0; }
+static void row_free(row* r) { free(r->e); free(r->tw); r->e = NULL; r->tw = NULL; r->n = r->cap = 0; }
 static void row_copy(row* dst, const row* src)
 {
   row_reserve(dst, src->n);
@@ -142,6 +144,10 @@ struct or_plan {
    * may lie outside the data's range (e.g. -n/2 -> +n/2) and is still traced */
   int tmax;
   row* basic;
+  /* time of flight */
+  int ntof;
+  float z_offset;  /* scanner z of image plane 0 subtracted in apply_tof_kernel */
+  double* erf_tab;  /* FastErf samples */
 };
 
 #define SEGI(p, s) ((s) - (p)->g.min_segment_num)
@@ -526,6 +532,109 @@ static int related_vs(const or_plan* p, int view, int seg, int* rv, int* rs)
 /* ------------------------------------------------------------------ */
 /* plan creation                                                       */
 /* ------------------------------------------------------------------ */
+/* ------------------------------------------------------------------ */
+/* time of flight                                                       */
+/* ------------------------------------------------------------------ */
+/* FastErf (FastErf.inl:30-80): erf sampled at 200000+2 points on [-5, 5],
+ * linear interpolation; ProjMatrixByBin.cxx:174-176 sets 200000 samples and
+ * the maximum sample value keeps its default 5 (FastErf.h:57). */
+#define OR_ERF_SAMPLES 200000
+#define OR_ERF_MAX 5.0
+static double erf_period(void) { return (2 * OR_ERF_MAX) / OR_ERF_SAMPLES; }
+static double* erf_table(void)
+{
+  double* t = (double*)malloc(sizeof(double) * (OR_ERF_SAMPLES + 2));
+  const double per = erf_period();
+  for (int i = 0; i < OR_ERF_SAMPLES + 2; ++i) t[i] = erf(i * per - OR_ERF_MAX);
+  return t;
+}
+static double erf_interp(const double* tab, double xp)
+{
+  xp = fmax(fmin(OR_ERF_MAX, xp), -OR_ERF_MAX);
+  const double xi = (xp + OR_ERF_MAX) / erf_period();
+  const int lo = (int)floor(xi);
+  return tab[lo] + (xi - lo) * (tab[lo + 1] - tab[lo]);
+}
+/* ProjMatrixByBin::get_tof_value (ProjMatrixByBin.inl:186-195) */
+static float tof_value(const or_plan* p, float d1, float d2)
+{
+  const float r = p->g.tof_r_sqrt2_gauss_sigma;
+  const float d1_n = d1 * r, d2_n = d2 * r;
+  if ((d1_n >= 4.F && d2_n >= 4.F) || (d1_n <= -4.F && d2_n <= -4.F)) return 0.F;
+  return (float)(0.5 * (erf_interp(p->erf_tab, d2_n) - erf_interp(p->erf_tab, d1_n)));
+}
+/* to_0_2pi for floats: modulo(float, float) (modulo.h:62-76,136-141) */
+static float to_0_2pi_f(float phi)
+{
+  const float b = (float)(2 * OR_PI);
+  double r = fmod((double)phi, (double)b);
+  r = r < 0 ? r + fabs((double)b) : r;
+  float res = (float)r;
+  if (res >= b) res -= b;
+  return res;
+}
+/* Mid-point and unit direction (z, y, x) of a bin's LOR as apply_tof_kernel
+ * forms them (ProjMatrixByBin.inl:117-137): get_LOR
+ * (ProjDataInfoCylindrical.cxx:515-547), LORInAxialAndNoArcCorrSinogramCoordinates
+ * (LORCoordinates.inl:106-121), LORInCylinderCoordinates and LORAs2Points
+ * (:128-139,287-298). Float arithmetic; asin/cos/sin are the float overloads
+ * (parity at the last bit unpinned: the overload the reference picks depends
+ * on its header inclusion order). */
+static void tof_lor(const or_plan* p, int seg, int view, int ax, int tang, float mid[3], float unit[3])
+{
+  const float s = get_s(p, tang), m = get_m(p, seg, ax), tt = get_tantheta(p, seg, tang), phi = get_phi(p, view);
+  const float R = p->radius;
+  const float max_a = sqrtf(R * R - s * s), min_a = -max_a;
+  float z1 = m - max_a * tt, z2 = m - min_a * tt;
+  float ph = to_0_2pi_f(phi), beta = asinf(s / R);
+  int swapped = 0;
+  if ((double)ph >= OR_PI) {
+    ph -= (float)OR_PI;
+    beta = -beta;
+    const float tz = z1; z1 = z2; z2 = tz;
+    swapped = 1;
+  }
+  const float psi1 = to_0_2pi_f(ph + beta), psi2 = to_0_2pi_f(ph - beta + (float)OR_PI);
+  float p1[3] = {z1, -R * cosf(psi1), R * sinf(psi1)};
+  float p2[3] = {z2, -R * cosf(psi2), R * sinf(psi2)};
+  if (swapped)
+    for (int i = 0; i < 3; ++i) { const float t = p1[i]; p1[i] = p2[i]; p2[i] = t; }
+  float diff[3];
+  for (int i = 0; i < 3; ++i) {
+    mid[i] = (p1[i] + p2[i]) * 0.5F;
+    diff[i] = p2[i] - mid[i];
+  }
+  const float n = (float)norm3(diff[0], diff[1], diff[2]);
+  for (int i = 0; i < 3; ++i) unit[i] = diff[i] / n;
+}
+/* apply_tof_kernel (ProjMatrixByBin.inl:115-181) on a basic row: TOF weights
+ * value * kernel for every TOF bin, 0 where the kernel value is not positive. */
+static void tof_weights(const or_plan* p, int seg, int view, int tang, row* r)
+{
+  const int nt = p->ntof;
+  r->tw = (float*)malloc(sizeof(float) * (size_t)(r->n > 0 ? r->n : 1) * nt);
+  float mid[3], unit[3];
+  tof_lor(p, seg, view, 0, tang, mid, unit);
+  for (int e = 0; e < r->n; ++e) {
+    /* get_physical_coordinates_for_indices (DiscretisedDensity.inl:175-179,
+     * DiscretisedDensityOnCartesianGrid.inl:104-112), minus the scanner offset */
+    const float vz = (p->g.vox_z * (float)r->e[e].z + p->g.origin_z) - p->z_offset;
+    const float vy = (p->g.vox_y * (float)r->e[e].y + p->g.origin_y) - 0.F;
+    const float vx = (p->g.vox_x * (float)r->e[e].x + p->g.origin_x) - 0.F;
+    float ip = 0.F;
+    ip += (vz - mid[0]) * unit[0];
+    ip += (vy - mid[1]) * unit[1];
+    ip += (vx - mid[2]) * unit[2];
+    const float d2 = -ip;
+    for (int k = 0; k < nt; ++k) {
+      const float v = tof_value(p, p->g.tof_low[k] - d2, p->g.tof_high[k] - d2);
+      float w = 0.F;
+      if (v > 0 && r->e[e].w != 0) w = r->e[e].w * v;
+      r->tw[(size_t)e * nt + k] = w;
+    }
+  }
+}
+
 static size_t basic_index(const or_plan* p, int seg, int view, int tang)
 {
   return ((size_t)SEGI(p, seg) * p->g.num_views + view) * (size_t)(2 * p->tmax + 1) + (size_t)(tang + p->tmax);
@@ -591,6 +700,15 @@ or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
   p->symseg = g->do_symmetry_swap_segment;
   p->syms = g->do_symmetry_swap_s;
   p->symz = g->do_symmetry_shift_z;
+  p->ntof = g->num_tof_bins > 1 ? g->num_tof_bins : 1;
+  if (p->ntof > OR_MAX_TOF) { snprintf(err, errlen, "too many TOF bins"); free(p); return NULL; }
+  if (p->ntof > 1) {
+    /* TOF keeps only the z-shift symmetry (DataSymmetriesForBins_PET_CartesianGrid.cxx:345-366) */
+    p->sym90 = p->sym180 = p->symseg = p->syms = 0;
+    /* ProjMatrixByBin.inl:128-131 */
+    p->z_offset = (g->num_rings - 1) / 2.0F * g->ring_spacing;
+    p->erf_tab = erf_table();
+  }
   {
     const float zo = g->origin_z / g->vox_z;
     if (fabsf(round_f(zo) - zo) > 1.E-3F) { snprintf(err, errlen, "z origin not a multiple of plane spacing"); free(p); return NULL; }
@@ -644,6 +762,15 @@ or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
     or_plan_destroy(p);
     return NULL;
   }
+  if (p->ntof > 1) {
+#pragma omp parallel for schedule(dynamic) collapse(2)
+    for (int si = 0; si < p->nseg; ++si)
+      for (int v = 0; v < g->num_views; ++v)
+        for (int t = -p->tmax; t <= p->tmax; ++t) {
+          row* r = &p->basic[basic_index(p, si + g->min_segment_num, v, t)];
+          if (r->e || r->n) tof_weights(p, si + g->min_segment_num, v, t, r);
+        }
+  }
   return p;
 }
 
@@ -655,6 +782,7 @@ void or_plan_destroy(or_plan* p)
     for (size_t i = 0; i < nb; ++i) row_free(&p->basic[i]);
     free(p->basic);
   }
+  free(p->erf_tab);
   free(p);
 }
 
@@ -663,7 +791,7 @@ void or_symmetry_flags(const or_plan* p, int32_t* out)
   out[0] = p->sym90; out[1] = p->sym180; out[2] = p->symseg; out[3] = p->syms; out[4] = p->symz;
 }
 
-int64_t or_num_bins(const or_plan* p) { return p->num_bins; }
+int64_t or_num_bins(const or_plan* p) { return p->num_bins * p->ntof; }
 
 int64_t or_bin_index(const or_plan* p, int seg, int view, int ax, int tang)
 {
@@ -678,6 +806,17 @@ static const row* row_and_op(const or_plan* p, int seg, int view, int ax, int ta
   int bs = seg, bv = view, ba = ax, bt = tang;
   *op = sym_op_for_bin(p, &bs, &bv, &ba, &bt);
   return &p->basic[basic_index(p, bs, bv, bt)];
+}
+
+/* weight of element k of a basic row for TOF bin kk (the plain value for non-TOF) */
+static float elem_w(const or_plan* p, const row* b, int k, int kk)
+{
+  return p->ntof > 1 ? b->tw[(size_t)k * p->ntof + kk] : b->e[k].w;
+}
+/* index of a bin in (TOF) projection data: TOF bin outermost */
+static int64_t tbin(const or_plan* p, int seg, int view, int ax, int tang, int kk)
+{
+  return (int64_t)kk * p->num_bins + or_bin_index(p, seg, view, ax, tang);
 }
 
 static int export_row(const row* r, int16_t* z, int16_t* y, int16_t* x, float* w, int cap)
@@ -699,6 +838,30 @@ int or_get_row(const or_plan* p, int seg, int view, int ax, int tang,
     z[k] = (int16_t)oz; y[k] = (int16_t)oy; x[k] = (int16_t)ox; w[k] = b->e[k].w;
   }
   return b->n;
+}
+
+/* TOF row of a bin (ProjMatrixByBin::get_proj_matrix_elems_for_one_bin with
+ * apply_tof_kernel on the basic bin, ProjMatrixByBin.inl:48-113,115-181):
+ * elements of the basic row whose TOF weight is non-zero, transformed. */
+int or_get_row_tof(const or_plan* p, int seg, int view, int ax, int tang, int tof,
+                   int16_t* z, int16_t* y, int16_t* x, float* w, int cap)
+{
+  if (p->ntof <= 1 || tof < 0 || tof >= p->ntof) return 0;
+  symop op;
+  const row* b = row_and_op(p, seg, view, ax, tang, &op);
+  int n = 0;
+  for (int k = 0; k < b->n; ++k) if (elem_w(p, b, k, tof) != 0.F) ++n;
+  if (n > cap) return -n;
+  n = 0;
+  for (int k = 0; k < b->n; ++k) {
+    const float wt = elem_w(p, b, k, tof);
+    if (wt == 0.F) continue;
+    int oz, oy, ox;
+    apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &oz, &oy, &ox);
+    z[n] = (int16_t)oz; y[n] = (int16_t)oy; x[n] = (int16_t)ox; w[n] = wt;
+    ++n;
+  }
+  return n;
 }
 
 int or_trace_row(const or_plan* p, int seg, int view, int ax, int tang,
@@ -819,7 +982,7 @@ static size_t img_index(const or_plan* p, int z, int y, int x)
 }
 
 /* one bin FP: ProjMatrixElemsForOneBin::forward_project -- .cxx:354-371 */
-static float fp_bin(const or_plan* p, const float* img, int seg, int view, int ax, int tang)
+static float fp_bin(const or_plan* p, const float* img, int seg, int view, int ax, int tang, int kk)
 {
   symop op;
   const row* b = row_and_op(p, seg, view, ax, tang, &op);
@@ -827,11 +990,11 @@ static float fp_bin(const or_plan* p, const float* img, int seg, int view, int a
   for (int k = 0; k < b->n; ++k) {
     int z, y, x;
     apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &z, &y, &x);
-    if (z >= p->g.min_z && z <= p->max_z) acc += img[img_index(p, z, y, x)] * b->e[k].w;
+    if (z >= p->g.min_z && z <= p->max_z) acc += img[img_index(p, z, y, x)] * elem_w(p, b, k, kk);
   }
   return acc;
 }
-static double fp_bin64(const or_plan* p, const double* img, int seg, int view, int ax, int tang)
+static double fp_bin64(const or_plan* p, const double* img, int seg, int view, int ax, int tang, int kk)
 {
   symop op;
   const row* b = row_and_op(p, seg, view, ax, tang, &op);
@@ -839,7 +1002,7 @@ static double fp_bin64(const or_plan* p, const double* img, int seg, int view, i
   for (int k = 0; k < b->n; ++k) {
     int z, y, x;
     apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &z, &y, &x);
-    if (z >= p->g.min_z && z <= p->max_z) acc += img[img_index(p, z, y, x)] * (double)b->e[k].w;
+    if (z >= p->g.min_z && z <= p->max_z) acc += img[img_index(p, z, y, x)] * (double)elem_w(p, b, k, kk);
   }
   return acc;
 }
@@ -851,7 +1014,8 @@ void or_forward_project(const or_plan* p, const float* img, float* sino, int sub
     const int seg = vs[k].seg, view = vs[k].view;
     const int nax = p->g.num_axial_poss[SEGI(p, seg)];
     for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
-      for (int a = 0; a < nax; ++a) sino[or_bin_index(p, seg, view, a, t)] = fp_bin(p, img, seg, view, a, t);
+      for (int a = 0; a < nax; ++a)
+        for (int kk = 0; kk < p->ntof; ++kk) sino[tbin(p, seg, view, a, t, kk)] = fp_bin(p, img, seg, view, a, t, kk);
   }
   free(vs);
 }
@@ -864,7 +1028,8 @@ void or_forward_project_omp(const or_plan* p, const float* img, float* sino, int
     const int seg = vs[k].seg, view = vs[k].view;
     const int nax = p->g.num_axial_poss[SEGI(p, seg)];
     for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
-      for (int a = 0; a < nax; ++a) sino[or_bin_index(p, seg, view, a, t)] = fp_bin(p, img, seg, view, a, t);
+      for (int a = 0; a < nax; ++a)
+        for (int kk = 0; kk < p->ntof; ++kk) sino[tbin(p, seg, view, a, t, kk)] = fp_bin(p, img, seg, view, a, t, kk);
   }
   free(vs);
 }
@@ -877,13 +1042,14 @@ void or_forward_project_f64(const or_plan* p, const double* img, double* sino, i
     const int seg = vs[k].seg, view = vs[k].view;
     const int nax = p->g.num_axial_poss[SEGI(p, seg)];
     for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
-      for (int a = 0; a < nax; ++a) sino[or_bin_index(p, seg, view, a, t)] = fp_bin64(p, img, seg, view, a, t);
+      for (int a = 0; a < nax; ++a)
+        for (int kk = 0; kk < p->ntof; ++kk) sino[tbin(p, seg, view, a, t, kk)] = fp_bin64(p, img, seg, view, a, t, kk);
   }
   free(vs);
 }
 
 /* ProjMatrixElemsForOneBin::back_project -- .cxx:333-352 (skips zero data) */
-static void bp_bin(const or_plan* p, float* img, float data, int seg, int view, int ax, int tang)
+static void bp_bin(const or_plan* p, float* img, float data, int seg, int view, int ax, int tang, int kk)
 {
   if (data == 0) return;
   symop op;
@@ -891,10 +1057,10 @@ static void bp_bin(const or_plan* p, float* img, float data, int seg, int view, 
   for (int k = 0; k < b->n; ++k) {
     int z, y, x;
     apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &z, &y, &x);
-    if (z >= p->g.min_z && z <= p->max_z) img[img_index(p, z, y, x)] += b->e[k].w * data;
+    if (z >= p->g.min_z && z <= p->max_z) img[img_index(p, z, y, x)] += elem_w(p, b, k, kk) * data;
   }
 }
-static void bp_bin64(const or_plan* p, double* img, double data, int seg, int view, int ax, int tang)
+static void bp_bin64(const or_plan* p, double* img, double data, int seg, int view, int ax, int tang, int kk)
 {
   if (data == 0) return;
   symop op;
@@ -902,7 +1068,7 @@ static void bp_bin64(const or_plan* p, double* img, double data, int seg, int vi
   for (int k = 0; k < b->n; ++k) {
     int z, y, x;
     apply_op(&op, b->e[k].z, b->e[k].y, b->e[k].x, &z, &y, &x);
-    if (z >= p->g.min_z && z <= p->max_z) img[img_index(p, z, y, x)] += (double)b->e[k].w * data;
+    if (z >= p->g.min_z && z <= p->max_z) img[img_index(p, z, y, x)] += (double)elem_w(p, b, k, kk) * data;
   }
 }
 
@@ -916,7 +1082,8 @@ void or_back_project(const or_plan* p, const float* sino, float* img, int subset
     const int seg = vs[k].seg, view = vs[k].view;
     const int nax = p->g.num_axial_poss[SEGI(p, seg)];
     for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
-      for (int a = 0; a < nax; ++a) bp_bin(p, img, sino[or_bin_index(p, seg, view, a, t)], seg, view, a, t);
+      for (int a = 0; a < nax; ++a)
+        for (int kk = 0; kk < p->ntof; ++kk) bp_bin(p, img, sino[tbin(p, seg, view, a, t, kk)], seg, view, a, t, kk);
   }
   free(vs);
 }
@@ -941,7 +1108,8 @@ void or_back_project_f64(const or_plan* p, const double* sino, double* img, int 
     const int nax = p->g.num_axial_poss[SEGI(p, seg)];
     for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
       for (int a = 0; a < nax; ++a)
-        bp_bin64(p, part + (size_t)tid * nv, sino[or_bin_index(p, seg, view, a, t)], seg, view, a, t);
+        for (int kk = 0; kk < p->ntof; ++kk)
+          bp_bin64(p, part + (size_t)tid * nv, sino[tbin(p, seg, view, a, t, kk)], seg, view, a, t, kk);
   }
   for (int t = 0; t < nt; ++t)
     for (size_t i = 0; i < nv; ++i) img[i] += part[(size_t)t * nv + i];
@@ -971,7 +1139,8 @@ void or_back_project_omp(const or_plan* p, const float* sino, float* img, int su
     const int nax = p->g.num_axial_poss[SEGI(p, seg)];
     for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
       for (int a = 0; a < nax; ++a)
-        bp_bin(p, part + (size_t)tid * nv, sino[or_bin_index(p, seg, view, a, t)], seg, view, a, t);
+        for (int kk = 0; kk < p->ntof; ++kk)
+          bp_bin(p, part + (size_t)tid * nv, sino[tbin(p, seg, view, a, t, kk)], seg, view, a, t, kk);
   }
   for (int t = 0; t < nt; ++t)
     for (size_t i = 0; i < nv; ++i) img[i] += part[(size_t)t * nv + i];
@@ -993,7 +1162,10 @@ int64_t or_nnz(const or_plan* p, int subset, int nsub)
         const row* b = row_and_op(p, seg, view, a, t, &op);
         for (int e = 0; e < b->n; ++e) {
           const int z = op.zsign * b->e[e].z + op.zadd;
-          if (z >= p->g.min_z && z <= p->max_z) total++;
+          if (z < p->g.min_z || z > p->max_z) continue;
+          if (p->ntof == 1) total++;
+          else
+            for (int kk = 0; kk < p->ntof; ++kk) total += elem_w(p, b, e, kk) != 0.F;
         }
       }
   }
@@ -1014,37 +1186,39 @@ static int is_end_plane(const or_plan* p, int seg, int ax)
  * (distributable.cxx:200-223). */
 void or_subset_sensitivity(const or_plan* p, int subset, int nsub, int zero_end_planes, float* img)
 {
-  const int64_t nb = p->num_bins;
+  const int64_t nb = p->num_bins * p->ntof;
   float* ones = (float*)malloc(sizeof(float) * (size_t)nb);
   for (int64_t i = 0; i < nb; ++i) ones[i] = 1.F;
   if (zero_end_planes) {
-    for (int v = 0; v < p->g.num_views; ++v)
-      for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
-        ones[or_bin_index(p, 0, v, 0, t)] = 0;
-        ones[or_bin_index(p, 0, v, p->g.num_axial_poss[SEGI(p, 0)] - 1, t)] = 0;
-      }
+    for (int kk = 0; kk < p->ntof; ++kk)
+      for (int v = 0; v < p->g.num_views; ++v)
+        for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+          ones[tbin(p, 0, v, 0, t, kk)] = 0;
+          ones[tbin(p, 0, v, p->g.num_axial_poss[SEGI(p, 0)] - 1, t, kk)] = 0;
+        }
   }
   or_back_project(p, ones, img, subset, nsub);
   free(ones);
 }
 
-/* divide_and_truncate for one viewgram -- recon_array_functions.cxx:172-266
- * (rim_truncation_sino = 0). Writes the ratio into q. */
+/* divide_and_truncate for one viewgram (segment, view, TOF bin) --
+ * recon_array_functions.cxx:172-266 (rim_truncation_sino = 0). Writes the
+ * ratio into q. */
 static void divide_and_truncate_vg(const or_plan* p, const float* y, const float* ybar, float* q,
-                                   int seg, int view, int zero_end_planes)
+                                   int seg, int view, int kk, int zero_end_planes)
 {
   const int nax = p->g.num_axial_poss[SEGI(p, seg)];
   float mx = -FLT_MAX;
   for (int a = 0; a < nax; ++a)
     for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
-      float yv = y[or_bin_index(p, seg, view, a, t)];
+      float yv = y[tbin(p, seg, view, a, t, kk)];
       if (zero_end_planes && is_end_plane(p, seg, a)) yv = 0;
       if (yv > mx) mx = yv;
     }
   const float small_value = maxf_(mx * 0.000001F, 0.F);
   for (int a = 0; a < nax; ++a)
     for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
-      const int64_t i = or_bin_index(p, seg, view, a, t);
+      const int64_t i = tbin(p, seg, view, a, t, kk);
       float num = y[i];
       if (zero_end_planes && is_end_plane(p, seg, a)) num = 0;
       if (num <= small_value) num = 0;
@@ -1058,12 +1232,24 @@ static void divide_and_truncate_vg(const or_plan* p, const float* y, const float
     }
 }
 
+/* ybar += b over one viewgram (b zeroed at end planes when requested) */
+static void add_background_vg(const or_plan* p, float* ybar, const float* bkg, int seg, int view, int kk,
+                              int zero_end_planes)
+{
+  const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+  for (int a = 0; a < nax; ++a)
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+      const int64_t i = tbin(p, seg, view, a, t, kk);
+      ybar[i] += (zero_end_planes && is_end_plane(p, seg, a)) ? 0.F : bkg[i];
+    }
+}
+
 /* compute_sub_gradient_without_penalty[_plus_sensitivity]
  * -- ...AndProjData.cxx:694-735, RPC at :1423-1481 */
 void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, const float* x,
                         int subset, int nsub, int zero_end_planes, int add_sensitivity, float* grad)
 {
-  const int64_t nb = p->num_bins;
+  const int64_t nb = p->num_bins * p->ntof;
   float* ybar = (float*)calloc((size_t)nb, sizeof(float));
   float* q = (float*)calloc((size_t)nb, sizeof(float));
   or_forward_project_omp(p, x, ybar, subset, nsub);
@@ -1071,22 +1257,17 @@ void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, cons
   for (int k = 0; k < n; ++k) {
     const int seg = vs[k].seg, view = vs[k].view;
     const int nax = p->g.num_axial_poss[SEGI(p, seg)];
-    if (bkg)
-      for (int a = 0; a < nax; ++a)
-        for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
-          const int64_t i = or_bin_index(p, seg, view, a, t);
-          float b = bkg[i];
-          if (zero_end_planes && is_end_plane(p, seg, a)) b = 0;
-          ybar[i] += b;
-        }
-    divide_and_truncate_vg(p, y, ybar, q, seg, view, zero_end_planes);
-    if (!add_sensitivity)
-      for (int a = 0; a < nax; ++a)
-        for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
-          const int64_t i = or_bin_index(p, seg, view, a, t);
-          const float m = (zero_end_planes && is_end_plane(p, seg, a)) ? 0.F : 1.F;
-          q[i] -= m;
-        }
+    for (int kk = 0; kk < p->ntof; ++kk) {
+      if (bkg) add_background_vg(p, ybar, bkg, seg, view, kk, zero_end_planes);
+      divide_and_truncate_vg(p, y, ybar, q, seg, view, kk, zero_end_planes);
+      if (!add_sensitivity)
+        for (int a = 0; a < nax; ++a)
+          for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+            const int64_t i = tbin(p, seg, view, a, t, kk);
+            const float m = (zero_end_planes && is_end_plane(p, seg, a)) ? 0.F : 1.F;
+            q[i] -= m;
+          }
+    }
   }
   free(vs);
   or_back_project(p, q, grad, subset, nsub);
@@ -1098,7 +1279,7 @@ void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, cons
 double or_log_likelihood(const or_plan* p, const float* y, const float* bkg, const float* x,
                          int subset, int nsub, int zero_end_planes)
 {
-  const int64_t nb = p->num_bins;
+  const int64_t nb = p->num_bins * p->ntof;
   float* ybar = (float*)calloc((size_t)nb, sizeof(float));
   or_forward_project_omp(p, x, ybar, subset, nsub);
   vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
@@ -1106,30 +1287,32 @@ double or_log_likelihood(const or_plan* p, const float* y, const float* bkg, con
   for (int k = 0; k < n; ++k) {
     const int seg = vs[k].seg, view = vs[k].view;
     const int nax = p->g.num_axial_poss[SEGI(p, seg)];
-    float mx = -FLT_MAX;
-    for (int a = 0; a < nax; ++a)
-      for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
-        float yv = y[or_bin_index(p, seg, view, a, t)];
-        if (zero_end_planes && is_end_plane(p, seg, a)) yv = 0;
-        if (yv > mx) mx = yv;
+    for (int kk = 0; kk < p->ntof; ++kk) {
+      float mx = -FLT_MAX;
+      for (int a = 0; a < nax; ++a)
+        for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+          float yv = y[tbin(p, seg, view, a, t, kk)];
+          if (zero_end_planes && is_end_plane(p, seg, a)) yv = 0;
+          if (yv > mx) mx = yv;
+        }
+      const float small_value = maxf_(mx * 0.000001F, 0.F);
+      double result = 0;
+      for (int a = 0; a < nax; ++a) {
+        double sub = 0;
+        const int ep = zero_end_planes && is_end_plane(p, seg, a);
+        for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+          const int64_t i = tbin(p, seg, view, a, t, kk);
+          float est = ybar[i] + (bkg ? (ep ? 0.F : bkg[i]) : 0.F);
+          if (ep) est = est * 0.F;
+          const float yv = ep ? 0.F : y[i];
+          const float ne = maxf_(est, yv / 10000.F);
+          if (yv <= small_value) sub += -(double)ne;
+          else sub += yv * log((double)ne) - (double)ne;
+        }
+        result += sub;
       }
-    const float small_value = maxf_(mx * 0.000001F, 0.F);
-    double result = 0;
-    for (int a = 0; a < nax; ++a) {
-      double sub = 0;
-      const int ep = zero_end_planes && is_end_plane(p, seg, a);
-      for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
-        const int64_t i = or_bin_index(p, seg, view, a, t);
-        float est = ybar[i] + (bkg ? (ep ? 0.F : bkg[i]) : 0.F);
-        if (ep) est = est * 0.F;
-        const float yv = ep ? 0.F : y[i];
-        const float ne = maxf_(est, yv / 10000.F);
-        if (yv <= small_value) sub += -(double)ne;
-        else sub += yv * log((double)ne) - (double)ne;
-      }
-      result += sub;
+      total += result;
     }
-    total += result;
   }
   free(vs);
   free(ybar);
@@ -1165,21 +1348,17 @@ void or_osem(const or_plan* p, const float* y, const float* bkg, float* x, int n
     const int subset = (k + start_subset - 1) % nsub;
     if (use_omp) {
       /* same maths, OpenMP back projection */
-      const int64_t nb = p->num_bins;
+      const int64_t nb = p->num_bins * p->ntof;
       float* ybar = (float*)calloc((size_t)nb, sizeof(float));
       float* q = (float*)calloc((size_t)nb, sizeof(float));
       or_forward_project_omp(p, x, ybar, subset, nsub);
       vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
       for (int j = 0; j < n; ++j) {
         const int seg = vs[j].seg, view = vs[j].view;
-        const int nax = p->g.num_axial_poss[SEGI(p, seg)];
-        if (bkg)
-          for (int a = 0; a < nax; ++a)
-            for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
-              const int64_t i = or_bin_index(p, seg, view, a, t);
-              ybar[i] += (zero_end_planes && is_end_plane(p, seg, a)) ? 0.F : bkg[i];
-            }
-        divide_and_truncate_vg(p, y, ybar, q, seg, view, zero_end_planes);
+        for (int kk = 0; kk < p->ntof; ++kk) {
+          if (bkg) add_background_vg(p, ybar, bkg, seg, view, kk, zero_end_planes);
+          divide_and_truncate_vg(p, y, ybar, q, seg, view, kk, zero_end_planes);
+        }
       }
       free(vs);
       or_back_project_omp(p, q, u, subset, nsub);
@@ -1209,4 +1388,69 @@ void or_osem(const or_plan* p, const float* y, const float* bkg, float* x, int n
   free(u);
   for (int s = 0; s < nsub; ++s) free(sens[s]);
   free(sens);
+}
+
+/* ------------------------------------------------------------------ */
+/* QuadraticPrior (C2 loss term)                                       */
+/* ------------------------------------------------------------------ */
+
+/* compute_weights (QuadraticPrior.cxx:202-226): w = vox_x / |offset|, 0 at
+ * the centre; weights[dz+1][dy+1][dx+1]. The squared length is summed in
+ * float; the unqualified sqrt there takes the C library's double overload
+ * (no <math.h> reaches that file, only <cmath>), so the quotient is formed in
+ * double and rounded to float once. The float overload would differ in the
+ * last bit for some offsets (parity unpinned: a build of the reference would
+ * settle it). */
+void or_quadratic_prior_weights(float vox_z, float vox_y, float vox_x, float* w27) {
+  for (int z = -1; z <= 1; ++z)
+    for (int y = -1; y <= 1; ++y)
+      for (int x = -1; x <= 1; ++x) {
+        float* w = &w27[((z + 1) * 3 + (y + 1)) * 3 + (x + 1)];
+        if (z == 0 && y == 0 && x == 0) {
+          *w = 0;
+        } else {
+          const float sx = (float)x * vox_x, sy = (float)y * vox_y, sz = (float)z * vox_z;
+          *w = (float)((double)vox_x / sqrt((double)(sx * sx + sy * sy + sz * sz)));
+        }
+      }
+}
+
+/* compute_value (QuadraticPrior.cxx:229-298) and compute_gradient
+ * (:301-367) over an image [nz][ny][nx]; neighbours outside the image are
+ * skipped; kappa may be NULL. Per term: float w*(a-b)^2/4 (value) or
+ * float w*(a-b) (gradient), accumulated in double; gradient scaled by beta
+ * in double then rounded to float. */
+double or_quadratic_prior(const float* img, const float* kappa, int nz, int ny, int nx, const float* w27,
+                          int only_2D, float beta, float* grad) {
+  double result = 0.;
+  const int wz0 = only_2D ? 0 : -1, wz1 = only_2D ? 0 : 1;
+  for (int z = 0; z < nz; ++z) {
+    const int min_dz = wz0 > -z ? wz0 : -z, max_dz = wz1 < nz - 1 - z ? wz1 : nz - 1 - z;
+    for (int y = 0; y < ny; ++y) {
+      const int min_dy = -1 > -y ? -1 : -y, max_dy = 1 < ny - 1 - y ? 1 : ny - 1 - y;
+      for (int x = 0; x < nx; ++x) {
+        const int min_dx = -1 > -x ? -1 : -x, max_dx = 1 < nx - 1 - x ? 1 : nx - 1 - x;
+        const int64_t v = ((int64_t)z * ny + y) * nx + x;
+        double g = 0.;
+        for (int dz = min_dz; dz <= max_dz; ++dz)
+          for (int dy = min_dy; dy <= max_dy; ++dy)
+            for (int dx = min_dx; dx <= max_dx; ++dx) {
+              const int64_t n = ((int64_t)(z + dz) * ny + (y + dy)) * nx + (x + dx);
+              const float w = w27[((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)];
+              const float d = img[v] - img[n];
+              double cv = (double)(w * (d * d) / 4);
+              double cg = (double)(w * d);
+              if (kappa) {
+                const float kk = kappa[v] * kappa[n];
+                cv *= kk;
+                cg *= kk;
+              }
+              result += cv;
+              g += cg;
+            }
+        if (grad) grad[v] = (float)(g * beta);
+      }
+    }
+  }
+  return result * beta;
 }

@@ -37,6 +37,7 @@ extern "C" {
 #endif
 
 #define OR_MAX_SEGMENTS 256
+#define OR_MAX_TOF 64
 
 /* Geometry of one reconstruction problem. Segment arrays are indexed by
  * (segment_num - min_segment_num). Image index ranges follow
@@ -71,6 +72,16 @@ typedef struct {
   int32_t do_symmetry_swap_segment;
   int32_t do_symmetry_swap_s;
   int32_t do_symmetry_shift_z;
+  /* time of flight (ProjDataInfo::set_tof_mash_factor, ProjDataInfo.cxx:174-258;
+   * ProjMatrixByBin::enable_tof, ProjMatrixByBin.cxx:66-75). num_tof_bins <= 1:
+   * non-TOF. Bin k (0-based, = timing_pos_num - min_tof_pos_num) covers
+   * [tof_low[k], tof_high[k]] mm along the LOR; r_sqrt2_gauss_sigma =
+   * 1/(sigma*sqrt(2)) with sigma the Gaussian kernel's standard deviation in mm.
+   * TOF projection data hold num_tof_bins non-TOF blocks, TOF bin outermost. */
+  int32_t num_tof_bins;
+  float tof_r_sqrt2_gauss_sigma;
+  float tof_low[OR_MAX_TOF];
+  float tof_high[OR_MAX_TOF];
 } or_geometry;
 
 typedef struct or_plan or_plan;
@@ -96,6 +107,11 @@ int or_get_row(const or_plan* p, int seg, int view, int ax, int tang,
                int16_t* z, int16_t* y, int16_t* x, float* w, int cap);
 /* Row computed directly for the given bin (no symmetry), as
  * calculate_proj_matrix_elems_for_one_bin (PMRT.cxx:582-829). */
+/* TOF row of bin (seg, view, ax, tang, TOF bin tof in 0..num_tof_bins-1):
+ * the basic row's elements with a positive TOF kernel value, weights value *
+ * kernel (ProjMatrixByBin.inl:115-181), transformed to the bin. */
+int or_get_row_tof(const or_plan* p, int seg, int view, int ax, int tang, int tof,
+                   int16_t* z, int16_t* y, int16_t* x, float* w, int cap);
 int or_trace_row(const or_plan* p, int seg, int view, int ax, int tang,
                  int16_t* z, int16_t* y, int16_t* x, float* w, int cap);
 /* RayTraceVoxelsOnCartesianGrid on one segment (voxel units, coordinates
@@ -148,6 +164,14 @@ double or_log_likelihood(const or_plan* p, const float* y, const float* bkg, con
 void or_osem(const or_plan* p, const float* y, const float* bkg, float* x,
              int num_subsets, int num_subiterations, int start_subset, int zero_end_planes,
              int use_omp);
+
+/* QuadraticPrior (QuadraticPrior.cxx:202-367): weights w27[dz+1][dy+1][dx+1]
+ * = vox_x/|offset|; value = beta * sum w (x_v - x_n)^2 / 4 (returned),
+ * grad_v = beta * sum_n w (x_v - x_n) (written if grad != NULL), kappa
+ * optional (NULL). Images [nz][ny][nx]. */
+void or_quadratic_prior_weights(float vox_z, float vox_y, float vox_x, float* w27);
+double or_quadratic_prior(const float* img, const float* kappa, int nz, int ny, int nx, const float* w27,
+                          int only_2D, float beta, float* grad);
 
 #ifdef __cplusplus
 }

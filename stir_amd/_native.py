@@ -24,8 +24,11 @@ EXPORTED_SYMBOLS = (
     "mi_back_project_dev", "mi_obj_set_data", "mi_obj_compute_sensitivities", "mi_obj_get_sensitivity",
     "mi_obj_set_estimate", "mi_obj_get_estimate", "mi_osem_subset_step", "mi_osem_run", "mi_obj_subset_gradient",
     "mi_obj_log_likelihood", "mi_synchronize", "mi_set_kernel_timing", "mi_get_kernel_timing",
-    "mi_symmetry_operations",
+    "mi_symmetry_operations", "mi_set_tuning", "mi_obj_value_and_gradient_dev", "mi_quadratic_prior_dev",
+    "mi_osem_subset_step_dev", "mi_osem_subset_step_vjp_dev",
 )
+
+MI_TUNE_BP_TILE_X, MI_TUNE_BP_TILE_Y, MI_TUNE_BP_XCD_REMAP, MI_TUNE_BP_SLOT_SKIP = 1, 2, 3, 4
 
 
 class NativeError(RuntimeError):
@@ -90,6 +93,11 @@ def load():
     L.mi_obj_log_likelihood.argtypes = [vp, c_int, ctypes.POINTER(ctypes.c_double)]
     L.mi_synchronize.argtypes = [vp]
     L.mi_symmetry_operations.argtypes = [ctypes.POINTER(GeometryStruct), c_int, _i32p, _i32p, _i32p]
+    L.mi_set_tuning.argtypes = [vp, c_int, c_int]
+    L.mi_obj_value_and_gradient_dev.argtypes = [vp, vp, c_int, c_int, vp, vp, vp]
+    L.mi_quadratic_prior_dev.argtypes = [vp, vp, vp, vp, ctypes.c_float, c_int, vp, vp, vp]
+    L.mi_osem_subset_step_dev.argtypes = [vp, vp, vp, vp, c_int, c_int, vp]
+    L.mi_osem_subset_step_vjp_dev.argtypes = [vp, vp, vp, vp, vp, c_int, vp]
     L.mi_set_kernel_timing.argtypes = [vp, c_int]
     L.mi_get_kernel_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
@@ -136,6 +144,9 @@ class Context:
         out = InfoStruct()
         check(load().mi_get_info(self._h, ctypes.byref(out)), "mi_get_info")
         return out
+
+    def set_tuning(self, param: int, value: int) -> None:
+        check(load().mi_set_tuning(self._h, int(param), int(value)), "mi_set_tuning")
 
     def set_kernel_timing(self, enable: bool) -> None:
         check(load().mi_set_kernel_timing(self._h, int(enable)), "mi_set_kernel_timing")

@@ -100,6 +100,7 @@ struct mi_ctx {
   int64_t num_matrix_elements = 0;
   int64_t num_nonzeros = 0;
   int uniform_ppa = 0;  // planes per axial position if equal for all families, else 0
+  mi::BpTuning bp_tuning;
   // kernel timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> fp_events, bp_events;
@@ -131,7 +132,7 @@ struct mi_ctx {
   std::vector<std::unique_ptr<DBuf<float>>> sens;
   DBuf<int64_t> vg_off;
   DBuf<int32_t> vg_len;
-  DBuf<double> partial;
+  DBuf<double> partial, prior_partial;
   bool have_data = false, have_sens = false, have_x = false;
   float add_const = 0.F;
   int zero_end_planes = 0;
@@ -144,7 +145,8 @@ struct mi_ctx {
   void check_subset(int subset) const {
     if (subset >= plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
   }
-  hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
+  // *_dev entry points: the caller's stream; NULL is the HIP null stream.
+  static hipStream_t pick(void* s) { return static_cast<hipStream_t>(s); }
 
   // ---- layout conversions
   void image_to_device(const float* src_stir, float* dst, hipStream_t st) const {
@@ -184,7 +186,7 @@ struct mi_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
     mi::launch_back_project(mode, uniform_ppa, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj, d_img,
-                            dims, data, st);
+                            dims, data, bp_tuning, st);
     if (timing) { HIPCHECK(hipEventRecord(e1, st)); bp_events.emplace_back(e0, e1); }
   }
 
@@ -205,6 +207,24 @@ struct mi_ctx {
 };
 
 namespace {
+// compute_weights (QuadraticPrior.cxx:202-226): vox_x / |offset|, the squared
+// length summed in float and the quotient formed in double (the reference's
+// unqualified sqrt resolves to the C library's double overload there).
+void default_prior_weights(const mi_geometry& g, float* w27) {
+  for (int z = -1; z <= 1; ++z)
+    for (int y = -1; y <= 1; ++y)
+      for (int x = -1; x <= 1; ++x) {
+        float& w = w27[((z + 1) * 3 + (y + 1)) * 3 + (x + 1)];
+        if (z == 0 && y == 0 && x == 0) {
+          w = 0;
+        } else {
+          const float sx = static_cast<float>(x) * g.vox_x, sy = static_cast<float>(y) * g.vox_y;
+          const float sz = static_cast<float>(z) * g.vox_z;
+          w = static_cast<float>(static_cast<double>(g.vox_x) / std::sqrt(static_cast<double>(sx * sx + sy * sy + sz * sz)));
+        }
+      }
+}
+
 int64_t fam_key(int seg, int view, int tang, int a0) {
   return (((static_cast<int64_t>(seg + 512) * 4096 + view) * 8192) + (tang + 4096)) * 8192 + a0;
 }
@@ -436,7 +456,7 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   pool.release(); fams.release(); tasks.release(); list_off.release(); entries.release();
   img.release(); img2.release(); sino.release(); ratio.release(); stage_img.release(); stage_sino.release();
   y.release(); add.release(); mult.release(); ones.release(); vg_small.release(); x.release(); sens.clear();
-  vg_off.release(); vg_len.release(); partial.release();
+  vg_off.release(); vg_len.release(); partial.release(); prior_partial.release();
   have_data = have_sens = have_x = false;
   plan.reset();
   plan = std::make_unique<mi::Plan>(g, nsub);
@@ -456,6 +476,7 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   ratio.alloc(plan->num_bins());
   stage_img.alloc(plan->num_voxels());
   stage_sino.alloc(plan->num_bins());
+  prior_partial.alloc(mi::prior_num_partials(g.nz, g.ny, g.nx));
   // viewgram extents in the device layout
   {
     const int nvg = plan->num_viewgrams();
@@ -489,7 +510,9 @@ int mi_ctx_create(int device, mi_ctx** out) {
     auto ctx = std::make_unique<mi_ctx>();
     ctx->device = device;
     HIPCHECK(hipSetDevice(device));
-    HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // a blocking stream: ordered with work on the HIP null stream, so host-buffer
+    // calls and *_dev calls on the null stream see each other's results
+    HIPCHECK(hipStreamCreate(&ctx->stream));
     *out = ctx.release();
   });
 }
@@ -658,7 +681,7 @@ int mi_forward_project(mi_ctx* ctx, const float* image, float* proj_data, int su
     ctx->check_subset(subset);
     HIPCHECK(hipSetDevice(ctx->device));
     HIPCHECK(hipMemcpyAsync(ctx->stage_img.p, image, ctx->stage_img.bytes(), hipMemcpyHostToDevice, ctx->stream));
-    if (mi_forward_project_dev(ctx, ctx->stage_img.p, ctx->stage_sino.p, subset, nullptr) != MI_OK)
+    if (mi_forward_project_dev(ctx, ctx->stage_img.p, ctx->stage_sino.p, subset, ctx->stream) != MI_OK)
       throw ApiError(MI_ERR_DEVICE, g_last_error);
     HIPCHECK(hipMemcpyAsync(proj_data, ctx->stage_sino.p, ctx->stage_sino.bytes(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -672,7 +695,7 @@ int mi_back_project(mi_ctx* ctx, const float* proj_data, float* image, int subse
     ctx->check_subset(subset);
     HIPCHECK(hipSetDevice(ctx->device));
     HIPCHECK(hipMemcpyAsync(ctx->stage_sino.p, proj_data, ctx->stage_sino.bytes(), hipMemcpyHostToDevice, ctx->stream));
-    if (mi_back_project_dev(ctx, ctx->stage_sino.p, ctx->stage_img.p, subset, nullptr) != MI_OK)
+    if (mi_back_project_dev(ctx, ctx->stage_sino.p, ctx->stage_img.p, subset, ctx->stream) != MI_OK)
       throw ApiError(MI_ERR_DEVICE, g_last_error);
     HIPCHECK(hipMemcpyAsync(image, ctx->stage_img.p, ctx->stage_img.bytes(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -874,6 +897,130 @@ int mi_obj_log_likelihood(mi_ctx* ctx, int subset, double* value) {
     double acc = 0;
     for (double v : h) acc += v;
     *value = acc;
+  });
+}
+
+int mi_obj_value_and_gradient_dev(mi_ctx* ctx, const float* d_image, int subset, int add_sensitivity,
+                                  float* d_gradient, double* d_value, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_image) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (!d_gradient && !d_value) throw ApiError(MI_ERR_ARGUMENT, "neither a gradient nor a value was asked for");
+    ctx->require_setup();
+    ctx->check_subset(subset);
+    if (!ctx->have_data) throw ApiError(MI_ERR_STATE, "mi_obj_set_data has not been called");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->pick(stream);
+    ctx->image_to_device(d_image, ctx->img.p, st);
+    mi::FpMode mode;
+    if (!d_gradient) mode = mi::FP_LOGLIK;
+    else if (!d_value) mode = add_sensitivity ? mi::FP_RATIO : mi::FP_GRADIENT;
+    else mode = add_sensitivity ? mi::FP_RATIO_LOGLIK : mi::FP_GRADIENT_LOGLIK;
+    ctx->fp(mode, ctx->img.p, ctx->ratio.p, subset, ctx->fp_data(), st);
+    if (d_value) {
+      const int tb = ctx->task_off[ctx->subset_begin(subset)], te = ctx->task_off[ctx->subset_end(subset)];
+      mi::launch_sum_partials(ctx->partial.p, te - tb, 1.0, d_value, st);
+    }
+    if (d_gradient) {
+      ctx->bp(mi::BP_PLAIN, ctx->ratio.p, ctx->img2.p, subset, mi::BpData{}, st);
+      ctx->image_from_device(ctx->img2.p, d_gradient, st);
+    }
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_quadratic_prior_dev(mi_ctx* ctx, const float* d_image, const float* d_kappa, const float* weights, float beta,
+                           int only_2D, float* d_gradient, double* d_value, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_image) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (!d_gradient && !d_value) throw ApiError(MI_ERR_ARGUMENT, "neither a gradient nor a value was asked for");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->pick(stream);
+    const mi_geometry& g = ctx->plan->geometry();
+    if (beta == 0.F) {  // QuadraticPrior.cxx:233-237,305-309
+      if (d_gradient) mi::launch_fill(d_gradient, ctx->plan->num_voxels(), 0.F, st);
+      if (d_value) mi::launch_sum_partials(ctx->prior_partial.p, 0, 0.0, d_value, st);
+    } else {
+      mi::PriorWeights W;
+      if (weights) std::memcpy(W.w, weights, sizeof(W.w));
+      else default_prior_weights(g, W.w);
+      mi::launch_quadratic_prior(d_image, d_kappa, g.nz, g.ny, g.nx, W, only_2D, beta, d_gradient, ctx->prior_partial.p,
+                                 st);
+      if (d_value)
+        mi::launch_sum_partials(ctx->prior_partial.p, mi::prior_num_partials(g.nz, g.ny, g.nx),
+                                static_cast<double>(beta), d_value, st);
+    }
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_osem_subset_step_dev(mi_ctx* ctx, const float* d_x_in, float* d_x_out, float* d_update, int subset,
+                            int subiteration_num, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_x_in || !d_x_out) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    if (!ctx->have_data || !ctx->have_sens) throw ApiError(MI_ERR_STATE, "data and sensitivities must be set first");
+    if (subset < 0 || subset >= ctx->plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->pick(stream);
+    ctx->image_to_device(d_x_in, ctx->img.p, st);
+    ctx->fp(mi::FP_RATIO, ctx->img.p, ctx->ratio.p, subset, ctx->fp_data(), st);
+    mi::BpData bd{};
+    bd.sens = ctx->sens[subset]->p;
+    bd.estimate = ctx->img.p;
+    bd.update = d_update ? ctx->img2.p : nullptr;
+    bd.threshold = subiteration_num != 1;
+    ctx->bp(mi::BP_OSEM_UPDATE, ctx->ratio.p, nullptr, subset, bd, st);
+    ctx->image_from_device(ctx->img.p, d_x_out, st);
+    if (d_update) ctx->image_from_device(ctx->img2.p, d_update, st);
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_osem_subset_step_vjp_dev(mi_ctx* ctx, const float* d_x_in, const float* d_update, const float* d_grad_out,
+                                float* d_grad_in, int subset, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_x_in || !d_update || !d_grad_out || !d_grad_in) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (d_grad_in == d_grad_out || d_grad_in == d_update)
+      throw ApiError(MI_ERR_ARGUMENT, "d_grad_in must not alias d_grad_out or d_update");
+    ctx->require_setup();
+    if (!ctx->have_data || !ctx->have_sens) throw ApiError(MI_ERR_STATE, "data and sensitivities must be set first");
+    if (subset < 0 || subset >= ctx->plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->pick(stream);
+    const int64_t ndev = static_cast<int64_t>(ctx->dims.ncols) * ctx->dims.nzp;
+    // r'(FP_s(x) + b) on the subset's bins
+    ctx->image_to_device(d_x_in, ctx->img.p, st);
+    ctx->fp(mi::FP_DRATIO, ctx->img.p, ctx->ratio.p, subset, ctx->fp_data(), st);
+    // v = g * x / sens_s, then FP_s(v) * r'
+    ctx->image_to_device(d_grad_out, ctx->img2.p, st);
+    mi::launch_osem_vjp_scale(ctx->img2.p, ctx->img.p, ctx->sens[subset]->p, ctx->img.p, ndev, st);
+    mi::FpData fd = ctx->fp_data();
+    fd.scale = ctx->ratio.p;
+    ctx->fp(mi::FP_SCALE, ctx->img.p, ctx->sino.p, subset, fd, st);
+    ctx->bp(mi::BP_PLAIN, ctx->sino.p, ctx->img2.p, subset, mi::BpData{}, st);
+    ctx->image_from_device(ctx->img2.p, d_grad_in, st);
+    mi::launch_add_product(d_grad_in, d_grad_out, d_update, ctx->plan->num_voxels(), st);
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_set_tuning(mi_ctx* ctx, int param, int value) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    switch (param) {
+      case MI_TUNE_BP_TILE_X:
+        if (value < 1 || value > 1024) throw ApiError(MI_ERR_ARGUMENT, "tile x out of range");
+        ctx->bp_tuning.tile_x = value;
+        break;
+      case MI_TUNE_BP_TILE_Y:
+        if (value < 1 || value > 1024) throw ApiError(MI_ERR_ARGUMENT, "tile y out of range");
+        ctx->bp_tuning.tile_y = value;
+        break;
+      case MI_TUNE_BP_XCD_REMAP: ctx->bp_tuning.xcd_remap = value != 0; break;
+      case MI_TUNE_BP_SLOT_SKIP: ctx->bp_tuning.slot_skip = value != 0; break;
+      default: throw ApiError(MI_ERR_ARGUMENT, "unknown tuning parameter");
+    }
   });
 }
 

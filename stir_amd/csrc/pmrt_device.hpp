@@ -39,10 +39,14 @@ struct ImageDims {
 
 // Forward-projection epilogues.
 enum FpMode : int {
-  FP_PLAIN = 0,      // out = A x
-  FP_RATIO = 1,      // out = divide_and_truncate(y, A x + b)
-  FP_GRADIENT = 2,   // out = divide_and_truncate(y, A x + b) - m
-  FP_LOGLIK = 3      // partial sums of the Poisson log-likelihood (FP64)
+  FP_PLAIN = 0,            // out = A x
+  FP_RATIO = 1,            // out = divide_and_truncate(y, A x + b)
+  FP_GRADIENT = 2,         // out = divide_and_truncate(y, A x + b) - m
+  FP_LOGLIK = 3,           // partial sums of the Poisson log-likelihood (FP64)
+  FP_GRADIENT_LOGLIK = 4,  // FP_GRADIENT and FP_LOGLIK in one pass
+  FP_RATIO_LOGLIK = 5,     // FP_RATIO and FP_LOGLIK in one pass
+  FP_DRATIO = 6,           // out = d divide_and_truncate(y, ybar) / d ybar at ybar = A x + b
+  FP_SCALE = 7             // out = (A x) * scale
 };
 
 struct FpData {
@@ -52,6 +56,7 @@ struct FpData {
   const float* vg_small;      // per-viewgram small_value = max(max(y_viewgram)*1e-6, 0)
   const float* mult;          // m for FP_GRADIENT (ones, zero at end planes), or nullptr (= 1)
   double* partial;            // FP_LOGLIK: one double per workgroup
+  const float* scale;         // FP_SCALE: per-bin factor, device layout
 };
 
 // Back-projection epilogues.
@@ -63,6 +68,7 @@ enum BpMode : int {
 struct BpData {
   const float* sens;  // subset sensitivity, device image layout
   float* estimate;    // x, device image layout, updated in place
+  float* update;      // if not null, receives the multiplicative update u
   int threshold;      // apply threshold_upper_lower(0, FLT_MAX)
 };
 
@@ -77,11 +83,21 @@ void launch_pack_rows(int first, int count, const uint64_t* scratch_k, const flo
 void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams, const int2* d_pool,
                             const float* d_img, float* d_out, const ImageDims& dims, const FpData& data,
                             hipStream_t stream);
+// Launch shape of the back projection: image columns are visited in
+// tile_x x tile_y tiles (1 x 1 = raster order), optionally remapped so each XCD
+// gets a contiguous range of tiles, optionally skipping z slots an entry cannot reach.
+struct BpTuning {
+  int tile_x = 1;
+  int tile_y = 1;
+  int xcd_remap = 1;
+  int slot_skip = 0;
+};
+
 // ppa: planes per axial position shared by all families (1 or 2 select a
 // specialised kernel), 0 when it varies between segments.
 void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const BpEntry* d_entries, int subset_begin,
                          int subset_end, const float* d_proj, float* d_img, const ImageDims& dims, const BpData& data,
-                         hipStream_t stream);
+                         const BpTuning& tune, hipStream_t stream);
 // 2-D transpose used for the STIR <-> device layout conversions:
 // dst[c * dst_ld + r] = src[r * src_ld + c] for r < rows, c < cols.
 void launch_transpose(const float* src, float* dst, int rows, int cols, int src_ld, int dst_ld, hipStream_t stream);
@@ -102,5 +118,24 @@ void launch_fill(float* d, int64_t n, float v, hipStream_t stream);
 void launch_viewgram_max(const float* d_proj, const int64_t* d_vg_off, const int32_t* d_vg_len, int nvg, float* d_small,
                          hipStream_t stream);
 void launch_zero_end_planes(float* d_proj, const int64_t* d_off, int count, int run, int stride, hipStream_t stream);
+
+// ---- objective-function helpers (pmrt_objective.hip)
+// QuadraticPrior weights w[dz+1][dy+1][dx+1].
+struct PriorWeights {
+  float w[27];
+};
+// Value partials (one double per workgroup, count returned by
+// prior_num_partials) and gradient (if d_grad) of the quadratic prior of a
+// [nz][ny][nx] image; kappa may be null.
+int prior_num_partials(int nz, int ny, int nx);
+void launch_quadratic_prior(const float* d_img, const float* d_kappa, int nz, int ny, int nx, const PriorWeights& w,
+                            int only_2D, float beta, float* d_grad, double* d_partial, hipStream_t stream);
+// *d_out = sum of n doubles in a fixed order (one workgroup); d_out may be null-checked by the caller.
+void launch_sum_partials(const double* d_partial, int n, double scale, double* d_out, hipStream_t stream);
+// v = g * x / sens (0 where sens == 0), device image layout, n elements; v may alias g or x
+void launch_osem_vjp_scale(const float* d_g, const float* d_x, const float* d_sens, float* d_v, int64_t n,
+                           hipStream_t stream);
+// out += g * u, n elements
+void launch_add_product(float* d_out, const float* d_g, const float* d_u, int64_t n, hipStream_t stream);
 
 }  // namespace mi

@@ -63,6 +63,7 @@ Plan::Plan(const mi_geometry& g, int num_subsets) : g_(g), nsub_(num_subsets) {
   if (g.num_tangential_LORs < 1) fail("num_tangential_LORs should be at least 1");
   if (g.nx > 1022 || g.ny > 1022) fail("image x/y size above 1022 is not supported by the element packing");
   if (g.nz > 256) fail("images with more than 256 planes are not supported");
+  if (g.num_tof_bins > 1) fail("TOF projection data are not supported yet");
   max_tang_ = g.min_tangential_pos_num + g.num_tangential_poss - 1;
   max_z_ = g.min_z + g.nz - 1;
   max_y_ = g.min_y + g.ny - 1;

@@ -130,6 +130,11 @@ __device__ inline void fp_row(const int2* __restrict__ row, int n, const LaneBin
   }
 }
 
+constexpr bool fp_writes_ratio(int m) {
+  return m == FP_RATIO || m == FP_GRADIENT || m == FP_GRADIENT_LOGLIK || m == FP_RATIO_LOGLIK;
+}
+constexpr bool fp_accumulates_loglik(int m) { return m == FP_LOGLIK || m == FP_GRADIENT_LOGLIK || m == FP_RATIO_LOGLIK; }
+
 template <int MODE, int NSLOT>
 __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, const int2* __restrict__ pool,
                                const float* __restrict__ img, float* __restrict__ out, const ImageDims& d,
@@ -168,31 +173,45 @@ __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, co
     const int64_t i = L.out[q];
     if constexpr (MODE == FP_PLAIN) {
       out[i] = acc[q];
-    } else if constexpr (MODE == FP_RATIO || MODE == FP_GRADIENT) {
-      // estimated += additive; divide_and_truncate (recon_array_functions.cxx:172-266)
+    } else if constexpr (MODE == FP_SCALE) {
+      out[i] = acc[q] * data.scale[i];
+    } else {
       const float ybar = acc[q] + (data.additive ? data.additive[i] : data.additive_constant);
-      float num = data.measured[i];
+      const float yv = data.measured[i];
       const float small_value = data.vg_small[L.vg[q]];
-      if (num <= small_value) {
-        num = 0;
-      } else {
-        const float max_quotient = 10000.F;
-        if (num > max_quotient * ybar) num = max_quotient;
-        else num = num / ybar;
+      const float max_quotient = 10000.F;
+      if constexpr (fp_writes_ratio(MODE)) {
+        // estimated += additive; divide_and_truncate (recon_array_functions.cxx:172-266)
+        float num = yv;
+        if (num <= small_value) {
+          num = 0;
+        } else {
+          if (num > max_quotient * ybar) num = max_quotient;
+          else num = num / ybar;
+        }
+        if constexpr (MODE == FP_GRADIENT || MODE == FP_GRADIENT_LOGLIK) num = num - (data.mult ? data.mult[i] : 1.F);
+        out[i] = num;
       }
-      if constexpr (MODE == FP_GRADIENT) num = num - (data.mult ? data.mult[i] : 1.F);
-      out[i] = num;
-    } else {  // FP_LOGLIK: accumulate_loglikelihood (recon_array_functions.cxx:336-377)
-      float est = acc[q] + (data.additive ? data.additive[i] : data.additive_constant);
-      if (data.mult) est = est * data.mult[i];
-      const float y = data.measured[i];
-      const float small_value = data.vg_small[L.vg[q]];
-      const float ne = (est < y / 10000.F) ? y / 10000.F : est;
-      if (y <= small_value) ll += -static_cast<double>(ne);
-      else ll += y * log(static_cast<double>(ne)) - static_cast<double>(ne);
+      if constexpr (MODE == FP_DRATIO) {
+        // derivative of the quotient above: -y/ybar^2 where y/ybar is used, 0 where truncated
+        float dr = 0.F;
+        if (yv > small_value && !(yv > max_quotient * ybar)) {
+          const float r = yv / ybar;
+          dr = -r / ybar;
+        }
+        out[i] = dr;
+      }
+      if constexpr (fp_accumulates_loglik(MODE)) {
+        // accumulate_loglikelihood (recon_array_functions.cxx:336-377)
+        float est = ybar;
+        if (data.mult) est = est * data.mult[i];
+        const float ne = (est < yv / 10000.F) ? yv / 10000.F : est;
+        if (yv <= small_value) ll += -static_cast<double>(ne);
+        else ll += yv * log(static_cast<double>(ne)) - static_cast<double>(ne);
+      }
     }
   }
-  if constexpr (MODE == FP_LOGLIK) {
+  if constexpr (fp_accumulates_loglik(MODE)) {
     for (int off = kWave / 2; off > 0; off >>= 1) ll += __shfl_xor(ll, off, kWave);
     if (lane == 0) data.partial[t] = ll;
   }
@@ -251,14 +270,28 @@ __device__ inline unsigned entry_axial(int k, int nax_ppa, int zi, bool& ok) {
 // workgroup. Each table entry (wave-uniform, scalar loads) is decoded once
 // for all slots; kUnroll entries have their projection-data reads in flight
 // together and are accumulated in table order.
-template <int MODE, int PPA, int NSLOT>
+// work index -> image column, visiting the image in tile.x x tile.y tiles
+// (columns of one tile run at the same time and share projection-data runs);
+// returns -1 for the padding of partial tiles. A 1 x 1 tile is raster order.
+__device__ inline int tiled_column(int w, const ImageDims& d, int2 tile) {
+  const int tiles_x = (d.nx + tile.x - 1) / tile.x;
+  const int t = w / (tile.x * tile.y), r = w - t * (tile.x * tile.y);
+  const int y = (t / tiles_x) * tile.y + r / tile.x;
+  const int x = (t % tiles_x) * tile.x + r % tile.x;
+  return (y < d.ny && x < d.nx) ? y * d.nx + x : -1;
+}
+
+template <int MODE, int PPA, int NSLOT, bool SKIP>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t* __restrict__ list_off,
                                                                    const BpEntry* __restrict__ ent, int sb, int se,
                                                                    const float* __restrict__ proj,
-                                                                   float* __restrict__ img, ImageDims d, BpData data) {
+                                                                   float* __restrict__ img, ImageDims d, int nwork,
+                                                                   int2 tile, int remap, BpData data) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int col = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
-  if (col >= d.ncols) return;
+  const int w = (remap ? xcd_remap(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kWavesPerBlock + wave;
+  if (w >= nwork) return;
+  const int col = tiled_column(w, d, tile);
+  if (col < 0) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   float acc[NSLOT];
 #pragma unroll
@@ -275,10 +308,18 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t
         const BpEntry E = ent[e + j];
         const float* run = proj + E.sino_base;  // wave-uniform
         wt[j] = E.w;
+        // z range the entry's run can reach (wave-uniform): [k, k + ppa*(nax-1)]
+        const int ppa_e = PPA > 0 ? PPA : (E.nax_ppa >> 16);
+        const int zlo = E.k - d.min_z, zhi = zlo + ppa_e * ((E.nax_ppa & 0xffff) - 1);
 #pragma unroll
         for (int q = 0; q < NSLOT; ++q) {
-          const unsigned a = entry_axial<PPA>(E.k, E.nax_ppa, d.min_z + lane + kWave * q, ok[j][q]);
-          p[j][q] = load_u32off(run, a);
+          if (!SKIP || (zhi >= kWave * q && zlo < kWave * (q + 1))) {
+            const unsigned a = entry_axial<PPA>(E.k, E.nax_ppa, d.min_z + lane + kWave * q, ok[j][q]);
+            p[j][q] = load_u32off(run, a);
+          } else {
+            ok[j][q] = false;
+            p[j][q] = 0.F;
+          }
         }
       }
 #pragma unroll
@@ -318,6 +359,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t
           else if (0.F > u) u = 0.F;
         }
         data.estimate[idx] = data.estimate[idx] * u;
+        if (data.update) data.update[idx] = u;
       }
     }
   }
@@ -382,36 +424,42 @@ void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, cons
   if (ntasks <= 0) return;
   const dim3 grid((ntasks + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
   switch (mode) {
-    case FP_PLAIN:
-      hipLaunchKernelGGL(fp_kernel<FP_PLAIN>, grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, dims, data);
-      break;
-    case FP_RATIO:
-      hipLaunchKernelGGL(fp_kernel<FP_RATIO>, grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, dims, data);
-      break;
-    case FP_GRADIENT:
-      hipLaunchKernelGGL(fp_kernel<FP_GRADIENT>, grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, dims,
-                         data);
-      break;
-    case FP_LOGLIK:
-      hipLaunchKernelGGL(fp_kernel<FP_LOGLIK>, grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, dims, data);
-      break;
+#define MI_FP_CASE(M)                                                                                            \
+  case M:                                                                                                        \
+    hipLaunchKernelGGL(fp_kernel<M>, grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, dims, \
+                       data);                                                                                    \
+    break;
+    MI_FP_CASE(FP_PLAIN)
+    MI_FP_CASE(FP_RATIO)
+    MI_FP_CASE(FP_GRADIENT)
+    MI_FP_CASE(FP_LOGLIK)
+    MI_FP_CASE(FP_GRADIENT_LOGLIK)
+    MI_FP_CASE(FP_RATIO_LOGLIK)
+    MI_FP_CASE(FP_DRATIO)
+    MI_FP_CASE(FP_SCALE)
+#undef MI_FP_CASE
   }
 }
 
 void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const BpEntry* d_entries, int subset_begin,
                          int subset_end, const float* d_proj, float* d_img, const ImageDims& dims, const BpData& data,
-                         hipStream_t stream) {
+                         const BpTuning& tune, hipStream_t stream) {
   const int nslot = (dims.nzp + kWave - 1) / kWave;
-  const dim3 grid((dims.ncols + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
-#define MI_BP_LAUNCH(M, P, S)                                                                                       \
-  hipLaunchKernelGGL((bp_kernel<M, P, S>), grid, block, 0, stream, d_list_off, d_entries, subset_begin, subset_end, \
-                     d_proj, d_img, dims, data)
+  const int2 tile = make_int2(tune.tile_x > 0 ? tune.tile_x : 1, tune.tile_y > 0 ? tune.tile_y : 1);
+  const int nwork = ((dims.nx + tile.x - 1) / tile.x) * ((dims.ny + tile.y - 1) / tile.y) * tile.x * tile.y;
+  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+#define MI_BP_LAUNCH(M, P, S, K)                                                                                       \
+  hipLaunchKernelGGL((bp_kernel<M, P, S, K>), grid, block, 0, stream, d_list_off, d_entries, subset_begin, subset_end, \
+                     d_proj, d_img, dims, nwork, tile, tune.xcd_remap, data)
+#define MI_BP_SKIP(M, P, S)             \
+  if (tune.slot_skip) MI_BP_LAUNCH(M, P, S, true); \
+  else MI_BP_LAUNCH(M, P, S, false);
 #define MI_BP_SLOTS(M, P)                    \
   switch (nslot) {                           \
-    case 1: MI_BP_LAUNCH(M, P, 1); break;    \
-    case 2: MI_BP_LAUNCH(M, P, 2); break;    \
-    case 3: MI_BP_LAUNCH(M, P, 3); break;    \
-    default: MI_BP_LAUNCH(M, P, 4); break;   \
+    case 1: MI_BP_SKIP(M, P, 1) break;       \
+    case 2: MI_BP_SKIP(M, P, 2) break;       \
+    case 3: MI_BP_SKIP(M, P, 3) break;       \
+    default: MI_BP_SKIP(M, P, 4) break;      \
   }
   if (mode == BP_PLAIN) {
     if (ppa == 1) { MI_BP_SLOTS(BP_PLAIN, 1) }
@@ -423,6 +471,7 @@ void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const 
     else { MI_BP_SLOTS(BP_OSEM_UPDATE, 0) }
   }
 #undef MI_BP_SLOTS
+#undef MI_BP_SKIP
 #undef MI_BP_LAUNCH
 }
 

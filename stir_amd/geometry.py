@@ -25,6 +25,8 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 MAX_SEGMENTS = 256
+MAX_TOF_BINS = 64
+SPEED_OF_LIGHT_MM_PER_PS_DIV2 = 0.299792458 * 0.5  # common.h:146-148
 
 
 def f32(x: float) -> float:
@@ -46,8 +48,15 @@ class Scanner:
     intrinsic_tilt: float = 0.0  # rad
     max_num_non_arccorrected_bins: int = 0
     default_num_arccorrected_bins: int = 0
-    # TOF description (None for non-TOF scanners)
-    timing_resolution_ps: Optional[float] = None
+    # TOF description (Scanner::get_timing_resolution, get_size_of_timing_pos,
+    # get_max_num_timing_poss); a scanner is TOF-ready when all three are set
+    timing_resolution_ps: float = 0.0
+    size_of_timing_pos_ps: float = 0.0
+    max_num_timing_poss: int = 0
+
+    @property
+    def is_tof_ready(self) -> bool:
+        return self.max_num_timing_poss > 0 and self.size_of_timing_pos_ps > 0 and self.timing_resolution_ps > 0
 
     @property
     def effective_ring_radius(self) -> float:
@@ -68,6 +77,14 @@ class Scanner:
         (effective 335 mm); the workload description states 328 mm, which is the
         default here (DOI 0)."""
         return Scanner("mMR", 64, 504, radius_mm, doi_mm, 4.0625, f32(2.08626), 0.0, 344, 344)
+
+    @staticmethod
+    def Siemens_mMR_TOF(radius_mm: float = 328.0, timing_resolution_ps: float = 400.0,
+                        size_of_timing_pos_ps: float = 594.0, max_num_timing_poss: int = 11) -> "Scanner":
+        """The mMR geometry with a TOF description (the reference's mMR entry has
+        none): 400 ps FWHM, 11 timing positions of 594 ps (89.0 mm each)."""
+        return replace(Scanner.Siemens_mMR(radius_mm), name="mMR_TOF", timing_resolution_ps=timing_resolution_ps,
+                       size_of_timing_pos_ps=f32(size_of_timing_pos_ps), max_num_timing_poss=max_num_timing_poss)
 
     @staticmethod
     def cylindrical(num_rings: int, num_detectors_per_ring: int, radius_mm: float,
@@ -92,7 +109,7 @@ class ProjDataInfoCylindrical:
     num_tangential_poss: int
     arc_corrected: bool = False
     bin_size: float = 0.0
-    num_tof_bins: int = 1
+    tof_mash_factor: int = 0  # 0: non-TOF (ProjDataInfo::set_tof_mash_factor)
 
     # ------------------------------------------------------------------
     @staticmethod
@@ -147,6 +164,62 @@ class ProjDataInfoCylindrical:
             arc_corrected=arc_corrected,
             bin_size=scanner.default_bin_size,
         )
+
+    # ------------------------------------------------------------------
+    # time of flight
+    def with_tof(self, tof_mash_factor: int) -> "ProjDataInfoCylindrical":
+        """Copy with TOF mashing factor set (``ProjDataInfo.cxx:174-258``)."""
+        if tof_mash_factor > 0:
+            sc = self.scanner
+            if not sc.is_tof_ready:
+                raise ValueError("scanner has no TOF description")
+            if tof_mash_factor > sc.max_num_timing_poss or sc.max_num_timing_poss % tof_mash_factor:
+                raise ValueError("TOF mashing factor must divide the scanner's number of timing positions")
+        return replace(self, tof_mash_factor=tof_mash_factor)
+
+    @property
+    def is_tof_data(self) -> bool:
+        return self.tof_mash_factor > 0 and self.scanner.is_tof_ready
+
+    @property
+    def num_tof_bins(self) -> int:
+        return self.scanner.max_num_timing_poss // self.tof_mash_factor if self.is_tof_data else 1
+
+    @property
+    def min_tof_pos_num(self) -> int:
+        return -(self.num_tof_bins // 2) if self.is_tof_data else 0
+
+    @property
+    def tof_increment_in_mm(self) -> float:
+        # tof_delta_time_to_mm(mash * size_of_timing_pos) (TOF_conversions.h:29-32)
+        dt = np.float32(self.tof_mash_factor) * np.float32(self.scanner.size_of_timing_pos_ps)
+        return float(np.float32(float(dt) * SPEED_OF_LIGHT_MM_PER_PS_DIV2))
+
+    def tof_bin_boundaries_mm(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(low, high) of each TOF bin in mm, float32 as the reference computes them
+        (get_k, get_sampling_in_k, ``ProjDataInfo.cxx:69-91,233-246``)."""
+        n = self.num_tof_bins
+        inc = np.float32(self.tof_increment_in_mm)
+
+        def k_of(k):
+            if n % 2 == 0:
+                return np.float32(np.float32(k) * inc) + np.float32(inc / np.float32(2))
+            return np.float32(np.float32(k) * inc)
+
+        lo, hi = [], []
+        for k in range(self.min_tof_pos_num, self.min_tof_pos_num + n):
+            sampling = np.float32((k_of(k + 1) - k_of(k - 1)) / np.float32(2))
+            centre = k_of(k)
+            lo.append(np.float32(centre - np.float32(sampling / np.float32(2))))
+            hi.append(np.float32(centre + np.float32(sampling / np.float32(2))))
+        return np.array(lo, np.float32), np.array(hi, np.float32)
+
+    @property
+    def tof_r_sqrt2_gauss_sigma(self) -> float:
+        """1 / (sigma * sqrt(2)), sigma = FWHM in mm / 2.355 (``ProjMatrixByBin.cxx:66-75``)."""
+        fwhm_mm = np.float32(float(np.float32(self.scanner.timing_resolution_ps)) * SPEED_OF_LIGHT_MM_PER_PS_DIV2)
+        sigma = np.float32(fwhm_mm / np.float32(2.355))
+        return float(np.float32(np.float32(1.0) / np.float32(sigma * np.float32(math.sqrt(2.0)))))
 
     def restrict_segments(self, max_abs_segment: int) -> "ProjDataInfoCylindrical":
         """Keep segments in [-max_abs_segment, max_abs_segment]
@@ -341,6 +414,10 @@ class GeometryStruct(ctypes.Structure):
         ("do_symmetry_swap_segment", ctypes.c_int32),
         ("do_symmetry_swap_s", ctypes.c_int32),
         ("do_symmetry_shift_z", ctypes.c_int32),
+        ("num_tof_bins", ctypes.c_int32),
+        ("tof_r_sqrt2_gauss_sigma", ctypes.c_float),
+        ("tof_low", ctypes.c_float * MAX_TOF_BINS),
+        ("tof_high", ctypes.c_float * MAX_TOF_BINS),
     ]
 
 
@@ -378,6 +455,15 @@ def make_geometry_struct(pdi: ProjDataInfoCylindrical, img: VoxelsOnCartesianGri
     g.do_symmetry_swap_segment = int(pm.do_symmetry_swap_segment)
     g.do_symmetry_swap_s = int(pm.do_symmetry_swap_s)
     g.do_symmetry_shift_z = int(pm.do_symmetry_shift_z)
+    g.num_tof_bins = pdi.num_tof_bins
+    if pdi.is_tof_data:
+        if pdi.num_tof_bins > MAX_TOF_BINS:
+            raise ValueError("too many TOF bins")
+        lo, hi = pdi.tof_bin_boundaries_mm()
+        for k in range(pdi.num_tof_bins):
+            g.tof_low[k] = lo[k]
+            g.tof_high[k] = hi[k]
+        g.tof_r_sqrt2_gauss_sigma = pdi.tof_r_sqrt2_gauss_sigma
     return g
 
 
@@ -399,6 +485,27 @@ def config1(radius_mm: float = 328.0) -> Tuple[ProjDataInfoCylindrical, VoxelsOn
     sc = Scanner.Siemens_mMR(radius_mm)
     pdi = ProjDataInfoCylindrical.construct(sc, span=11, max_delta=60, num_views=252, num_tangential_poss=344)
     img = VoxelsOnCartesianGrid(127, 344, 344, 2.03125, f32(2.08626), f32(2.08626))
+    return pdi, img
+
+
+def config3(radius_mm: float = 328.0) -> Tuple[ProjDataInfoCylindrical, VoxelsOnCartesianGrid]:
+    """TOF at the mMR-like geometry: 11 TOF bins of 89.04 mm (594 ps), 400 ps
+    FWHM kernel; 11 x 72 557 856 bins."""
+    sc = Scanner.Siemens_mMR_TOF(radius_mm)
+    pdi = ProjDataInfoCylindrical.construct(sc, span=11, max_delta=60, num_views=252,
+                                            num_tangential_poss=344).with_tof(1)
+    img = VoxelsOnCartesianGrid(127, 344, 344, 2.03125, f32(2.08626), f32(2.08626))
+    return pdi, img
+
+
+def config0_tof(num_timing_poss: int = 9) -> Tuple[ProjDataInfoCylindrical, VoxelsOnCartesianGrid]:
+    """The 8-ring x 192-crystal geometry of config0 with a TOF description
+    (400 ps FWHM, timing positions of 594 ps = 89.04 mm)."""
+    sc = replace(Scanner.cylindrical(8, 192, 300.0, 4.0, name="cyl8x192_tof"), timing_resolution_ps=400.0,
+                 size_of_timing_pos_ps=594.0, max_num_timing_poss=num_timing_poss)
+    pdi = ProjDataInfoCylindrical.construct(sc, span=1, max_delta=3, num_views=96,
+                                            num_tangential_poss=128).with_tof(1)
+    img = VoxelsOnCartesianGrid(15, 128, 128, 2.0, 2.0, 2.0)
     return pdi, img
 
 

@@ -1,0 +1,165 @@
+"""Pins of the oracle's time-of-flight projector (apply_tof_kernel,
+ProjMatrixByBin.inl:115-197; FastErf.inl:30-80) on the 8-ring x 192-crystal
+geometry with 9 TOF bins of 89.04 mm and a 400 ps FWHM kernel:
+
+* the reference's own checks (test_time_of_flight.cxx:296-420): the TOF rows
+  of a bin sum to its non-TOF row, and the first and last TOF bins of an
+  oblique LOR through the centre mirror each other;
+* a closed form: each TOF weight / non-TOF weight equals
+  0.5 (erf((high - d)/(sigma sqrt 2)) - erf((low - d)/(sigma sqrt 2))) with d the
+  signed distance of the voxel centre from the LOR mid-point, computed here in
+  FP64 from the scanner geometry (no shared code with the oracle);
+* the symmetry restriction TOF imposes (only the z shift stays on);
+* adjoint identity and TOF-summed projections against non-TOF projections.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from stir_amd.geometry import RayTracingMatrixParameters, config0, config0_tof
+from oracle.oracle import OraclePlan
+
+
+@pytest.fixture(scope="module")
+def tof():
+    pdi, img = config0_tof()
+    pm = RayTracingMatrixParameters()
+    return pdi, img, OraclePlan(pdi, img, pm)
+
+
+@pytest.fixture(scope="module")
+def nontof():
+    pdi, img = config0()
+    return OraclePlan(pdi, img, RayTracingMatrixParameters(do_symmetry_90degrees_min_phi=False,
+                                                           do_symmetry_180degrees_min_phi=False,
+                                                           do_symmetry_swap_segment=False,
+                                                           do_symmetry_swap_s=False))
+
+
+def test_tof_layout_and_symmetries(tof):
+    pdi, img, O = tof
+    assert pdi.num_tof_bins == 9 and pdi.min_tof_pos_num == -4
+    lo, hi = pdi.tof_bin_boundaries_mm()
+    assert np.all(hi[:-1] == lo[1:])
+    assert abs(pdi.tof_increment_in_mm - 594 * 0.299792458 / 2) < 1e-3
+    assert O.num_bins == 9 * 540672
+    assert O.symmetry_flags() == (0, 0, 0, 0, 1)
+
+
+def test_tof_rows_sum_to_nontof_row(tof, nontof):
+    pdi, img, O = tof
+    rng = np.random.default_rng(2)
+    checked = 0
+    for _ in range(60):
+        seg = int(rng.integers(-3, 4))
+        b = (seg, int(rng.integers(0, 96)), int(rng.integers(0, pdi.num_axial_poss_of(seg))), int(rng.integers(-40, 40)))
+        z, y, x, w = nontof.row(*b)
+        if len(z) == 0:
+            continue
+        acc = {}
+        for k in range(pdi.num_tof_bins):
+            for zz, yy, xx, ww in zip(*O.row_tof(*b, k)):
+                acc[(zz, yy, xx)] = acc.get((zz, yy, xx), 0.0) + float(ww)
+        for zz, yy, xx, ww in zip(z, y, x, w):
+            assert acc.get((zz, yy, xx), 0.0) == pytest.approx(float(ww), rel=2e-6, abs=1e-7)
+        checked += 1
+    assert checked > 30
+
+
+def test_first_and_last_tof_bins_mirror(tof):
+    pdi, img, O = tof
+    seg = pdi.max_segment_num
+    ax = (pdi.num_axial_poss_of(seg) - 1) // 2
+    last = O.row_tof(seg, 0, ax, 0, pdi.num_tof_bins - 2)
+    first = O.row_tof(seg, 0, ax, 0, 1)
+    assert len(first[0]) == len(last[0]) > 0
+    np.testing.assert_allclose(last[3][::-1], first[3], rtol=1e-4, atol=1e-9)
+
+
+def _lor_fp64(pdi, seg, view, ax, tang):
+    """LOR mid-point and unit direction in scanner coordinates (z, y, x),
+    FP64: detector angles phi +- asin(s/R), ends at z = m -+ a tan(theta)."""
+    sc = pdi.scanner
+    R = sc.effective_ring_radius
+    s = R * math.sin(tang * math.pi / sc.num_detectors_per_ring)
+    phi = view * math.pi / pdi.num_views
+    i = seg - pdi.min_segment_num
+    delta = (pdi.min_ring_diff[i] + pdi.max_ring_diff[i]) / 2
+    tt = delta * sc.ring_spacing / (2 * math.sqrt(R * R - s * s))
+    dz = sc.ring_spacing / (2 if pdi.max_ring_diff[i] != pdi.min_ring_diff[i] else 1)
+    m = ax * dz - (pdi.num_axial_poss[i] - 1) * dz / 2
+    a = math.sqrt(R * R - s * s)
+    beta = math.asin(s / R)
+    p1 = np.array([m - a * tt, -R * math.cos(phi + beta), R * math.sin(phi + beta)])
+    p2 = np.array([m + a * tt, -R * math.cos(phi - beta + math.pi), R * math.sin(phi - beta + math.pi)])
+    mid = (p1 + p2) / 2
+    u = (p2 - mid) / np.linalg.norm(p2 - mid)
+    return mid, u
+
+
+def test_tof_weights_closed_form(tof, nontof):
+    pdi, img, O = tof
+    lo, hi = pdi.tof_bin_boundaries_mm()
+    sigma = 400 * 0.299792458 / 2 / 2.355
+    z_off = (pdi.scanner.num_rings - 1) / 2 * pdi.scanner.ring_spacing
+    worst = 0.0
+    for b in [(0, 10, 3, 5), (2, 30, 1, -17), (-3, 70, 4, 22), (1, 95, 0, 0), (-1, 47, 6, -30)]:
+        seg, view, ax, tang = b
+        mid, u = _lor_fp64(pdi, seg, view, 0, tang)
+        z, y, x, w = nontof.row(*b)
+        wmap = {(zz, yy, xx): float(ww) for zz, yy, xx, ww in zip(z, y, x, w)}
+        for k in range(pdi.num_tof_bins):
+            for zz, yy, xx, wt in zip(*O.row_tof(*b, k)):
+                zb = zz - 2 * ax  # span 1: two 2 mm planes per axial position
+                vox = np.array([zb * img.vox_z - z_off, yy * img.vox_y, xx * img.vox_x])
+                d = -float(np.dot(vox - mid, u))
+                f = 0.5 * (math.erf((hi[k] - d) / (sigma * math.sqrt(2))) - math.erf((lo[k] - d) / (sigma * math.sqrt(2))))
+                ratio = float(wt) / wmap[(zz, yy, xx)]
+                worst = max(worst, abs(ratio - f))
+                assert ratio == pytest.approx(f, rel=1e-4, abs=2e-6), (b, k, zz, yy, xx)
+    print(f"closed form: max |weight ratio - kernel| {worst:.2e}")
+
+
+def test_tof_adjoint_fp64(tof):
+    pdi, img, O = tof
+    rng = np.random.default_rng(8)
+    x = rng.uniform(0, 1, img.shape)
+    y = rng.uniform(0, 1, O.num_bins)
+    lhs = float(np.dot(O.forward_project_f64(x), y))
+    rhs = float(np.dot(x.reshape(-1), O.back_project_f64(y).reshape(-1)))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_tof_sum_of_projections_matches_nontof(tof, nontof):
+    pdi, img, O = tof
+    x = np.random.default_rng(9).uniform(0, 1, img.shape).astype(np.float32)
+    t = O.forward_project(x).astype(np.float64).reshape(pdi.num_tof_bins, -1).sum(axis=0)
+    n = nontof.forward_project(x).astype(np.float64)
+    np.testing.assert_allclose(t, n, rtol=1e-5, atol=1e-5 * n.max())
+    # and the back projection of TOF-replicated data
+    yv = np.random.default_rng(10).uniform(0, 1, n.size).astype(np.float32)
+    bt = O.back_project(np.tile(yv, pdi.num_tof_bins))
+    bn = nontof.back_project(yv)
+    np.testing.assert_allclose(bt, bn, rtol=1e-4, atol=1e-5 * bn.max())
+
+
+def test_tof_planted_sign_error_fails_closed_form(tof, nontof):
+    """The closed form tells the two TOF directions apart: with the sign of d
+    flipped (a plausible slip in the mid-point convention) it no longer
+    matches the oracle's weights."""
+    pdi, img, O = tof
+    lo, hi = pdi.tof_bin_boundaries_mm()
+    sigma = 400 * 0.299792458 / 2 / 2.355
+    z_off = (pdi.scanner.num_rings - 1) / 2 * pdi.scanner.ring_spacing
+    b = (1, 20, 0, 5)
+    mid, u = _lor_fp64(pdi, b[0], b[1], 0, b[3])
+    w = {(zz, yy, xx): float(ww) for zz, yy, xx, ww in zip(*nontof.row(*b))}
+    mismatch = 0
+    for k in range(pdi.num_tof_bins):
+        for zz, yy, xx, wt in zip(*O.row_tof(*b, k)):
+            vox = np.array([zz * img.vox_z - z_off, yy * img.vox_y, xx * img.vox_x])
+            d = float(np.dot(vox - mid, u))  # sign flipped
+            f = 0.5 * (math.erf((hi[k] - d) / (sigma * math.sqrt(2))) - math.erf((lo[k] - d) / (sigma * math.sqrt(2))))
+            mismatch += abs(float(wt) / w[(zz, yy, xx)] - f) > 1e-3
+    assert mismatch > 10
