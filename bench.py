@@ -36,11 +36,13 @@ def main():
     ap.add_argument("--skip-osem", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-autograd", action="store_true")
+    ap.add_argument("--skip-tof", action="store_true")
+    ap.add_argument("--tof-steps", type=int, default=3)
     ap.add_argument("--autograd-steps", type=int, default=3)
     ap.add_argument("--cpu-subsets", type=int, default=1, help="subsets of 21 in the CPU baseline sample")
     ap.add_argument("--radius", type=float, default=328.0)
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="run one 256 MiB device copy first (a kernel of known traffic for rocprof counters)")
+                    help="run one 256 MiB read + 256 MiB write element-wise kernel first (known traffic for rocprof counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -57,9 +59,11 @@ def main():
 
     dev = torch.device("cuda:0")
     if args.copy_calibration:
+        # one element-wise kernel that reads 256 MiB and writes 256 MiB: the
+        # reference for scaling rocprof FETCH_SIZE / WRITE_SIZE to bytes
         src = torch.ones(64 * 2**20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.add(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     pdi, img = config1(args.radius)
@@ -195,6 +199,9 @@ def main():
     if not args.skip_autograd:
         result["autograd"] = autograd_bench(pair, obj, img, pdi, dev, args.autograd_steps, ms_per_step)
 
+    if not args.skip_tof:
+        result["tof"] = tof_bench(dev, x, args.tof_steps, nsub, args.radius)
+
     if not args.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(pdi, img, pm, x_true, nsub, args.cpu_subsets)
     print(json.dumps(result), flush=True)
@@ -237,6 +244,67 @@ def autograd_bench(pair, obj, img, pdi, dev, steps, pair_ms):
             "gradient_vs_pair": round(per_image_ms / pair_ms, 4), "batch": int(xs.shape[0]),
             "loss": "Poisson NLL (mean 2/bin, b=0.05) + 0.01 * QuadraticPrior, log-normal(0, 0.5) images",
             "adjoint_rel_err_fp64": abs(lhs - rhs) / abs(lhs)}
+
+
+def tof_bench(dev, x, steps, nsub, radius):
+    """TOF projector pairs at the mMR-like geometry with 11 TOF bins of 89 mm
+    (400 ps FWHM), same phantom; one OSEM iteration on Poisson data of mean
+    0.3/bin (+0.05)."""
+    import numpy as np
+    import torch
+
+    from stir_amd import phantoms
+    from stir_amd.geometry import RayTracingMatrixParameters, config3
+    from stir_amd.projectors import (OSMAPOSLReconstruction, PoissonLogLikelihoodWithLinearModelForMeanAndProjData,
+                                     ProjectorByBinPairMI355X, ProjMatrixByBinUsingRayTracing)
+
+    pdi, img = config3(radius)
+    t0 = time.time()
+    pair = ProjectorByBinPairMI355X(ProjMatrixByBinUsingRayTracing(RayTracingMatrixParameters()))
+    pair.set_up(pdi, img, num_subsets=nsub)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    info = pair.info()
+    log(f"TOF set-up {setup_s:.2f}s: {pdi.num_bins} bins, table {info.num_table_entries}, "
+        f"device {info.device_bytes / 2**30:.2f} GiB")
+    sino = torch.empty(pdi.num_bins, dtype=torch.float32, device=dev)
+    back = torch.empty(img.shape, dtype=torch.float32, device=dev)
+    pair.forward_project(x, out=sino)
+    pair.back_project(sino, out=back)
+    torch.cuda.synchronize()
+    ctx = pair.context
+    ctx.kernel_timing()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pair.forward_project(x, out=sino)
+        pair.back_project(sino, out=back)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    fp_ms, nfp, bp_ms, nbp = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    out = {"pairs_per_s": round(1.0 / dt, 4), "ms_per_pair": round(dt * 1000, 3),
+           "fp_kernel_ms": round(fp_ms / max(nfp, 1), 3), "bp_kernel_ms": round(bp_ms / max(nbp, 1), 3),
+           "num_bins": int(pdi.num_bins), "tof_bins": int(pdi.num_tof_bins), "setup_s": round(setup_s, 3)}
+    log(f"TOF pair {dt * 1000:.2f} ms (fp {out['fp_kernel_ms']} ms, bp {out['bp_kernel_ms']} ms)")
+    y, _ = phantoms.poisson_data(sino.cpu().numpy(), 0.3, 0.05, seed=303)
+    del sino, back
+    obj = PoissonLogLikelihoodWithLinearModelForMeanAndProjData(pair)
+    obj.set_up(y, 0.05)
+    del y
+    recon = OSMAPOSLReconstruction(obj, num_subiterations=nsub)
+    obj.set_estimate(np.ones(img.shape, np.float32), enforce_positivity=True)
+    recon.update_estimate(0, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(2, nsub + 2):
+        recon.update_estimate((k - 1) % nsub, k)
+    torch.cuda.synchronize()
+    it_s = time.perf_counter() - t0
+    out["osem_ms_per_iteration"] = round(it_s * 1000, 3)
+    log(f"TOF OSEM iteration {it_s * 1000:.2f} ms")
+    pair.close()
+    return out
 
 
 def cpu_baseline(pdi, img, pm, x_true, nsub, nsample):

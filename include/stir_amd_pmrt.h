@@ -174,6 +174,11 @@ int mi_obj_set_data(mi_ctx* ctx, const float* measured, const float* additive, f
 /* Subset sensitivities BP_subset(1) (compute_sensitivities,
  * PoissonLogLikelihoodWithLinearModelForMean.cxx:374-455, use subset sensitivities). */
 int mi_obj_compute_sensitivities(mi_ctx* ctx);
+/* "use time-of-flight sensitivities" (...AndProjData.cxx:143, default 0): with
+ * TOF data and 0 the sensitivities are non-TOF back projections
+ * (:533-536,648-670), with 1 TOF back projections of TOF ones. Takes effect at
+ * the next mi_obj_compute_sensitivities. */
+int mi_obj_set_use_tof_sensitivities(mi_ctx* ctx, int use);
 int mi_obj_get_sensitivity(mi_ctx* ctx, int subset, float* image);
 /* Current estimate (host, STIR layout). mi_obj_set_estimate applies
  * threshold_min_to_small_positive_value(x, 1e-6) when enforce_positivity
@@ -244,9 +249,9 @@ int mi_synchronize(mi_ctx* ctx);
 /* ---- launch tuning ---------------------------------------------------------- */
 /* No counterpart in the reference. Launch shape of the back projection: image
  * columns are visited in tiles of MI_TUNE_BP_TILE_X x MI_TUNE_BP_TILE_Y
- * (default 1 x 1, raster order), with or without the XCD-contiguous block
- * remap (default on), and with or without skipping z slots an entry cannot
- * reach (default off). Results do not depend on these settings. */
+ * (default 32 x 8; 1 x 1 is raster order), with or without the XCD-contiguous
+ * block remap (default off), and with or without skipping z slots an entry
+ * cannot reach (default off). Results do not depend on these settings. */
 enum { MI_TUNE_BP_TILE_X = 1, MI_TUNE_BP_TILE_Y = 2, MI_TUNE_BP_XCD_REMAP = 3, MI_TUNE_BP_SLOT_SKIP = 4 };
 int mi_set_tuning(mi_ctx* ctx, int param, int value);
 

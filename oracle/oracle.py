@@ -63,6 +63,7 @@ def lib():
             fn.argtypes = [vp, _f64p, _f64p, ctypes.c_int, ctypes.c_int]
         L.or_nnz.restype = ctypes.c_int64
         L.or_nnz.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.or_set_use_tof_sensitivities.argtypes = [vp, ctypes.c_int]
         L.or_subset_sensitivity.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
         L.or_subset_gradient.argtypes = [vp, _f32p, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, _f32p]
@@ -235,6 +236,9 @@ class OraclePlan:
 
     def nnz(self, subset=-1, num_subsets=1) -> int:
         return int(lib().or_nnz(self._p, subset, num_subsets))
+
+    def set_use_tof_sensitivities(self, use: bool) -> None:
+        lib().or_set_use_tof_sensitivities(self._p, int(use))
 
     def subset_sensitivity(self, subset, num_subsets, zero_end_planes=False):
         out = np.zeros(self.img.shape, np.float32)

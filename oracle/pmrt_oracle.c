@@ -148,6 +148,8 @@ struct or_plan {
   int ntof;
   float z_offset;  /* scanner z of image plane 0 subtracted in apply_tof_kernel */
   double* erf_tab;  /* FastErf samples */
+  int tofsens;      /* "use time-of-flight sensitivities" (default 0) */
+  int nt_sym[4];    /* 90, 180, swap segment, swap s before the TOF restriction */
 };
 
 #define SEGI(p, s) ((s) - (p)->g.min_segment_num)
@@ -703,8 +705,6 @@ or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
   p->ntof = g->num_tof_bins > 1 ? g->num_tof_bins : 1;
   if (p->ntof > OR_MAX_TOF) { snprintf(err, errlen, "too many TOF bins"); free(p); return NULL; }
   if (p->ntof > 1) {
-    /* TOF keeps only the z-shift symmetry (DataSymmetriesForBins_PET_CartesianGrid.cxx:345-366) */
-    p->sym90 = p->sym180 = p->symseg = p->syms = 0;
     /* ProjMatrixByBin.inl:128-131 */
     p->z_offset = (g->num_rings - 1) / 2.0F * g->ring_spacing;
     p->erf_tab = erf_table();
@@ -719,6 +719,12 @@ or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
     if (fabsf(g->origin_x) > .01F || fabsf(g->origin_y) > .01F) p->sym90 = p->sym180 = p->symseg = p->syms = 0;
     if (g->min_segment_num > g->max_segment_num) { free(p); return NULL; }
     if (fabsf(get_tantheta(p, 0, 0)) > 1.E-4F) { snprintf(err, errlen, "segment 0 is not direct"); free(p); return NULL; }
+  }
+  if (p->ntof > 1) {
+    /* TOF keeps only the z-shift symmetry (DataSymmetriesForBins_PET_CartesianGrid.cxx:345-366);
+     * the flags without that rule are those of the non-TOF sensitivity projector */
+    p->nt_sym[0] = p->sym90; p->nt_sym[1] = p->sym180; p->nt_sym[2] = p->symseg; p->nt_sym[3] = p->syms;
+    p->sym90 = p->sym180 = p->symseg = p->syms = 0;
   }
   /* find_relation_between_coordinate_systems -- .cxx:51-125 */
   {
@@ -811,7 +817,7 @@ static const row* row_and_op(const or_plan* p, int seg, int view, int ax, int ta
 /* weight of element k of a basic row for TOF bin kk (the plain value for non-TOF) */
 static float elem_w(const or_plan* p, const row* b, int k, int kk)
 {
-  return p->ntof > 1 ? b->tw[(size_t)k * p->ntof + kk] : b->e[k].w;
+  return (p->ntof > 1 && kk >= 0) ? b->tw[(size_t)k * p->ntof + kk] : b->e[k].w;
 }
 /* index of a bin in (TOF) projection data: TOF bin outermost */
 static int64_t tbin(const or_plan* p, int seg, int view, int ax, int tang, int kk)
@@ -1184,8 +1190,35 @@ static int is_end_plane(const or_plan* p, int seg, int ax)
 /* RPC_process_related_viewgrams_sensitivity_computation -- ...AndProjData.cxx:1517-1539,
  * with mult viewgrams = ones and zeroed seg-0 end planes when requested
  * (distributable.cxx:200-223). */
+void or_set_use_tof_sensitivities(or_plan* p, int use) { p->tofsens = use != 0; }
+
+/* With TOF data and "use time-of-flight sensitivities" off (the default,
+ * ...AndProjData.cxx:99,533-536,648-670) the sensitivity is the non-TOF back
+ * projection of ones: every element with its non-TOF value. */
+static void subset_sensitivity_nontof(const or_plan* p, int subset, int nsub, int zero_end_planes, float* img)
+{
+  memset(img, 0, sizeof(float) * (size_t)p->g.nz * p->g.ny * p->g.nx);
+  /* the subset's view-segments are those of the non-TOF projector's symmetries
+   * (add_subset_sensitivity uses sens_symmetries_sptr, ...AndProjData.cxx:884-903) */
+  or_plan q = *p;
+  q.sym90 = p->nt_sym[0]; q.sym180 = p->nt_sym[1]; q.symseg = p->nt_sym[2]; q.syms = p->nt_sym[3];
+  vs_t* vs; const int n = all_vs(&q, subset, nsub, &vs);
+  for (int k = 0; k < n; ++k) {
+    const int seg = vs[k].seg, view = vs[k].view;
+    const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
+      for (int a = 0; a < nax; ++a)
+        bp_bin(p, img, (zero_end_planes && is_end_plane(p, seg, a)) ? 0.F : 1.F, seg, view, a, t, -1);
+  }
+  free(vs);
+}
+
 void or_subset_sensitivity(const or_plan* p, int subset, int nsub, int zero_end_planes, float* img)
 {
+  if (p->ntof > 1 && !p->tofsens) {
+    subset_sensitivity_nontof(p, subset, nsub, zero_end_planes, img);
+    return;
+  }
   const int64_t nb = p->num_bins * p->ntof;
   float* ones = (float*)malloc(sizeof(float) * (size_t)nb);
   for (int64_t i = 0; i < nb; ++i) ones[i] = 1.F;

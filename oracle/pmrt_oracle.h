@@ -152,6 +152,9 @@ int64_t or_nnz(const or_plan* p, int subset, int num_subsets);
  * bkg may be NULL (no additive term) and is a full sinogram otherwise.
  * zero_end_planes: "zero end planes of segment 0". */
 void or_subset_sensitivity(const or_plan* p, int subset, int num_subsets, int zero_end_planes, float* img);
+/* "use time-of-flight sensitivities" (...AndProjData.cxx:143): off by default,
+ * then TOF data get the non-TOF sensitivity (:533-536,648-670). */
+void or_set_use_tof_sensitivities(or_plan* p, int use);
 /* compute_sub_gradient_without_penalty(_plus_sensitivity):
  * grad = BP_subset(y/ybar) (+sens) or BP_subset(y/ybar - 1). */
 void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, const float* x,

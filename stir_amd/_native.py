@@ -25,7 +25,7 @@ EXPORTED_SYMBOLS = (
     "mi_obj_set_estimate", "mi_obj_get_estimate", "mi_osem_subset_step", "mi_osem_run", "mi_obj_subset_gradient",
     "mi_obj_log_likelihood", "mi_synchronize", "mi_set_kernel_timing", "mi_get_kernel_timing",
     "mi_symmetry_operations", "mi_set_tuning", "mi_obj_value_and_gradient_dev", "mi_quadratic_prior_dev",
-    "mi_osem_subset_step_dev", "mi_osem_subset_step_vjp_dev",
+    "mi_osem_subset_step_dev", "mi_osem_subset_step_vjp_dev", "mi_obj_set_use_tof_sensitivities",
 )
 
 MI_TUNE_BP_TILE_X, MI_TUNE_BP_TILE_Y, MI_TUNE_BP_XCD_REMAP, MI_TUNE_BP_SLOT_SKIP = 1, 2, 3, 4
@@ -84,6 +84,7 @@ def load():
     L.mi_back_project_dev.argtypes = [vp, vp, vp, c_int, vp]
     L.mi_obj_set_data.argtypes = [vp, _f32p, vp, ctypes.c_float, c_int]
     L.mi_obj_compute_sensitivities.argtypes = [vp]
+    L.mi_obj_set_use_tof_sensitivities.argtypes = [vp, c_int]
     L.mi_obj_get_sensitivity.argtypes = [vp, c_int, _f32p]
     L.mi_obj_set_estimate.argtypes = [vp, _f32p, c_int]
     L.mi_obj_get_estimate.argtypes = [vp, _f32p]

@@ -106,11 +106,11 @@ def back_project(p: torch.Tensor, pair: ProjectorByBinPairMI355X, subset: int = 
 # objective
 class _NegLogLikelihood(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x: torch.Tensor, obj: PoissonLogLikelihoodWithLinearModelForMeanAndProjData, subset: int):
+    def forward(ctx, x: torch.Tensor, obj: PoissonLogLikelihoodWithLinearModelForMeanAndProjData, subset: int,
+                want_grad: bool):
         pair = obj.projector
         nvox = pair.img.num_voxels
         xs = _as_batch(x, nvox, "image")
-        want_grad = torch.is_grad_enabled() and x.requires_grad
         values = torch.empty(xs.shape[0], dtype=torch.float64, device=x.device)
         grads = torch.empty_like(xs) if want_grad else None
         L, st = _native.load(), _stream(x)
@@ -129,7 +129,7 @@ class _NegLogLikelihood(torch.autograd.Function):
         (grads,) = ctx.saved_tensors
         # d(-loglik)/dx = -(sub-gradient)
         scale = (-g.reshape(-1, 1)).to(torch.float32)
-        return (grads * scale).reshape(ctx.in_shape), None, None
+        return (grads * scale).reshape(ctx.in_shape), None, None, None
 
 
 class PoissonLogLikelihoodLoss:
@@ -145,16 +145,18 @@ class PoissonLogLikelihoodLoss:
         self.subset = subset
 
     def __call__(self, x: torch.Tensor) -> torch.Tensor:
-        return _NegLogLikelihood.apply(x, self.objective, self.subset)
+        # grad mode is off inside Function.forward: decide here whether the
+        # gradient image is computed along with the value
+        want_grad = torch.is_grad_enabled() and x.requires_grad
+        return _NegLogLikelihood.apply(x, self.objective, self.subset, want_grad)
 
 
 class _QuadraticPrior(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x: torch.Tensor, prior: "QuadraticPriorLoss"):
+    def forward(ctx, x: torch.Tensor, prior: "QuadraticPriorLoss", want_grad: bool):
         pair = prior.pair
         nvox = pair.img.num_voxels
         xs = _as_batch(x, nvox, "image")
-        want_grad = torch.is_grad_enabled() and x.requires_grad
         values = torch.empty(xs.shape[0], dtype=torch.float64, device=x.device)
         grads = torch.empty_like(xs) if want_grad else None
         L, st = _native.load(), _stream(x)
@@ -172,7 +174,7 @@ class _QuadraticPrior(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g: torch.Tensor):
         (grads,) = ctx.saved_tensors
-        return (grads * g.reshape(-1, 1).to(torch.float32)).reshape(ctx.in_shape), None
+        return (grads * g.reshape(-1, 1).to(torch.float32)).reshape(ctx.in_shape), None, None
 
 
 class QuadraticPriorLoss:
@@ -198,7 +200,8 @@ class QuadraticPriorLoss:
         return None if self._w is None else self._w.ctypes.data_as(ctypes.c_void_p)
 
     def __call__(self, x: torch.Tensor) -> torch.Tensor:
-        return _QuadraticPrior.apply(x, self)
+        want_grad = torch.is_grad_enabled() and x.requires_grad
+        return _QuadraticPrior.apply(x, self, want_grad)
 
 
 # ---------------------------------------------------------------------------

@@ -100,6 +100,15 @@ struct mi_ctx {
   int64_t num_matrix_elements = 0;
   int64_t num_nonzeros = 0;
   int uniform_ppa = 0;  // planes per axial position if equal for all families, else 0
+  // time of flight: ntof blocks of block_bins bins each
+  int ntof = 1;
+  int64_t block_bins = 0;
+  DBuf<mi::FamilyLor> lors;
+  DBuf<float> entry_d;   // TOF distance of every back-projection table entry
+  DBuf<double> erf_tab;  // FastErf samples
+  mi::TofConsts tofc{};
+  bool tof_sensitivities = false;  // "use time-of-flight sensitivities" (...AndProjData.cxx:143)
+  int64_t total_bins() const { return block_bins * ntof; }
   mi::BpTuning bp_tuning;
   // kernel timing
   bool timing = false;
@@ -155,23 +164,26 @@ struct mi_ctx {
   void image_from_device(const float* src, float* dst_stir, hipStream_t st) const {
     mi::launch_transpose(src, dst_stir, dims.ncols, dims.nz, dims.nzp, dims.ncols, st);
   }
+  // projection data: each TOF block (one block without TOF) is converted on its own
   void proj_to_device(const float* src_stir, float* dst, hipStream_t st) const {
     const mi_geometry& g = plan->geometry();
     const int cols = g.num_views * g.num_tangential_poss;
-    for (int seg : plan->segment_sequence()) {
-      const int64_t off = plan->segment_offset(seg);
-      const int nax = plan->num_axial(seg);
-      mi::launch_transpose(src_stir + off, dst + off, nax, cols, cols, nax, st);
-    }
+    for (int k = 0; k < ntof; ++k)
+      for (int seg : plan->segment_sequence()) {
+        const int64_t off = plan->segment_offset(seg) + k * block_bins;
+        const int nax = plan->num_axial(seg);
+        mi::launch_transpose(src_stir + off, dst + off, nax, cols, cols, nax, st);
+      }
   }
   void proj_from_device(const float* src, float* dst_stir, hipStream_t st) const {
     const mi_geometry& g = plan->geometry();
     const int cols = g.num_views * g.num_tangential_poss;
-    for (int seg : plan->segment_sequence()) {
-      const int64_t off = plan->segment_offset(seg);
-      const int nax = plan->num_axial(seg);
-      mi::launch_transpose(src + off, dst_stir + off, cols, nax, nax, cols, st);
-    }
+    for (int k = 0; k < ntof; ++k)
+      for (int seg : plan->segment_sequence()) {
+        const int64_t off = plan->segment_offset(seg) + k * block_bins;
+        const int nax = plan->num_axial(seg);
+        mi::launch_transpose(src + off, dst_stir + off, cols, nax, nax, cols, st);
+      }
   }
 
   // ---- projections on device-layout buffers
@@ -179,14 +191,25 @@ struct mi_ctx {
     const int tb = task_off[subset_begin(subset)], te = task_off[subset_end(subset)];
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
-    mi::launch_forward_project(mode, tasks.p + tb, te - tb, fams.p, pool.p, d_img, d_out, dims, data, st);
+    if (ntof > 1)
+      mi::launch_forward_project_tof(mode, tasks.p + tb, te - tb, fams.p, lors.p, pool.p, d_img, d_out, dims, tofc, data,
+                                     st);
+    else
+      mi::launch_forward_project(mode, tasks.p + tb, te - tb, fams.p, pool.p, d_img, d_out, dims, data, st);
     if (timing) { HIPCHECK(hipEventRecord(e1, st)); fp_events.emplace_back(e0, e1); }
   }
-  void bp(mi::BpMode mode, const float* d_proj, float* d_img, int subset, const mi::BpData& data, hipStream_t st) {
+  // nontof: back-project one non-TOF block with the non-TOF values of the
+  // table (the sensitivity of TOF data, unless TOF sensitivities are asked for)
+  void bp(mi::BpMode mode, const float* d_proj, float* d_img, int subset, const mi::BpData& data, hipStream_t st,
+          bool nontof = false) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
-    mi::launch_back_project(mode, uniform_ppa, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj, d_img,
-                            dims, data, bp_tuning, st);
+    if (ntof > 1 && !nontof)
+      mi::launch_back_project_tof(mode, uniform_ppa, list_off.p, entries.p, entry_d.p, subset_begin(subset),
+                                  subset_end(subset), d_proj, d_img, dims, tofc, data, bp_tuning, st);
+    else
+      mi::launch_back_project(mode, uniform_ppa, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj,
+                              d_img, dims, data, bp_tuning, st);
     if (timing) { HIPCHECK(hipEventRecord(e1, st)); bp_events.emplace_back(e0, e1); }
   }
 
@@ -370,7 +393,7 @@ void mi_ctx::build_tasks() {
       taken[f] = 1;
       if (h_fams[f].nax > 256) throw ApiError(MI_ERR_ARGUMENT, "more than 256 axial positions in a segment");
       int partner = -1;
-      if (keys[f].seg > 0) {
+      if (ntof == 1 && keys[f].seg > 0) {
         auto it = fam_lookup.find(fam_key(-keys[f].seg, keys[f].view, keys[f].tang, keys[f].a0));
         if (it != fam_lookup.end()) {
           const int g = it->second;
@@ -411,10 +434,14 @@ void mi_ctx::build_table() {
   DBuf<mi::BpEntry> raw;
   keys_in.alloc(std::max<int64_t>(n, 1));
   raw.alloc(std::max<int64_t>(n, 1));
+  DBuf<float> raw_d;
+  if (ntof > 1) raw_d.alloc(std::max<int64_t>(n, 1));
   for (int s = 0; s < nsub; ++s) {
     const int fb = plan->subset_family_begin(s), fe = plan->subset_family_begin(s + 1);
     mi::launch_count_entries(fams.p, fb, fe, pool.p, s, dims, counts.p, stream);
     mi::launch_make_entries(fams.p, fb, fe, d_famoff.p + fb, pool.p, s, dims, keys_in.p, raw.p, stream);
+    if (ntof > 1)
+      mi::launch_entry_tof_distance(fams.p, fb, fe, d_famoff.p + fb, lors.p, pool.p, dims, tofc, raw_d.p, stream);
   }
   HIPCHECK(hipGetLastError());
   // list offsets = exclusive scan of counts, plus the total at the end
@@ -445,6 +472,10 @@ void mi_ctx::build_table() {
     HIPCHECK(mi::radix_sort_pairs(tmp.p, tb, keys_in.p, keys_out.p, vals_in.p, vals_out.p, static_cast<int>(n), end_bit,
                                   stream));
     mi::launch_gather_entries(raw.p, vals_out.p, entries.p, n, stream);
+    if (ntof > 1) {
+      entry_d.alloc(n);
+      mi::launch_gather_floats(raw_d.p, vals_out.p, entry_d.p, n, stream);
+    }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(stream));
   }
@@ -457,6 +488,7 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   img.release(); img2.release(); sino.release(); ratio.release(); stage_img.release(); stage_sino.release();
   y.release(); add.release(); mult.release(); ones.release(); vg_small.release(); x.release(); sens.clear();
   vg_off.release(); vg_len.release(); partial.release(); prior_partial.release();
+  lors.release(); entry_d.release(); erf_tab.release();
   have_data = have_sens = have_x = false;
   plan.reset();
   plan = std::make_unique<mi::Plan>(g, nsub);
@@ -464,6 +496,37 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   dims.min_z = g.min_z; dims.min_y = g.min_y; dims.min_x = g.min_x;
   dims.nzp = plan->nz_padded();
   dims.ncols = plan->num_columns();
+  ntof = plan->num_tof_bins();
+  block_bins = plan->num_bins();
+  if (ntof > 1) {
+    for (const mi::Family& f : plan->families()) {
+      if (f.inplane != mi::IP_ID || f.zsign != 1) throw ApiError(MI_ERR_INTERNAL, "TOF family with an in-plane symmetry");
+      if (f.nax > 128) throw ApiError(MI_ERR_ARGUMENT, "TOF data with more than 128 axial positions in a segment");
+    }
+    const auto& hl = plan->family_lors();
+    lors.alloc(hl.size());
+    HIPCHECK(hipMemcpyAsync(lors.p, hl.data(), hl.size() * sizeof(mi::FamilyLor), hipMemcpyHostToDevice, stream));
+    // FastErf samples (FastErf.inl:30-40)
+    std::vector<double> tab(mi::kErfSamples + 2);
+    const double per = (2 * mi::kErfMax) / mi::kErfSamples;
+    for (int i = 0; i < mi::kErfSamples + 2; ++i) tab[i] = std::erf(i * per - mi::kErfMax);
+    erf_tab.alloc(tab.size());
+    HIPCHECK(hipMemcpyAsync(erf_tab.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    tofc = mi::TofConsts{};
+    tofc.ntof = ntof;
+    tofc.r = g.tof_r_sqrt2_gauss_sigma;
+    tofc.z_off = plan->tof_z_offset();
+    tofc.vz = g.vox_z; tofc.vy = g.vox_y; tofc.vx = g.vox_x;
+    tofc.oz = g.origin_z; tofc.oy = g.origin_y; tofc.ox = g.origin_x;
+    tofc.nvg = plan->num_viewgrams();
+    tofc.block = block_bins;
+    tofc.erf = erf_tab.p;
+    for (int k = 0; k < ntof; ++k) { tofc.low[k] = g.tof_low[k]; tofc.high[k] = g.tof_high[k]; }
+    tofc.shared_bounds = 1;
+    for (int k = 0; k + 1 < ntof; ++k)
+      if (g.tof_high[k] != g.tof_low[k + 1]) tofc.shared_bounds = 0;
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
   build_rows();
   build_tasks();
   build_table();
@@ -472,22 +535,24 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   img2.alloc(nvox_dev);
   HIPCHECK(hipMemsetAsync(img.p, 0, img.bytes(), stream));
   HIPCHECK(hipMemsetAsync(img2.p, 0, img2.bytes(), stream));
-  sino.alloc(plan->num_bins());
-  ratio.alloc(plan->num_bins());
+  sino.alloc(total_bins());
+  ratio.alloc(total_bins());
   stage_img.alloc(plan->num_voxels());
-  stage_sino.alloc(plan->num_bins());
+  stage_sino.alloc(total_bins());
   prior_partial.alloc(mi::prior_num_partials(g.nz, g.ny, g.nx));
   // viewgram extents in the device layout
   {
-    const int nvg = plan->num_viewgrams();
+    // TOF viewgram (segment, view, TOF bin k) has index k * num_viewgrams + viewgram_index
+    const int nvg1 = plan->num_viewgrams(), nvg = nvg1 * ntof;
     std::vector<int64_t> off(nvg, 0);
     std::vector<int32_t> len(nvg, 0);
-    for (int seg = g.min_segment_num; seg <= g.max_segment_num; ++seg)
-      for (int v = 0; v < g.num_views; ++v) {
-        const int i = plan->viewgram_index(seg, v);
-        off[i] = plan->device_bin_offset(seg, v, 0, g.min_tangential_pos_num);
-        len[i] = g.num_tangential_poss * plan->num_axial(seg);
-      }
+    for (int k = 0; k < ntof; ++k)
+      for (int seg = g.min_segment_num; seg <= g.max_segment_num; ++seg)
+        for (int v = 0; v < g.num_views; ++v) {
+          const int i = k * nvg1 + plan->viewgram_index(seg, v);
+          off[i] = plan->device_bin_offset(seg, v, 0, g.min_tangential_pos_num) + k * block_bins;
+          len[i] = g.num_tangential_poss * plan->num_axial(seg);
+        }
     vg_off.alloc(nvg);
     vg_len.alloc(nvg);
     HIPCHECK(hipMemcpyAsync(vg_off.p, off.data(), nvg * sizeof(int64_t), hipMemcpyHostToDevice, stream));
@@ -538,7 +603,7 @@ int mi_get_info(mi_ctx* ctx, mi_info* out) {
     if (!ctx || !out) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     ctx->require_setup();
     std::memset(out, 0, sizeof(*out));
-    out->num_bins = ctx->plan->num_bins();
+    out->num_bins = ctx->total_bins();
     out->num_voxels = ctx->plan->num_voxels();
     out->num_basic_rows = static_cast<int64_t>(ctx->plan->basic_rays().size());
     out->num_basic_elements = ctx->num_basic_elements;
@@ -710,7 +775,7 @@ int mi_obj_set_data(mi_ctx* ctx, const float* measured, const float* additive, f
     HIPCHECK(hipSetDevice(ctx->device));
     const mi_geometry& g = ctx->plan->geometry();
     hipStream_t st = ctx->stream;
-    const int64_t nb = ctx->plan->num_bins();
+    const int64_t nb = ctx->total_bins();
     ctx->y.alloc(nb);
     HIPCHECK(hipMemcpyAsync(ctx->stage_sino.p, measured, nb * sizeof(float), hipMemcpyHostToDevice, st));
     ctx->proj_to_device(ctx->stage_sino.p, ctx->y.p, st);
@@ -729,21 +794,25 @@ int mi_obj_set_data(mi_ctx* ctx, const float* measured, const float* additive, f
       // "zero end planes of segment 0" (distributable.cxx:218-223): y, b and the
       // multiplicative viewgrams get their first and last sinogram zeroed
       DBuf<int64_t> off;
-      off.alloc(1);
-      const int64_t o = ctx->plan->segment_offset(0);
-      HIPCHECK(hipMemcpyAsync(off.p, &o, sizeof(int64_t), hipMemcpyHostToDevice, st));
+      off.alloc(ctx->ntof);
+      std::vector<int64_t> o(ctx->ntof);
+      for (int k = 0; k < ctx->ntof; ++k) o[k] = ctx->plan->segment_offset(0) + k * ctx->block_bins;
+      HIPCHECK(hipMemcpyAsync(off.p, o.data(), o.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
       const int runs = g.num_views * g.num_tangential_poss, run = ctx->plan->num_axial(0);
-      mi::launch_zero_end_planes(ctx->y.p, off.p, runs, run, 0, st);
-      if (ctx->add.p) mi::launch_zero_end_planes(ctx->add.p, off.p, runs, run, 0, st);
       ctx->mult.alloc(nb);
       mi::launch_fill(ctx->mult.p, nb, 1.F, st);
-      mi::launch_zero_end_planes(ctx->mult.p, off.p, runs, run, 0, st);
+      for (int k = 0; k < ctx->ntof; ++k) {
+        mi::launch_zero_end_planes(ctx->y.p, off.p + k, runs, run, 0, st);
+        if (ctx->add.p) mi::launch_zero_end_planes(ctx->add.p, off.p + k, runs, run, 0, st);
+        mi::launch_zero_end_planes(ctx->mult.p, off.p + k, runs, run, 0, st);
+      }
       HIPCHECK(hipStreamSynchronize(st));
     } else {
       ctx->mult.release();
     }
-    ctx->vg_small.alloc(ctx->plan->num_viewgrams());
-    mi::launch_viewgram_max(ctx->y.p, ctx->vg_off.p, ctx->vg_len.p, ctx->plan->num_viewgrams(), ctx->vg_small.p, st);
+    const int nvg = ctx->plan->num_viewgrams() * ctx->ntof;
+    ctx->vg_small.alloc(nvg);
+    mi::launch_viewgram_max(ctx->y.p, ctx->vg_off.p, ctx->vg_len.p, nvg, ctx->vg_small.p, st);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
     ctx->have_data = true;
@@ -760,11 +829,41 @@ int mi_obj_compute_sensitivities(mi_ctx* ctx) {
     const int nsub = ctx->plan->num_subsets();
     ctx->sens.clear();
     const float* in = ctx->mult.p ? ctx->mult.p : ctx->ones.p;
-    for (int s = 0; s < nsub; ++s) {
-      auto b = std::make_unique<DBuf<float>>();
-      b->alloc(ctx->img.n);
-      ctx->bp(mi::BP_PLAIN, in, b->p, s, mi::BpData{}, ctx->stream);
-      ctx->sens.push_back(std::move(b));
+    if (ctx->ntof > 1 && !ctx->tof_sensitivities) {
+      // TOF data: the non-TOF sensitivity (...AndProjData.cxx:533-536,648-670),
+      // computed by the non-TOF projector, whose symmetries decide the
+      // view-segments of each subset (add_subset_sensitivity, :884-903). Each
+      // subset's viewgrams of the first (non-TOF) block of ones / mult are
+      // back-projected through the whole table.
+      mi_geometry g1 = ctx->plan->geometry();
+      g1.num_tof_bins = 1;
+      const mi::Plan nontof_plan(g1, nsub);
+      DBuf<float> mask;
+      mask.alloc(ctx->block_bins);
+      std::vector<int64_t> h_off(ctx->plan->num_viewgrams());
+      std::vector<int32_t> h_len(ctx->plan->num_viewgrams());
+      HIPCHECK(hipMemcpy(h_off.data(), ctx->vg_off.p, h_off.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(h_len.data(), ctx->vg_len.p, h_len.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (int s = 0; s < nsub; ++s) {
+        HIPCHECK(hipMemsetAsync(mask.p, 0, mask.bytes(), ctx->stream));
+        for (const auto& vs : nontof_plan.subset_view_segments(s)) {
+          const int vg = ctx->plan->viewgram_index(vs.second, vs.first);
+          HIPCHECK(hipMemcpyAsync(mask.p + h_off[vg], in + h_off[vg], h_len[vg] * sizeof(float),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        auto b = std::make_unique<DBuf<float>>();
+        b->alloc(ctx->img.n);
+        ctx->bp(mi::BP_PLAIN, mask.p, b->p, -1, mi::BpData{}, ctx->stream, true);
+        ctx->sens.push_back(std::move(b));
+      }
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+    } else {
+      for (int s = 0; s < nsub; ++s) {
+        auto b = std::make_unique<DBuf<float>>();
+        b->alloc(ctx->img.n);
+        ctx->bp(mi::BP_PLAIN, in, b->p, s, mi::BpData{}, ctx->stream);
+        ctx->sens.push_back(std::move(b));
+      }
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -1002,6 +1101,14 @@ int mi_osem_subset_step_vjp_dev(mi_ctx* ctx, const float* d_x_in, const float* d
     ctx->image_from_device(ctx->img2.p, d_grad_in, st);
     mi::launch_add_product(d_grad_in, d_grad_out, d_update, ctx->plan->num_voxels(), st);
     HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_obj_set_use_tof_sensitivities(mi_ctx* ctx, int use) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->tof_sensitivities = use != 0;
+    ctx->have_sens = false;
   });
 }
 

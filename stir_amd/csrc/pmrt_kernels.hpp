@@ -1,0 +1,148 @@
+// Device helpers shared by the projection kernels (pmrt_project.hip,
+// pmrt_tof.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pmrt_device.hpp"
+
+namespace mi {
+namespace dev {
+
+constexpr int kWave = 64;
+constexpr int kNumXcd = 8;
+
+// blockIdx -> work index so that each XCD (block i runs on XCD i % 8) gets a
+// contiguous range of work: neighbouring families / columns share L2 lines.
+__device__ inline int xcd_remap(int bid, int nblocks) {
+  const int per = (nblocks + kNumXcd - 1) / kNumXcd;
+  const int xcd = bid % kNumXcd;
+  const int idx = bid / kNumXcd;
+  const int full = nblocks % kNumXcd == 0 ? kNumXcd : nblocks % kNumXcd;  // XCDs that get `per` blocks
+  // XCDs [0, full) own `per` blocks, the others `per - 1`
+  const int start = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return start + idx;
+}
+
+// base[idx] with a wave-uniform base and a 32-bit unsigned lane offset, so
+// the address is formed as scalar base + 32-bit vector offset
+__device__ inline float load_u32off(const float* base, unsigned idx) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (idx << 2));
+}
+
+template <int IP>
+__device__ inline void inplane_op(int x, int y, int& xo, int& yo) {
+  if constexpr (IP == IP_ID) { xo = x; yo = y; }
+  else if constexpr (IP == IP_XMX) { xo = -x; yo = y; }
+  else if constexpr (IP == IP_YMY) { xo = x; yo = -y; }
+  else if constexpr (IP == IP_XMX_YMY) { xo = -x; yo = -y; }
+  else if constexpr (IP == IP_XY_YX) { xo = y; yo = x; }
+  else if constexpr (IP == IP_XMY_YX) { xo = -y; yo = x; }
+  else if constexpr (IP == IP_XY_YMX) { xo = y; yo = -x; }
+  else { xo = -y; yo = -x; }
+}
+
+__device__ inline void decode(int c, int& x, int& y, int& z) {
+  x = (c & 1023) - 512;
+  y = ((c >> 10) & 1023) - 512;
+  z = static_cast<int>(static_cast<unsigned>(c) >> 20) - 2048;
+}
+
+constexpr int kWavesPerBlock = 4;
+
+// Per-lane, per-slot bin of a forward-projection task.
+template <int NSLOT>
+struct LaneBins {
+  int zs[NSLOT];     // z sign of the symmetry operation
+  int zb[NSLOT];     // image z of the element z = 0 for this bin (zadd0 + ppa*a - min_z)
+  bool valid[NSLOT];
+  int64_t out[NSLOT];  // projection-data index
+  int vg[NSLOT];     // viewgram index
+};
+
+// Axial index of voxel zi for a table entry; `ok` tells whether the entry
+// contributes, the returned index is always inside the entry's run. PPA is
+// the number of image planes per axial position (0: not known at compile time).
+template <int PPA>
+__device__ inline unsigned entry_axial(int k, int nax_ppa, int zi, bool& ok) {
+  const int nax = nax_ppa & 0xffff;
+  const int diff = zi - k;
+  int a;
+  if constexpr (PPA == 1) {
+    a = diff;
+    ok = static_cast<unsigned>(a) < static_cast<unsigned>(nax);
+  } else if constexpr (PPA == 2) {
+    a = diff >> 1;
+    ok = (diff & 1) == 0 && static_cast<unsigned>(a) < static_cast<unsigned>(nax);
+  } else {
+    const int ppa = nax_ppa >> 16;
+    a = diff / ppa;
+    ok = diff >= 0 && a * ppa == diff && a < nax;
+  }
+  return static_cast<unsigned>(min(max(a, 0), nax - 1));
+}
+
+// work index -> image column, visiting the image in tile.x x tile.y tiles
+// (columns of one tile run at the same time and share projection-data runs);
+// returns -1 for the padding of partial tiles. A 1 x 1 tile is raster order.
+__device__ inline int tiled_column(int w, const ImageDims& d, int2 tile) {
+  const int tiles_x = (d.nx + tile.x - 1) / tile.x;
+  const int t = w / (tile.x * tile.y), r = w - t * (tile.x * tile.y);
+  const int y = (t / tiles_x) * tile.y + r / tile.x;
+  const int x = (t % tiles_x) * tile.x + r % tile.x;
+  return (y < d.ny && x < d.nx) ? y * d.nx + x : -1;
+}
+
+constexpr bool fp_writes_ratio(int m) {
+  return m == FP_RATIO || m == FP_GRADIENT || m == FP_GRADIENT_LOGLIK || m == FP_RATIO_LOGLIK;
+}
+constexpr bool fp_accumulates_loglik(int m) { return m == FP_LOGLIK || m == FP_GRADIENT_LOGLIK || m == FP_RATIO_LOGLIK; }
+
+// Forward-projection epilogue of one bin: a = (A x)_i, vg = its viewgram.
+template <int MODE>
+__device__ inline void fp_epilogue(float a, int64_t i, int vg, const FpData& data, float* __restrict__ out, double& ll) {
+  if constexpr (MODE == FP_PLAIN) {
+    out[i] = a;
+  } else if constexpr (MODE == FP_SCALE) {
+    out[i] = a * data.scale[i];
+  } else {
+    const float ybar = a + (data.additive ? data.additive[i] : data.additive_constant);
+    const float yv = data.measured[i];
+    const float small_value = data.vg_small[vg];
+    const float max_quotient = 10000.F;
+    if constexpr (fp_writes_ratio(MODE)) {
+      // estimated += additive; divide_and_truncate (recon_array_functions.cxx:172-266)
+      float num = yv;
+      if (num <= small_value) {
+        num = 0;
+      } else {
+        if (num > max_quotient * ybar) num = max_quotient;
+        else num = num / ybar;
+      }
+      if constexpr (MODE == FP_GRADIENT || MODE == FP_GRADIENT_LOGLIK) num = num - (data.mult ? data.mult[i] : 1.F);
+      out[i] = num;
+    }
+    if constexpr (MODE == FP_DRATIO) {
+      // derivative of the quotient above: -y/ybar^2 where y/ybar is used, 0 where truncated
+      float dr = 0.F;
+      if (yv > small_value && !(yv > max_quotient * ybar)) {
+        const float r = yv / ybar;
+        dr = -r / ybar;
+      }
+      out[i] = dr;
+    }
+    if constexpr (fp_accumulates_loglik(MODE)) {
+      // accumulate_loglikelihood (recon_array_functions.cxx:336-377)
+      float est = ybar;
+      if (data.mult) est = est * data.mult[i];
+      const float ne = (est < yv / 10000.F) ? yv / 10000.F : est;
+      if (yv <= small_value) ll += -static_cast<double>(ne);
+      else ll += yv * log(static_cast<double>(ne)) - static_cast<double>(ne);
+    }
+  }
+}
+
+}  // namespace dev
+}  // namespace mi

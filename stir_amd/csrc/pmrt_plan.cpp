@@ -63,7 +63,11 @@ Plan::Plan(const mi_geometry& g, int num_subsets) : g_(g), nsub_(num_subsets) {
   if (g.num_tangential_LORs < 1) fail("num_tangential_LORs should be at least 1");
   if (g.nx > 1022 || g.ny > 1022) fail("image x/y size above 1022 is not supported by the element packing");
   if (g.nz > 256) fail("images with more than 256 planes are not supported");
-  if (g.num_tof_bins > 1) fail("TOF projection data are not supported yet");
+  if (g.num_tof_bins > kMaxTofBins) fail("more than 16 TOF bins are not supported");
+  ntof_ = g.num_tof_bins > 1 ? g.num_tof_bins : 1;
+  // scanner z of image plane 0, subtracted from voxel positions in apply_tof_kernel
+  // (ProjMatrixByBin.inl:128-131)
+  if (ntof_ > 1) z_off_ = (g.num_rings - 1) / 2.0F * g.ring_spacing;
   max_tang_ = g.min_tangential_pos_num + g.num_tangential_poss - 1;
   max_z_ = g.min_z + g.nz - 1;
   max_y_ = g.min_y + g.ny - 1;
@@ -129,6 +133,8 @@ Plan::Plan(const mi_geometry& g, int num_subsets) : g_(g), nsub_(num_subsets) {
       fail("get_s must be odd in the tangential position");
     if (std::fabs(get_phi(0)) > 1.E-4F && (sym90_ || sym180_)) sym90_ = sym180_ = false;
     if (std::fabs(g.origin_x) > .01F || std::fabs(g.origin_y) > .01F) sym90_ = sym180_ = symseg_ = syms_ = false;
+    // TOF keeps only the z shift (DataSymmetriesForBins_PET_CartesianGrid.cxx:345-366)
+    if (ntof_ > 1) sym90_ = sym180_ = symseg_ = syms_ = false;
   }
   // ProjMatrixByBinUsingRayTracing.cxx:276-277
   if (std::fabs(g.origin_x) > .05F || std::fabs(g.origin_y) > .05F) fail("shifted x/y origin is not supported");
@@ -386,6 +392,52 @@ void Plan::build_basic_rays() {
   row_bound_ = static_cast<int>(per_trace * g.num_tangential_LORs * 2) * axial + 64;
 }
 
+// to_0_2pi for floats: modulo(float, float) (modulo.h:62-76,136-141)
+static float to_0_2pi_f(float phi) {
+  const float b = static_cast<float>(2 * kPi);
+  double r = std::fmod(static_cast<double>(phi), static_cast<double>(b));
+  r = r < 0 ? r + std::fabs(static_cast<double>(b)) : r;
+  float res = static_cast<float>(r);
+  if (res >= b) res -= b;
+  return res;
+}
+
+// LOR of a bin as apply_tof_kernel forms it (ProjMatrixByBin.inl:117-137):
+// get_LOR (ProjDataInfoCylindrical.cxx:515-547), the sinogram -> cylinder ->
+// two-point conversions (LORCoordinates.inl:106-139,287-298), mid-point and
+// unit direction. Float arithmetic with the float libm overloads.
+void Plan::tof_lor(int seg, int view, int ax, int tang, FamilyLor& out) const {
+  const float s = get_s(tang), m = get_m(seg, ax), tt = get_tantheta(seg, tang), phi = get_phi(view);
+  const float R = radius_;
+  const float max_a = sqrtf(R * R - s * s), min_a = -max_a;
+  float z1 = m - max_a * tt, z2 = m - min_a * tt;
+  float ph = to_0_2pi_f(phi), beta = asinf(s / R);
+  bool swapped = false;
+  if (static_cast<double>(ph) >= kPi) {
+    ph -= static_cast<float>(kPi);
+    beta = -beta;
+    std::swap(z1, z2);
+    swapped = true;
+  }
+  const float psi1 = to_0_2pi_f(ph + beta), psi2 = to_0_2pi_f(ph - beta + static_cast<float>(kPi));
+  float p1[3] = {z1, -R * cosf(psi1), R * sinf(psi1)};
+  float p2[3] = {z2, -R * cosf(psi2), R * sinf(psi2)};
+  if (swapped)
+    for (int i = 0; i < 3; ++i) std::swap(p1[i], p2[i]);
+  float mid[3], diff[3];
+  for (int i = 0; i < 3; ++i) {
+    mid[i] = (p1[i] + p2[i]) * 0.5F;
+    diff[i] = p2[i] - mid[i];
+  }
+  float acc = 0.F;
+  acc += diff[0] * diff[0];
+  acc += diff[1] * diff[1];
+  acc += diff[2] * diff[2];
+  const float n = static_cast<float>(std::sqrt(static_cast<double>(acc)));
+  out.mz = mid[0]; out.my = mid[1]; out.mx = mid[2];
+  out.uz = diff[0] / n; out.uy = diff[1] / n; out.ux = diff[2] / n;
+}
+
 void Plan::build_families() {
   const mi_geometry& g = g_;
   sub_fam_off_.assign(nsub_ + 1, 0);
@@ -413,6 +465,11 @@ void Plan::build_families() {
           f.viewgram = viewgram_index(seg, view);
           fams_.push_back(f);
           fam_keys_.push_back({seg, view, tang, a0, bi});
+          if (ntof_ > 1) {
+            FamilyLor lor;
+            tof_lor(bs, bv, ba, bt, lor);
+            fam_lor_.push_back(lor);
+          }
         }
       }
     }

@@ -18,6 +18,9 @@
 
 namespace mi {
 
+// TOF bins the kernels support
+constexpr int kMaxTofBins = 16;
+
 // In-plane part of a symmetry operation: where (x,y) of a basic-row element
 // goes. Named after SymmetryOperations_PET_CartesianGrid.h:48-464.
 enum InPlane : int8_t {
@@ -79,6 +82,14 @@ struct Family {
   int32_t pad1;
 };
 static_assert(sizeof(Family) == 32, "Family layout");
+
+// Mid-point and unit direction (z, y, x, scanner coordinates, mm) of the LOR
+// of a family's basic bin, as ProjMatrixByBin::apply_tof_kernel forms them
+// (ProjMatrixByBin.inl:117-137). TOF only.
+struct FamilyLor {
+  float mz, my, mx;
+  float uz, uy, ux;
+};
 
 // Device-layout position of a family in ProjDataInMemory order, for layout
 // conversion and for mi_get_row.
@@ -147,6 +158,13 @@ class Plan {
   // upper bound on the length of any basic row (tracer scratch sizing)
   int row_length_bound() const { return row_bound_; }
 
+  // time of flight: number of TOF bins (1 = non-TOF), the LOR of every
+  // family (TOF only), and the scanner z of image plane 0
+  int num_tof_bins() const { return ntof_; }
+  const std::vector<FamilyLor>& family_lors() const { return fam_lor_; }
+  float tof_z_offset() const { return z_off_; }
+  void tof_lor(int seg, int view, int ax, int tang, FamilyLor& out) const;
+
  private:
   int find_transform_z(int seg, int ax) const;
   SymOp op_bin0(int seg, int view, int ax) const;
@@ -165,6 +183,9 @@ class Plan {
   std::vector<int> ppa_;
   int ppr_;
   bool sym90_, sym180_, symseg_, syms_, symz_;
+  int ntof_ = 1;
+  float z_off_ = 0.F;
+  std::vector<FamilyLor> fam_lor_;
   std::vector<int> seg_seq_;
   std::vector<int64_t> seg_off_;
   std::vector<int> seg_pos_;  // position of a segment in the storage sequence

@@ -1,0 +1,426 @@
+// Time-of-flight forward and back projection.
+//
+// TOF rows are the non-TOF basic rows with each value multiplied by the TOF
+// kernel of the element's position along the LOR
+// (ProjMatrixByBin::apply_tof_kernel, ProjMatrixByBin.inl:115-197). TOF keeps
+// only the z-shift symmetry, so every family is one (segment, view,
+// tangential position) with its own row, and the kernel value of an element
+// is the same for all axial positions of the family. The weights are not
+// stored: a wave computes them for 64 row elements at a time (one element
+// per lane, FastErf interpolation in FP64) and broadcasts them with
+// readlane while it walks the row, so projection data are read or written
+// once per TOF bin and the image once per matrix element, as without TOF.
+//
+// Projection data: TOF bin outermost, each TOF block in the non-TOF device
+// layout. Arithmetic follows the oracle's restatement in float order
+// (compiled with -ffp-contract=off), so forward projections match it bit for
+// bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdint>
+
+#include "pmrt_device.hpp"
+#include "pmrt_kernels.hpp"
+
+namespace mi {
+
+namespace {
+
+using namespace dev;
+
+// d = -<voxel - mid, unit> with the voxel's scanner coordinates
+// (ProjMatrixByBin.inl:159-167)
+__device__ inline float tof_distance(const FamilyLor& L, int z, int y, int x, const TofConsts& T) {
+  const float pz = (T.vz * static_cast<float>(z) + T.oz) - T.z_off;
+  const float py = (T.vy * static_cast<float>(y) + T.oy) - 0.F;
+  const float px = (T.vx * static_cast<float>(x) + T.ox) - 0.F;
+  float ip = 0.F;
+  ip = ip + (pz - L.mz) * L.uz;
+  ip = ip + (py - L.my) * L.uy;
+  ip = ip + (px - L.mx) * L.ux;
+  return -ip;
+}
+
+// FastErf::get_erf_linear_interpolation (FastErf.inl:46-61)
+__device__ inline double erf_interp(const double* __restrict__ tab, double xp) {
+  xp = fmax(fmin(kErfMax, xp), -kErfMax);
+  const double xi = (xp + kErfMax) / ((2 * kErfMax) / kErfSamples);
+  const int lo = static_cast<int>(floor(xi));
+  return tab[lo] + (xi - lo) * (tab[lo + 1] - tab[lo]);
+}
+
+// value * get_tof_value(low - d, high - d) (ProjMatrixByBin.inl:163-172,186-195);
+// 0 where the reference drops the element
+__device__ inline float tof_weight(float w, float d, int k, const TofConsts& T) {
+  const float d1n = (T.low[k] - d) * T.r;
+  const float d2n = (T.high[k] - d) * T.r;
+  if ((d1n >= 4.F && d2n >= 4.F) || (d1n <= -4.F && d2n <= -4.F)) return 0.F;
+  const float v = static_cast<float>(0.5 * (erf_interp(T.erf, d2n) - erf_interp(T.erf, d1n)));
+  return (v > 0.F && w != 0.F) ? w * v : 0.F;
+}
+
+// TOF weights of one element for all TOF bins. With shared bin boundaries
+// (high[k] == low[k+1], as set_tof_mash_factor produces them) the erf of each
+// boundary is evaluated once, and only for bins that pass the zero test;
+// the values are those of tof_weight bit for bit.
+template <int NT>
+__device__ inline void tof_weights(float w, float d, const TofConsts& T, float (&tw)[NT]) {
+  if (!T.shared_bounds) {
+#pragma unroll
+    for (int k = 0; k < NT; ++k) tw[k] = k < T.ntof ? tof_weight(w, d, k, T) : 0.F;
+    return;
+  }
+  float xb[NT + 1];
+  xb[0] = (T.low[0] - d) * T.r;
+#pragma unroll
+  for (int j = 1; j <= NT; ++j) xb[j] = j <= T.ntof ? (T.high[j - 1] - d) * T.r : 0.F;
+  bool live[NT];
+  bool need[NT + 1];
+#pragma unroll
+  for (int j = 0; j <= NT; ++j) need[j] = false;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    live[k] = k < T.ntof && !((xb[k] >= 4.F && xb[k + 1] >= 4.F) || (xb[k] <= -4.F && xb[k + 1] <= -4.F));
+    if (live[k]) {
+      need[k] = true;
+      need[k + 1] = true;
+    }
+  }
+  double e[NT + 1];
+#pragma unroll
+  for (int j = 0; j <= NT; ++j) e[j] = need[j] ? erf_interp(T.erf, xb[j]) : 0.0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    const float v = static_cast<float>(0.5 * (e[k + 1] - e[k]));
+    tw[k] = (live[k] && v > 0.F && w != 0.F) ? w * v : 0.F;
+  }
+}
+
+__device__ inline float bcast(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+constexpr int kTofUnroll = 2;
+
+// One family per wave; lanes over axial positions (NSLOT per lane), NT >= ntof.
+template <int MODE, int NT, int NSLOT>
+__device__ void fp_tof_task(const Family& f, const FamilyLor& lor, const int2* __restrict__ pool,
+                            const float* __restrict__ img, float* __restrict__ out, const ImageDims& d,
+                            const TofConsts& T, const FpData& data, int t, int lane) {
+  int zb[NSLOT];
+  bool valid[NSLOT];
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) {
+    const int a = lane + kWave * q;
+    valid[q] = a < f.nax;
+    zb[q] = f.zadd0 + f.ppa * a - d.min_z;
+  }
+  float acc[NSLOT][NT];
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q)
+#pragma unroll
+    for (int k = 0; k < NT; ++k) acc[q][k] = 0.F;
+  const int2* row = pool + f.row_start;
+  const int n = f.row_len;
+  const int zs = f.zsign;
+  for (int c0 = 0; c0 < n; c0 += kWave) {
+    // TOF weights of element c0 + lane
+    float tw[NT];
+    {
+      const int e = c0 + lane;
+      float dist = 0.F, w = 0.F;
+      if (e < n) {
+        const int2 el = row[e];
+        int x, y, z;
+        decode(el.x, x, y, z);
+        dist = tof_distance(lor, z, y, x, T);
+        w = __int_as_float(el.y);
+      }
+      tof_weights<NT>(w, dist, T, tw);
+    }
+    const int m = min(kWave, n - c0);
+    int j = 0;
+    for (; j + kTofUnroll <= m; j += kTofUnroll) {
+      float v[kTofUnroll][NSLOT];
+      bool ok[kTofUnroll][NSLOT];
+#pragma unroll
+      for (int u = 0; u < kTofUnroll; ++u) {
+        int x, y, z;
+        decode(row[c0 + j + u].x, x, y, z);
+        const float* colp = img + static_cast<int64_t>((y - d.min_y) * d.nx + (x - d.min_x)) * d.nzp;
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) {
+          const int zz = zs * z + zb[q];
+          ok[u][q] = valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
+          v[u][q] = load_u32off(colp, static_cast<unsigned>(min(max(zz, 0), d.nz - 1)));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kTofUnroll; ++u)
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+          // a predicate, not a break: the loop must stay fully unrolled so that
+          // acc and tw remain in registers
+          if (k < T.ntof) {
+            const float wk = bcast(tw[k], j + u);
+#pragma unroll
+            for (int q = 0; q < NSLOT; ++q)
+              if (ok[u][q]) acc[q][k] = acc[q][k] + v[u][q] * wk;
+          }
+        }
+    }
+    for (; j < m; ++j) {
+      int x, y, z;
+      decode(row[c0 + j].x, x, y, z);
+      const float* colp = img + static_cast<int64_t>((y - d.min_y) * d.nx + (x - d.min_x)) * d.nzp;
+      float v[NSLOT];
+      bool ok[NSLOT];
+#pragma unroll
+      for (int q = 0; q < NSLOT; ++q) {
+        const int zz = zs * z + zb[q];
+        ok[q] = valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
+        v[q] = load_u32off(colp, static_cast<unsigned>(min(max(zz, 0), d.nz - 1)));
+      }
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
+        if (k < T.ntof) {
+          const float wk = bcast(tw[k], j);
+#pragma unroll
+          for (int q = 0; q < NSLOT; ++q)
+            if (ok[q]) acc[q][k] = acc[q][k] + v[q] * wk;
+        }
+      }
+    }
+  }
+  double ll = 0.0;
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) {
+    if (!valid[q]) continue;
+    const int64_t i0 = static_cast<int64_t>(f.sino_base) + lane + kWave * q;
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+      if (k < T.ntof)
+        fp_epilogue<MODE>(acc[q][k], static_cast<int64_t>(k) * T.block + i0, k * T.nvg + f.viewgram, data, out, ll);
+  }
+  if constexpr (fp_accumulates_loglik(MODE)) {
+    for (int off = kWave / 2; off > 0; off >>= 1) ll += __shfl_xor(ll, off, kWave);
+    if (lane == 0) data.partial[t] = ll;
+  }
+}
+
+template <int MODE, int NT>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_tof_kernel(const FpTask* __restrict__ tasks, int ntasks,
+                                                                       const Family* __restrict__ fams,
+                                                                       const FamilyLor* __restrict__ lors,
+                                                                       const int2* __restrict__ pool,
+                                                                       const float* __restrict__ img,
+                                                                       float* __restrict__ out, ImageDims d, TofConsts T,
+                                                                       FpData data) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int t = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (t >= ntasks) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  const int fi = tasks[t].first;
+  const Family f = fams[fi];
+  const FamilyLor lor = lors[fi];
+  // TOF families hold at most 2 x 64 axial positions (checked at set-up)
+  if (f.nax <= kWave) fp_tof_task<MODE, NT, 1>(f, lor, pool, img, out, d, T, data, t, lane);
+  else fp_tof_task<MODE, NT, 2>(f, lor, pool, img, out, d, T, data, t, lane);
+}
+
+// Voxel-driven TOF back projection: one wave per image column, as bp_kernel;
+// for each table entry only the TOF bins with a non-zero weight are read.
+template <int MODE, int PPA, int NSLOT, int NT>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_kernel(const int64_t* __restrict__ list_off,
+                                                                       const BpEntry* __restrict__ ent,
+                                                                       const float* __restrict__ ent_d, int sb, int se,
+                                                                       const float* __restrict__ proj,
+                                                                       float* __restrict__ img, ImageDims d, int nwork,
+                                                                       int2 tile, int remap, TofConsts T, BpData data) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int w = (remap ? xcd_remap(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kWavesPerBlock + wave;
+  if (w >= nwork) return;
+  const int col = tiled_column(w, d, tile);
+  if (col < 0) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  float acc[NSLOT];
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
+  for (int s = sb; s < se; ++s) {
+    const int64_t beg = list_off[static_cast<int64_t>(s) * d.ncols + col];
+    const int64_t end = list_off[static_cast<int64_t>(s) * d.ncols + col + 1];
+    for (int64_t c0 = beg; c0 < end; c0 += kWave) {
+      // TOF weights and the range of TOF bins with a non-zero weight, of entry c0 + lane
+      float tw[NT];
+      int klo = NT, khi = -1;
+      {
+        const int64_t e = c0 + lane;
+        float wv = 0.F, dist = 0.F;
+        if (e < end) {
+          wv = ent[e].w;
+          dist = ent_d[e];
+        }
+        tof_weights<NT>(wv, dist, T, tw);
+#pragma unroll
+        for (int k = 0; k < NT; ++k)
+          if (tw[k] != 0.F) {
+            klo = min(klo, k);
+            khi = k;
+          }
+      }
+      const int m = static_cast<int>(min(static_cast<int64_t>(kWave), end - c0));
+      for (int j = 0; j < m; ++j) {
+        const int kl = __builtin_amdgcn_readlane(klo, j), kh = __builtin_amdgcn_readlane(khi, j);
+        if (kh < 0) continue;
+        const BpEntry E = ent[c0 + j];
+        unsigned a[NSLOT];
+        bool ok[NSLOT];
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) a[q] = entry_axial<PPA>(E.k, E.nax_ppa, d.min_z + lane + kWave * q, ok[q]);
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+          if (k < kl || k > kh) continue;
+          const float wk = bcast(tw[k], j);
+          const float* run = proj + static_cast<int64_t>(k) * T.block + E.sino_base;
+#pragma unroll
+          for (int q = 0; q < NSLOT; ++q) {
+            const float p = load_u32off(run, a[q]);
+            if (ok[q]) acc[q] = acc[q] + wk * p;
+          }
+        }
+      }
+    }
+  }
+  float* outc = img + static_cast<int64_t>(col) * d.nzp;
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) {
+    const int z = lane + kWave * q;
+    if (z >= d.nzp) continue;
+    if constexpr (MODE == BP_PLAIN) {
+      outc[z] = z < d.nz ? acc[q] : 0.F;
+    } else {
+      if (z < d.nz) {
+        // divide(update, sens, 0) (numerics/divide.inl:26-47), threshold, multiply
+        const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
+        float u = acc[q];
+        const float sv = data.sens[idx];
+        if (fabsf(sv) <= 0.F && fabsf(u) <= 0.F) u = 0;
+        else u = u / sv;
+        if (data.threshold) {
+          if (u > FLT_MAX) u = FLT_MAX;
+          else if (0.F > u) u = 0.F;
+        }
+        data.estimate[idx] = data.estimate[idx] * u;
+        if (data.update) data.update[idx] = u;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) entry_tof_distance_kernel(const Family* __restrict__ fams, int fam_begin,
+                                                                int fam_end, const int64_t* __restrict__ fam_entry_off,
+                                                                const FamilyLor* __restrict__ lors,
+                                                                const int2* __restrict__ pool, TofConsts T,
+                                                                float* __restrict__ out) {
+  const int fi = fam_begin + blockIdx.x;
+  if (fi >= fam_end) return;
+  const Family f = fams[fi];
+  const FamilyLor L = lors[fi];
+  const int64_t base = fam_entry_off[fi - fam_begin];
+  for (int e = threadIdx.x; e < f.row_len; e += blockDim.x) {
+    int x, y, z;
+    decode(pool[f.row_start + e].x, x, y, z);
+    out[base + e] = tof_distance(L, z, y, x, T);
+  }
+}
+
+__global__ void gather_floats_kernel(const float* __restrict__ src, const uint32_t* __restrict__ idx,
+                                     float* __restrict__ dst, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[idx[i]];
+}
+
+}  // namespace
+
+void launch_forward_project_tof(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams,
+                                const FamilyLor* d_lors, const int2* d_pool, const float* d_img, float* d_out,
+                                const ImageDims& dims, const TofConsts& tof, const FpData& data, hipStream_t stream) {
+  if (ntasks <= 0) return;
+  const dim3 grid((ntasks + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+#define MI_FPT(M, NT)                                                                                              \
+  hipLaunchKernelGGL((fp_tof_kernel<M, NT>), grid, block, 0, stream, d_tasks, ntasks, d_fams, d_lors, d_pool, d_img, \
+                     d_out, dims, tof, data)
+#define MI_FPT_CASE(M)                      \
+  case M:                                   \
+    if (tof.ntof <= 12) MI_FPT(M, 12);      \
+    else MI_FPT(M, 16);                     \
+    break;
+  switch (mode) {
+    MI_FPT_CASE(FP_PLAIN)
+    MI_FPT_CASE(FP_RATIO)
+    MI_FPT_CASE(FP_GRADIENT)
+    MI_FPT_CASE(FP_LOGLIK)
+    MI_FPT_CASE(FP_GRADIENT_LOGLIK)
+    MI_FPT_CASE(FP_RATIO_LOGLIK)
+    MI_FPT_CASE(FP_DRATIO)
+    MI_FPT_CASE(FP_SCALE)
+  }
+#undef MI_FPT_CASE
+#undef MI_FPT
+}
+
+void launch_back_project_tof(BpMode mode, int ppa, const int64_t* d_list_off, const BpEntry* d_entries,
+                             const float* d_entry_d, int subset_begin, int subset_end, const float* d_proj,
+                             float* d_img, const ImageDims& dims, const TofConsts& tof, const BpData& data,
+                             const BpTuning& tune, hipStream_t stream) {
+  const int nslot = (dims.nzp + kWave - 1) / kWave;
+  const int2 tile = make_int2(tune.tile_x > 0 ? tune.tile_x : 1, tune.tile_y > 0 ? tune.tile_y : 1);
+  const int nwork = ((dims.nx + tile.x - 1) / tile.x) * ((dims.ny + tile.y - 1) / tile.y) * tile.x * tile.y;
+  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+#define MI_BPT(M, P, S, NT)                                                                                         \
+  hipLaunchKernelGGL((bp_tof_kernel<M, P, S, NT>), grid, block, 0, stream, d_list_off, d_entries, d_entry_d,       \
+                     subset_begin, subset_end, d_proj, d_img, dims, nwork, tile, tune.xcd_remap, tof, data)
+#define MI_BPT_NT(M, P, S)                 \
+  if (tof.ntof <= 12) MI_BPT(M, P, S, 12); \
+  else MI_BPT(M, P, S, 16);
+#define MI_BPT_SLOTS(M, P)            \
+  switch (nslot) {                    \
+    case 1: MI_BPT_NT(M, P, 1) break; \
+    case 2: MI_BPT_NT(M, P, 2) break; \
+    case 3: MI_BPT_NT(M, P, 3) break; \
+    default: MI_BPT_NT(M, P, 4) break; \
+  }
+  if (mode == BP_PLAIN) {
+    if (ppa == 1) { MI_BPT_SLOTS(BP_PLAIN, 1) }
+    else if (ppa == 2) { MI_BPT_SLOTS(BP_PLAIN, 2) }
+    else { MI_BPT_SLOTS(BP_PLAIN, 0) }
+  } else {
+    if (ppa == 1) { MI_BPT_SLOTS(BP_OSEM_UPDATE, 1) }
+    else if (ppa == 2) { MI_BPT_SLOTS(BP_OSEM_UPDATE, 2) }
+    else { MI_BPT_SLOTS(BP_OSEM_UPDATE, 0) }
+  }
+#undef MI_BPT_SLOTS
+#undef MI_BPT_NT
+#undef MI_BPT
+}
+
+void launch_entry_tof_distance(const Family* d_fams, int fam_begin, int fam_end, const int64_t* d_fam_entry_off,
+                               const FamilyLor* d_lors, const int2* d_pool, const ImageDims& dims,
+                               const TofConsts& tof, float* d_entry_d, hipStream_t stream) {
+  (void)dims;
+  if (fam_end <= fam_begin) return;
+  hipLaunchKernelGGL(entry_tof_distance_kernel, dim3(fam_end - fam_begin), dim3(64), 0, stream, d_fams, fam_begin,
+                     fam_end, d_fam_entry_off, d_lors, d_pool, tof, d_entry_d);
+}
+
+void launch_gather_floats(const float* d_src, const uint32_t* d_idx, float* d_dst, int64_t n, hipStream_t stream) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(gather_floats_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, d_src, d_idx,
+                     d_dst, n);
+}
+
+}  // namespace mi

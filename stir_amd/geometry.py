@@ -509,6 +509,30 @@ def config0_tof(num_timing_poss: int = 9) -> Tuple[ProjDataInfoCylindrical, Voxe
     return pdi, img
 
 
+def config4_problems(seed: int = 4, count: int = 24):
+    """Uneven problems for batched reconstruction: ring count uniform in
+    {8,16,24,32}, crystals per ring in {192,288,384}, image in-plane size in
+    {96,128,192,256} at 2 mm, planes = 2*rings-1 (ring spacing 4 mm = 2 z
+    voxels), span 1 up to 16 rings and span 3 above, maximum ring difference
+    rings-1, views = tangential positions = crystals/2, radius 350 mm;
+    180-degree symmetry only for 288 crystals (144 views do not balance 8
+    subsets with the 90-degree symmetry). Returns a list of
+    (ProjDataInfoCylindrical, VoxelsOnCartesianGrid, RayTracingMatrixParameters)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        rings = int(rng.choice([8, 16, 24, 32]))
+        crystals = int(rng.choice([192, 288, 384]))
+        xy = int(rng.choice([96, 128, 192, 256]))
+        sc = Scanner.cylindrical(rings, crystals, 350.0, 4.0, name=f"cyl{rings}x{crystals}")
+        pdi = ProjDataInfoCylindrical.construct(sc, span=1 if rings <= 16 else 3, max_delta=rings - 1,
+                                                num_views=crystals // 2, num_tangential_poss=crystals // 2)
+        img = VoxelsOnCartesianGrid(2 * rings - 1, xy, xy, 2.0, 2.0, 2.0)
+        pm = RayTracingMatrixParameters(do_symmetry_90degrees_min_phi=crystals != 288)
+        out.append((pdi, img, pm))
+    return out
+
+
 def rpt_sensitivity_case() -> Tuple[ProjDataInfoCylindrical, VoxelsOnCartesianGrid, RayTracingMatrixParameters]:
     """Geometry of the reference's RPTsens_seg3_PM fixture
     (recon_test_pack/OSMAPOSL_test_PM_for_sensitivity.par on Utahscat600k_ca_seg4.hs):

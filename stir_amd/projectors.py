@@ -230,8 +230,12 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
         self._ctx = projector.context
 
     def set_up(self, measured: np.ndarray, additive=0.0, zero_end_planes: bool = False,
-               compute_sensitivities: bool = True) -> None:
+               compute_sensitivities: bool = True, use_tof_sensitivities: bool = False) -> None:
+        """use_tof_sensitivities: "use time-of-flight sensitivities" (default off:
+        TOF data get non-TOF sensitivities, ``...AndProjData.cxx:533-536``)."""
         L = _native.load()
+        _native.check(L.mi_obj_set_use_tof_sensitivities(self._ctx.handle, int(use_tof_sensitivities)),
+                      "mi_obj_set_use_tof_sensitivities")
         y = np.ascontiguousarray(measured, dtype=np.float32).reshape(-1)
         if isinstance(additive, np.ndarray):
             ap, keep = _native.float_ptr_or_none(additive)

@@ -163,3 +163,26 @@ def test_tof_planted_sign_error_fails_closed_form(tof, nontof):
             f = 0.5 * (math.erf((hi[k] - d) / (sigma * math.sqrt(2))) - math.erf((lo[k] - d) / (sigma * math.sqrt(2))))
             mismatch += abs(float(wt) / w[(zz, yy, xx)] - f) > 1e-3
     assert mismatch > 10
+
+
+def test_tof_sensitivity_defaults_to_nontof(tof, nontof):
+    """With TOF data and "use time-of-flight sensitivities" off (the default)
+    the sensitivity is the non-TOF back projection of ones
+    (...AndProjData.cxx:533-536,648-670); with it on, the TOF back projection
+    of TOF ones, which differs only where the TOF kernels are truncated."""
+    pdi, img, O = tof
+    pdi0, img0 = config0()
+    O0 = OraclePlan(pdi0, img0, RayTracingMatrixParameters())
+    s_def = O.subset_sensitivity(1, 4)
+    s_ref = O0.subset_sensitivity(1, 4)
+    # rows traced directly vs obtained through symmetries: equal to float rounding
+    assert np.abs(s_def - s_ref).max() <= 1e-4 * s_ref.max()
+    # TOF sensitivities use the TOF projector's subsets, so compare all subsets together
+    O.set_use_tof_sensitivities(True)
+    try:
+        s_tof = O.subset_sensitivity(-1, 1)
+    finally:
+        O.set_use_tof_sensitivities(False)
+    # against the same directly traced rows (rows through symmetries differ at the FOV edge)
+    s_all = nontof.subset_sensitivity(-1, 1)
+    assert np.abs(s_tof - s_all).max() <= 1e-5 * s_all.max()
