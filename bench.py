@@ -23,6 +23,23 @@ METRIC = "mMR-size fwd+back proj pairs/s, OSEM iter/s, autograd grad/s; HBM GB/s
 HBM_PEAK_GBS = 8000.0
 
 
+TRAFFIC_FILE = "profiles/pair_mMR_pmc_traffic.txt"
+
+
+def measured_traffic(kernel):
+    """HBM-side bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            for line in f:
+                parts = line.split()
+                if parts[:3] == ["#", "traffic_bytes_per_launch", kernel]:
+                    return int(parts[3])
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -37,6 +54,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-autograd", action="store_true")
     ap.add_argument("--skip-tof", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--tof-steps", type=int, default=3)
     ap.add_argument("--autograd-steps", type=int, default=3)
     ap.add_argument("--cpu-subsets", type=int, default=1, help="subsets of 21 in the CPU baseline sample")
@@ -123,6 +141,7 @@ def main():
     dom_ms = bp_avg if dominant == "bp" else fp_avg
     dom_bytes = b_bp if dominant == "bp" else b_fp
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic("bp_kernel" if dominant == "bp" else "fp_kernel")
 
     result = {
         "metric": METRIC,
@@ -153,7 +172,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": TRAFFIC_FILE if traffic is not None else None,
         },
         "kernels": {
             "fp_kernel_ms": round(fp_avg, 4),
@@ -201,6 +221,9 @@ def main():
 
     if not args.skip_tof:
         result["tof"] = tof_bench(dev, x, args.tof_steps, nsub, args.radius)
+
+    if not args.skip_batch:
+        result["batch"] = batch_bench(dev)
 
     if not args.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(pdi, img, pm, x_true, nsub, args.cpu_subsets)
@@ -305,6 +328,69 @@ def tof_bench(dev, x, steps, nsub, radius):
     log(f"TOF OSEM iteration {it_s * 1000:.2f} ms")
     pair.close()
     return out
+
+
+def batch_bench(dev):
+    """24 uneven OSEM problems (8 subsets x 2 iterations each): batched (tuned
+    launch shape, HIP graph) against the same problems run one after the
+    other through the per-problem path."""
+    import numpy as np
+    import torch
+
+    from stir_amd import phantoms
+    from stir_amd.batch import BatchedOSMAPOSL
+    from stir_amd.geometry import config4_problems
+    from stir_amd.projectors import ProjectorByBinPairMI355X, ProjMatrixByBinUsingRayTracing
+
+    problems = config4_problems()
+    nsub, nsubiter = 8, 16
+    t0 = time.time()
+    B = BatchedOSMAPOSL(problems, nsub)
+    setup_s = time.time() - t0
+    nbins = 0
+    for i, (pdi, img, pm) in enumerate(problems):
+        P = ProjectorByBinPairMI355X(ProjMatrixByBinUsingRayTracing(pm))
+        P.set_up(pdi, img, num_subsets=1)
+        _, y, bkg = phantoms.config0_data(pdi, img, P.forward_project, seed=40 + i)
+        P.close()
+        B.set_data(i, y, bkg)
+        nbins += pdi.num_bins
+    log(f"batch: {len(problems)} problems, {nbins} bins, set-up {setup_s:.2f}s")
+
+    def reset():
+        for i, (pdi, img, pm) in enumerate(problems):
+            B.set_estimate(i, np.ones(img.shape, np.float32))
+        torch.cuda.synchronize()
+
+    # sequential per-problem path (warm-up run first)
+    reset()
+    B.run_sequential(nsubiter)
+    reset()
+    t0 = time.perf_counter()
+    B.run_sequential(nsubiter)
+    seq_s = time.perf_counter() - t0
+    seq_imgs = [B.get_estimate(i) for i in range(len(B))]
+    choice = B.tune()
+    reset()
+    B.run(nsubiter, use_graph=True)  # capture + warm-up
+    B.synchronize()
+    reset()
+    t0 = time.perf_counter()
+    B.run(nsubiter, use_graph=True)
+    B.synchronize()
+    bat_s = time.perf_counter() - t0
+    worst = 0.0
+    for i in range(len(B)):
+        a, b = seq_imgs[i], B.get_estimate(i)
+        m = a > 1e-3 * a.max()
+        worst = max(worst, float((np.abs(a - b)[m] / a[m]).max()))
+    B.close()
+    log(f"batch: sequential {seq_s * 1000:.1f} ms, batched {bat_s * 1000:.1f} ms ({seq_s / bat_s:.2f}x), "
+        f"max rel image difference {worst:.2e}, tuner choice {choice}")
+    return {"problems": len(problems), "subsets": nsub, "subiterations": nsubiter, "total_bins": int(nbins),
+            "sequential_ms": round(seq_s * 1000, 3), "batched_ms": round(bat_s * 1000, 3),
+            "speedup": round(seq_s / bat_s, 3), "reconstructions_per_s": round(len(problems) / bat_s, 3),
+            "max_rel_image_diff": worst, "tuner_choice": choice}
 
 
 def cpu_baseline(pdi, img, pm, x_true, nsub, nsample):

@@ -246,6 +246,39 @@ int mi_osem_subset_step_vjp_dev(mi_ctx* ctx, const float* d_x_in, const float* d
 /* Waits for all work of the context. */
 int mi_synchronize(mi_ctx* ctx);
 
+/* ---- batches of independent reconstructions ------------------------------- */
+/* No counterpart in the reference (it reconstructs one problem at a time).
+ * A batch holds n independent non-TOF problems, each a full context with its
+ * own geometry (mi_batch_context gives it, e.g. for mi_osem_run on one
+ * problem), and runs an OSMAPOSL subiteration of all of them with one launch
+ * per lane-group size: forward projection by families, back projection by
+ * image columns, packed several to a 64-lane wave when they are short
+ * (at most 64 axial positions per segment and 64 planes). Results equal the
+ * per-problem path's. All problems use num_subsets subsets. */
+typedef struct mi_batch mi_batch;
+int mi_batch_create(int device, int n, const mi_geometry* geometries, int num_subsets, mi_batch** out);
+void mi_batch_destroy(mi_batch* batch);
+int mi_batch_context(mi_batch* batch, int problem, mi_ctx** ctx);
+/* mi_obj_set_data + mi_obj_compute_sensitivities of one problem. */
+int mi_batch_set_data(mi_batch* batch, int problem, const float* measured, const float* additive,
+                      float additive_constant, int zero_end_planes);
+int mi_batch_set_estimate(mi_batch* batch, int problem, const float* image, int enforce_positivity);
+int mi_batch_get_estimate(mi_batch* batch, int problem, float* image);
+/* Launch shape: forward tasks of at most fp_group_max bins run in lane
+ * groups (the others one per wave), fp_widen 1 doubles those groups, bp_pack 0
+ * gives every column a full wave, tile_x x tile_y orders the columns.
+ * Defaults 32, 0, 1, 32, 8. */
+int mi_batch_set_launch(mi_batch* batch, int fp_group_max, int fp_widen, int bp_pack, int tile_x, int tile_y);
+/* Times the candidate launch shapes on subset 0 (no side effects on the
+ * estimates) and keeps the fastest; *choice receives its index. */
+int mi_batch_tune(mi_batch* batch, int* choice);
+/* num_subiterations subiterations of every problem, as mi_osem_run;
+ * use_graph: the launches are captured once into a HIP graph (per
+ * argument triple) and replayed. Returns without waiting. */
+int mi_batch_osem_run(mi_batch* batch, int start_subiteration_num, int num_subiterations, int start_subset,
+                      int use_graph);
+int mi_batch_synchronize(mi_batch* batch);
+
 /* ---- launch tuning ---------------------------------------------------------- */
 /* No counterpart in the reference. Launch shape of the back projection: image
  * columns are visited in tiles of MI_TUNE_BP_TILE_X x MI_TUNE_BP_TILE_Y

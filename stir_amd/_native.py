@@ -26,6 +26,8 @@ EXPORTED_SYMBOLS = (
     "mi_obj_log_likelihood", "mi_synchronize", "mi_set_kernel_timing", "mi_get_kernel_timing",
     "mi_symmetry_operations", "mi_set_tuning", "mi_obj_value_and_gradient_dev", "mi_quadratic_prior_dev",
     "mi_osem_subset_step_dev", "mi_osem_subset_step_vjp_dev", "mi_obj_set_use_tof_sensitivities",
+    "mi_batch_create", "mi_batch_destroy", "mi_batch_context", "mi_batch_set_data", "mi_batch_set_estimate",
+    "mi_batch_get_estimate", "mi_batch_set_launch", "mi_batch_tune", "mi_batch_osem_run", "mi_batch_synchronize",
 )
 
 MI_TUNE_BP_TILE_X, MI_TUNE_BP_TILE_Y, MI_TUNE_BP_XCD_REMAP, MI_TUNE_BP_SLOT_SKIP = 1, 2, 3, 4
@@ -102,9 +104,20 @@ def load():
     L.mi_set_kernel_timing.argtypes = [vp, c_int]
     L.mi_get_kernel_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+    L.mi_batch_create.argtypes = [c_int, c_int, ctypes.POINTER(GeometryStruct), c_int, ctypes.POINTER(vp)]
+    L.mi_batch_destroy.argtypes = [vp]
+    L.mi_batch_destroy.restype = None
+    L.mi_batch_context.argtypes = [vp, c_int, ctypes.POINTER(vp)]
+    L.mi_batch_set_data.argtypes = [vp, c_int, _f32p, vp, ctypes.c_float, c_int]
+    L.mi_batch_set_estimate.argtypes = [vp, c_int, _f32p, c_int]
+    L.mi_batch_get_estimate.argtypes = [vp, c_int, _f32p]
+    L.mi_batch_set_launch.argtypes = [vp, c_int, c_int, c_int, c_int, c_int]
+    L.mi_batch_tune.argtypes = [vp, ctypes.POINTER(c_int)]
+    L.mi_batch_osem_run.argtypes = [vp, c_int, c_int, c_int, c_int]
+    L.mi_batch_synchronize.argtypes = [vp]
     for name in EXPORTED_SYMBOLS:
         fn = getattr(L, name)
-        if name not in ("mi_last_error", "mi_ctx_destroy"):
+        if name not in ("mi_last_error", "mi_ctx_destroy", "mi_batch_destroy"):
             fn.restype = c_int
     _lib = L
     return L

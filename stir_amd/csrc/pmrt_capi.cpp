@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -906,7 +907,7 @@ int mi_obj_set_estimate(mi_ctx* ctx, const float* image, int enforce_positivity)
         std::fill(h.begin(), h.end(), small_number);
       }
     }
-    ctx->x.alloc(ctx->img.n);
+    if (ctx->x.n != ctx->img.n) ctx->x.alloc(ctx->img.n);  // keep the address: batches hold it
     HIPCHECK(hipMemsetAsync(ctx->x.p, 0, ctx->x.bytes(), ctx->stream));
     HIPCHECK(hipMemcpyAsync(ctx->stage_img.p, h.data(), nv * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     ctx->image_to_device(ctx->stage_img.p, ctx->x.p, ctx->stream);
@@ -1169,6 +1170,355 @@ int mi_synchronize(mi_ctx* ctx) {
     if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     HIPCHECK(hipSetDevice(ctx->device));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches of independent problems (grouped kernels, one launch per group size)
+struct mi_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nsub = 0;
+  std::vector<mi_ctx*> probs;  // owned
+  // launch choices (mi_batch_tune)
+  int fp_widen = 0;             // forward groups: 0 = smallest group size holding the family, 1 = twice that
+  int fp_group_max = 32;        // forward tasks with at most this many bins go to lane groups, the rest to full waves
+  int bp_pack = 1;              // back projection: columns of at most 32 planes in lane groups (1) or full waves (0)
+  int tile_x = 32, tile_y = 8;  // back-projection column order
+  // device descriptors and work lists
+  DBuf<mi::ProblemDesc> descs;
+  std::vector<std::unique_ptr<DBuf<const float*>>> sens_lists;
+  std::vector<std::array<int, 4>> fp_n;                       // [subset][group-size index]
+  std::vector<std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 4>> fp_work;
+  std::array<int, 4> bp_n{};
+  std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 4> bp_work;
+  // full-wave items: forward (problem, task) per subset, back projection (problem, column)
+  DBuf<const mi::FpTask*> task_lists;
+  std::vector<int> fpw_n;
+  std::vector<std::unique_ptr<DBuf<mi::WorkItem>>> fpw_work;
+  int bpw_n = 0;
+  std::unique_ptr<DBuf<mi::WorkItem>> bpw_work;
+  bool lists_ready = false;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_key[3] = {0, 0, 0};  // start subiteration, count, start subset of the captured graph
+  ~mi_batch() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    for (mi_ctx* c : probs) delete c;
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+constexpr int kGroupSizes[4] = {8, 16, 32, 64};
+int group_index_at_least(int n) {
+  for (int i = 0; i < 4; ++i)
+    if (n <= kGroupSizes[i]) return i;
+  return -1;
+}
+
+void batch_invalidate(mi_batch* b) {
+  b->lists_ready = false;
+  if (b->graph_exec) {
+    (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+  }
+}
+
+// descriptors (device pointers of every problem) and work lists
+void batch_prepare(mi_batch* b) {
+  if (b->lists_ready) return;
+  const int np = static_cast<int>(b->probs.size());
+  std::vector<mi::ProblemDesc> h(np);
+  b->sens_lists.clear();
+  for (int p = 0; p < np; ++p) {
+    mi_ctx* c = b->probs[p];
+    if (!c->have_data || !c->have_sens || !c->have_x)
+      throw ApiError(MI_ERR_STATE, "data, sensitivities and estimate must be set for every problem");
+    auto sl = std::make_unique<DBuf<const float*>>();
+    sl->alloc(b->nsub);
+    std::vector<const float*> hs(b->nsub);
+    for (int s = 0; s < b->nsub; ++s) hs[s] = c->sens[s]->p;
+    HIPCHECK(hipMemcpy(sl->p, hs.data(), hs.size() * sizeof(const float*), hipMemcpyHostToDevice));
+    mi::ProblemDesc& d = h[p];
+    d.fams = c->fams.p;
+    d.pool = c->pool.p;
+    d.fp_in = c->x.p;
+    d.fp_out = c->ratio.p;
+    d.fd = c->fp_data();
+    d.list_off = c->list_off.p;
+    d.entries = c->entries.p;
+    d.bp_in = c->ratio.p;
+    d.bp_out = c->img2.p;
+    d.sens = sl->p;
+    d.estimate = c->x.p;
+    d.d = c->dims;
+    d.ppa = c->uniform_ppa;
+    b->sens_lists.push_back(std::move(sl));
+  }
+  b->descs.alloc(np);
+  HIPCHECK(hipMemcpy(b->descs.p, h.data(), np * sizeof(mi::ProblemDesc), hipMemcpyHostToDevice));
+  // task lists of every problem (full-wave forward items index them)
+  {
+    std::vector<const mi::FpTask*> tl(np);
+    for (int p = 0; p < np; ++p) tl[p] = b->probs[p]->tasks.p;
+    b->task_lists.alloc(np);
+    HIPCHECK(hipMemcpy(b->task_lists.p, tl.data(), np * sizeof(const mi::FpTask*), hipMemcpyHostToDevice));
+  }
+  auto upload = [](const std::vector<mi::WorkItem>& v) {
+    auto d = std::make_unique<DBuf<mi::WorkItem>>();
+    if (!v.empty()) {
+      d->alloc(v.size());
+      HIPCHECK(hipMemcpy(d->p, v.data(), v.size() * sizeof(mi::WorkItem), hipMemcpyHostToDevice));
+    }
+    return d;
+  };
+  // forward work, per subset: tasks (single families or +/- pairs) with more
+  // than fp_group_max bins run a full wave each; the families of the others
+  // go to lane groups of the smallest size that holds them
+  b->fp_n.assign(b->nsub, {0, 0, 0, 0});
+  b->fp_work.clear();
+  b->fp_work.resize(b->nsub);
+  b->fpw_n.assign(b->nsub, 0);
+  b->fpw_work.clear();
+  for (int s = 0; s < b->nsub; ++s) {
+    std::array<std::vector<mi::WorkItem>, 4> lists;
+    std::vector<mi::WorkItem> wide;
+    for (int p = 0; p < np; ++p) {
+      const mi_ctx* c = b->probs[p];
+      std::vector<mi::FpTask> ht(c->task_off[s + 1] - c->task_off[s]);
+      if (!ht.empty())
+        HIPCHECK(hipMemcpy(ht.data(), c->tasks.p + c->task_off[s], ht.size() * sizeof(mi::FpTask), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < ht.size(); ++i) {
+        const mi::FpTask& t = ht[i];
+        const int total = c->h_fams[t.first].nax + (t.second >= 0 ? c->h_fams[t.second].nax : 0);
+        if (total > b->fp_group_max) {
+          wide.push_back({p, c->task_off[s] + static_cast<int>(i)});
+          continue;
+        }
+        for (int f : {t.first, t.second}) {
+          if (f < 0) continue;
+          const int gi = std::min(group_index_at_least(c->h_fams[f].nax) + b->fp_widen, 3);
+          lists[gi].push_back({p, f});
+        }
+      }
+    }
+    for (int gi = 0; gi < 4; ++gi) {
+      b->fp_n[s][gi] = static_cast<int>(lists[gi].size());
+      b->fp_work[s][gi] = upload(lists[gi]);
+    }
+    b->fpw_n[s] = static_cast<int>(wide.size());
+    b->fpw_work.push_back(upload(wide));
+  }
+  // back-projection work: the columns of every problem in tile order; images
+  // of at most 32 planes in lane groups (when packing), the others one column per wave
+  std::array<std::vector<mi::WorkItem>, 4> lists;
+  std::vector<mi::WorkItem> wide;
+  for (int p = 0; p < np; ++p) {
+    const mi::ImageDims& d = b->probs[p]->dims;
+    const int gi = group_index_at_least(d.nzp);
+    const bool grouped = b->bp_pack && gi >= 0 && gi <= 2;
+    const int tx = std::max(b->tile_x, 1), ty = std::max(b->tile_y, 1);
+    for (int y0 = 0; y0 < d.ny; y0 += ty)
+      for (int x0 = 0; x0 < d.nx; x0 += tx)
+        for (int y = y0; y < y0 + ty; ++y)
+          for (int x = x0; x < x0 + tx; ++x)
+            if (y < d.ny && x < d.nx) {
+              const mi::WorkItem w{p, y * d.nx + x};
+              if (grouped) lists[std::max(gi, 1)].push_back(w);
+              else wide.push_back(w);
+            }
+  }
+  for (int gi = 0; gi < 4; ++gi) {
+    b->bp_n[gi] = static_cast<int>(lists[gi].size());
+    b->bp_work[gi] = upload(lists[gi]);
+  }
+  b->bpw_n = static_cast<int>(wide.size());
+  b->bpw_work = upload(wide);
+  b->lists_ready = true;
+}
+
+void batch_fp(mi_batch* b, mi::FpMode mode, int s) {
+  mi::launch_fp_batch(mode, b->descs.p, b->task_lists.p, b->fpw_work[s]->p, b->fpw_n[s], b->stream);
+  for (int gi = 0; gi < 4; ++gi)
+    mi::launch_fp_group(mode, kGroupSizes[gi], b->descs.p, b->fp_work[s][gi]->p, b->fp_n[s][gi], b->stream);
+}
+void batch_bp(mi_batch* b, mi::BpMode mode, int s, int threshold) {
+  mi::launch_bp_batch(mode, b->descs.p, b->bpw_work->p, b->bpw_n, s, threshold, b->stream);
+  for (int gi = 1; gi < 4; ++gi)
+    mi::launch_bp_group(mode, kGroupSizes[gi], b->descs.p, b->bp_work[gi]->p, b->bp_n[gi], s, threshold, b->stream);
+}
+void batch_subiterations(mi_batch* b, int start_subiteration_num, int num_subiterations, int start_subset) {
+  for (int k = start_subiteration_num; k < start_subiteration_num + num_subiterations; ++k) {
+    const int s = (k + start_subset - 1) % b->nsub;  // IterativeReconstruction.cxx:630-639
+    batch_fp(b, mi::FP_RATIO, s);
+    batch_bp(b, mi::BP_OSEM_UPDATE, s, k != 1);
+  }
+}
+mi_ctx* batch_problem(mi_batch* b, int i) {
+  if (!b) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+  if (i < 0 || i >= static_cast<int>(b->probs.size())) throw ApiError(MI_ERR_ARGUMENT, "problem index out of range");
+  return b->probs[i];
+}
+}  // namespace
+
+extern "C" {
+
+int mi_batch_create(int device, int n, const mi_geometry* geometries, int num_subsets, mi_batch** out) {
+  return guarded([&] {
+    if (!out || !geometries || n < 1) throw ApiError(MI_ERR_ARGUMENT, "bad argument");
+    auto b = std::make_unique<mi_batch>();
+    b->device = device;
+    b->nsub = num_subsets;
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipStreamCreate(&b->stream));
+    for (int i = 0; i < n; ++i) {
+      if (geometries[i].num_tof_bins > 1) throw ApiError(MI_ERR_ARGUMENT, "batched problems are non-TOF");
+      mi_ctx* c = nullptr;
+      if (mi_ctx_create(device, &c) != MI_OK) throw ApiError(MI_ERR_DEVICE, g_last_error);
+      b->probs.push_back(c);
+      if (mi_set_up(c, &geometries[i], num_subsets) != MI_OK)
+        throw ApiError(MI_ERR_ARGUMENT, "problem " + std::to_string(i) + ": " + g_last_error);
+    }
+    *out = b.release();
+  });
+}
+
+void mi_batch_destroy(mi_batch* b) { delete b; }
+
+int mi_batch_context(mi_batch* b, int problem, mi_ctx** ctx) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    *ctx = batch_problem(b, problem);
+  });
+}
+
+int mi_batch_set_data(mi_batch* b, int problem, const float* measured, const float* additive, float additive_constant,
+                      int zero_end_planes) {
+  return guarded([&] {
+    mi_ctx* c = batch_problem(b, problem);
+    batch_invalidate(b);
+    if (mi_obj_set_data(c, measured, additive, additive_constant, zero_end_planes) != MI_OK ||
+        mi_obj_compute_sensitivities(c) != MI_OK)
+      throw ApiError(MI_ERR_DEVICE, g_last_error);
+  });
+}
+
+int mi_batch_set_estimate(mi_batch* b, int problem, const float* image, int enforce_positivity) {
+  return guarded([&] {
+    mi_ctx* c = batch_problem(b, problem);
+    const bool had = c->have_x;
+    if (mi_obj_set_estimate(c, image, enforce_positivity) != MI_OK) throw ApiError(MI_ERR_DEVICE, g_last_error);
+    if (!had) batch_invalidate(b);  // the estimate buffer is allocated on first use
+  });
+}
+
+int mi_batch_get_estimate(mi_batch* b, int problem, float* image) {
+  return guarded([&] {
+    mi_ctx* c = batch_problem(b, problem);
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    if (mi_obj_get_estimate(c, image) != MI_OK) throw ApiError(MI_ERR_DEVICE, g_last_error);
+  });
+}
+
+int mi_batch_set_launch(mi_batch* b, int fp_group_max, int fp_widen, int bp_pack, int tile_x, int tile_y) {
+  return guarded([&] {
+    if (!b) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (fp_group_max < 0 || fp_group_max > 64) throw ApiError(MI_ERR_ARGUMENT, "fp_group_max must be in [0, 64]");
+    b->fp_group_max = fp_group_max;
+    b->fp_widen = fp_widen ? 1 : 0;
+    b->bp_pack = bp_pack ? 1 : 0;
+    b->tile_x = std::max(tile_x, 1);
+    b->tile_y = std::max(tile_y, 1);
+    batch_invalidate(b);
+  });
+}
+
+int mi_batch_tune(mi_batch* b, int* choice) {
+  return guarded([&] {
+    if (!b) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    HIPCHECK(hipSetDevice(b->device));
+    struct Cand { int gmax, widen, pack, tx, ty; };
+    const Cand cands[] = {{32, 0, 1, 32, 8}, {16, 0, 1, 32, 8}, {0, 0, 0, 32, 8}, {32, 1, 1, 32, 8},
+                          {32, 0, 0, 32, 8}, {32, 0, 1, 8, 8},  {32, 0, 1, 1, 1},  {32, 0, 1, 64, 4}};
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    float best = 0;
+    int best_i = 0;
+    const int ncand = static_cast<int>(sizeof(cands) / sizeof(cands[0]));
+    for (int i = 0; i < ncand; ++i) {
+      b->fp_group_max = cands[i].gmax;
+      b->fp_widen = cands[i].widen;
+      b->bp_pack = cands[i].pack;
+      b->tile_x = cands[i].tx;
+      b->tile_y = cands[i].ty;
+      batch_invalidate(b);
+      batch_prepare(b);
+      // side-effect free timing: forward projection into the ratio buffers,
+      // plain back projection into scratch images, subset 0
+      batch_fp(b, mi::FP_PLAIN, 0);
+      batch_bp(b, mi::BP_PLAIN, 0, 0);
+      HIPCHECK(hipEventRecord(e0, b->stream));
+      for (int r = 0; r < 3; ++r) {
+        batch_fp(b, mi::FP_PLAIN, 0);
+        batch_bp(b, mi::BP_PLAIN, 0, 0);
+      }
+      HIPCHECK(hipEventRecord(e1, b->stream));
+      HIPCHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (i == 0 || ms < best) { best = ms; best_i = i; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->fp_group_max = cands[best_i].gmax;
+    b->fp_widen = cands[best_i].widen;
+    b->bp_pack = cands[best_i].pack;
+    b->tile_x = cands[best_i].tx;
+    b->tile_y = cands[best_i].ty;
+    batch_invalidate(b);
+    if (choice) *choice = best_i;
+  });
+}
+
+int mi_batch_osem_run(mi_batch* b, int start_subiteration_num, int num_subiterations, int start_subset, int use_graph) {
+  return guarded([&] {
+    if (!b) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (num_subiterations < 0 || start_subiteration_num < 1) throw ApiError(MI_ERR_ARGUMENT, "bad subiteration range");
+    HIPCHECK(hipSetDevice(b->device));
+    // problems' own streams may still be writing data or estimates
+    for (mi_ctx* c : b->probs) HIPCHECK(hipStreamSynchronize(c->stream));
+    batch_prepare(b);
+    if (!use_graph) {
+      batch_subiterations(b, start_subiteration_num, num_subiterations, start_subset);
+    } else {
+      const int key[3] = {start_subiteration_num, num_subiterations, start_subset};
+      if (!b->graph_exec || std::memcmp(key, b->graph_key, sizeof(key)) != 0) {
+        if (b->graph_exec) {
+          HIPCHECK(hipGraphExecDestroy(b->graph_exec));
+          b->graph_exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        batch_subiterations(b, start_subiteration_num, num_subiterations, start_subset);
+        HIPCHECK(hipStreamEndCapture(b->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(e);
+        std::memcpy(b->graph_key, key, sizeof(key));
+      }
+      HIPCHECK(hipGraphLaunch(b->graph_exec, b->stream));
+    }
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_batch_synchronize(mi_batch* b) {
+  return guarded([&] {
+    if (!b) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    HIPCHECK(hipStreamSynchronize(b->stream));
   });
 }
 

@@ -136,6 +136,44 @@ void launch_entry_tof_distance(const Family* d_fams, int fam_begin, int fam_end,
                                const TofConsts& tof, float* d_entry_d, hipStream_t stream);
 void launch_gather_floats(const float* d_src, const uint32_t* d_idx, float* d_dst, int64_t n, hipStream_t stream);
 
+// ---- grouped kernels for many problems in one launch (pmrt_batch.hip)
+// One problem of a batch as the grouped kernels see it (device pointers into
+// that problem's own buffers).
+struct ProblemDesc {
+  const Family* fams;
+  const int2* pool;
+  const float* fp_in;        // forward-projection input image (device layout)
+  float* fp_out;             // forward-projection output
+  FpData fd;                 // epilogue data of the forward projection
+  const int64_t* list_off;   // back-projection table offsets per (subset, column)
+  const BpEntry* entries;
+  const float* bp_in;        // back-projection input
+  float* bp_out;             // BP_PLAIN output image
+  const float* const* sens;  // per-subset sensitivity images (BP_OSEM_UPDATE)
+  float* estimate;           // BP_OSEM_UPDATE: x, updated in place
+  ImageDims d;
+  int32_t ppa;               // planes per axial position shared by all families, 0 if they differ
+};
+// work item of a grouped launch: (problem, family) or (problem, image column);
+// index -1 is padding
+struct WorkItem {
+  int32_t problem;
+  int32_t index;
+};
+// G lanes per work item (G in 8, 16, 32, 64): a forward-projection group
+// handles one family with at most G axial positions, a back-projection group
+// one image column with at most G planes.
+void launch_fp_group(FpMode mode, int G, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork,
+                     hipStream_t stream);
+void launch_bp_group(BpMode mode, int G, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork, int subset,
+                     int threshold, hipStream_t stream);
+// One work item per wave with the per-problem kernels' bodies (pmrt_project.hip):
+// forward items index each problem's task list (± pairs), back-projection items are columns.
+void launch_fp_batch(FpMode mode, const ProblemDesc* d_descs, const FpTask* const* d_task_lists, const WorkItem* d_work,
+                     int nwork, hipStream_t stream);
+void launch_bp_batch(BpMode mode, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork, int subset,
+                     int threshold, hipStream_t stream);
+
 // ---- BP table build (pmrt_table.hip)
 void launch_count_entries(const Family* d_fams, int fam_begin, int fam_end, const int2* d_pool, int subset,
                           const ImageDims& dims, int32_t* d_counts, hipStream_t stream);

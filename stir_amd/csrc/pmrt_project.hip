@@ -155,18 +155,11 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask*
 // workgroup. Each table entry (wave-uniform, scalar loads) is decoded once
 // for all slots; kUnroll entries have their projection-data reads in flight
 // together and are accumulated in table order.
+// Back projection of one image column (wave-uniform col) over subsets [sb, se).
 template <int MODE, int PPA, int NSLOT, bool SKIP>
-__global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t* __restrict__ list_off,
-                                                                   const BpEntry* __restrict__ ent, int sb, int se,
-                                                                   const float* __restrict__ proj,
-                                                                   float* __restrict__ img, ImageDims d, int nwork,
-                                                                   int2 tile, int remap, BpData data) {
-  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int w = (remap ? xcd_remap(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kWavesPerBlock + wave;
-  if (w >= nwork) return;
-  const int col = tiled_column(w, d, tile);
-  if (col < 0) return;
-  const int lane = static_cast<int>(threadIdx.x % kWave);
+__device__ inline void bp_column(int col, const int64_t* __restrict__ list_off, const BpEntry* __restrict__ ent,
+                                 int sb, int se, const float* __restrict__ proj, float* __restrict__ img,
+                                 const ImageDims& d, const BpData& data, int lane) {
   float acc[NSLOT];
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
@@ -237,6 +230,86 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t
       }
     }
   }
+}
+
+template <int MODE, int PPA, int NSLOT, bool SKIP>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t* __restrict__ list_off,
+                                                                   const BpEntry* __restrict__ ent, int sb, int se,
+                                                                   const float* __restrict__ proj,
+                                                                   float* __restrict__ img, ImageDims d, int nwork,
+                                                                   int2 tile, int remap, BpData data) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int w = (remap ? xcd_remap(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kWavesPerBlock + wave;
+  if (w >= nwork) return;
+  const int col = tiled_column(w, d, tile);
+  if (col < 0) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  bp_column<MODE, PPA, NSLOT, SKIP>(col, list_off, ent, sb, se, proj, img, d, data, lane);
+}
+
+// ---- batches: one work item per wave, the problem's descriptor read
+// uniformly, the same per-family / per-column bodies as above.
+
+// forward: work item = (problem, task of that problem's task list)
+template <int MODE>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_batch_kernel(const ProblemDesc* __restrict__ descs,
+                                                                         const FpTask* const* __restrict__ task_lists,
+                                                                         const WorkItem* __restrict__ work, int nwork) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int wi = static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
+  if (wi >= nwork) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  const int p = __builtin_amdgcn_readfirstlane(work[wi].problem);
+  const int ti = __builtin_amdgcn_readfirstlane(work[wi].index);
+  const ProblemDesc& P = descs[p];
+  const FpTask task = task_lists[p][ti];
+  const Family f1 = P.fams[task.first];
+  const bool pair = task.second >= 0;
+  const Family f2 = P.fams[pair ? task.second : task.first];
+  const ImageDims d = P.d;
+  const int total = f1.nax + (pair ? f2.nax : 0);
+  switch ((total + kWave - 1) / kWave) {
+    case 1: fp_task<MODE, 1>(f1, f2, pair, P.pool, P.fp_in, P.fp_out, d, P.fd, wi, lane); break;
+    case 2: fp_task<MODE, 2>(f1, f2, pair, P.pool, P.fp_in, P.fp_out, d, P.fd, wi, lane); break;
+    case 3: fp_task<MODE, 3>(f1, f2, pair, P.pool, P.fp_in, P.fp_out, d, P.fd, wi, lane); break;
+    default: fp_task<MODE, 4>(f1, f2, pair, P.pool, P.fp_in, P.fp_out, d, P.fd, wi, lane); break;
+  }
+}
+
+// back projection: work item = (problem, image column), subset `subset`
+template <int MODE>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const ProblemDesc* __restrict__ descs,
+                                                                         const WorkItem* __restrict__ work, int nwork,
+                                                                         int subset, int threshold) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int wi = static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
+  if (wi >= nwork) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  const int p = __builtin_amdgcn_readfirstlane(work[wi].problem);
+  const int col = __builtin_amdgcn_readfirstlane(work[wi].index);
+  if (col < 0) return;
+  const ProblemDesc& P = descs[p];
+  const ImageDims d = P.d;
+  BpData bd{};
+  bd.sens = P.sens ? P.sens[subset] : nullptr;
+  bd.estimate = P.estimate;
+  bd.update = nullptr;
+  bd.threshold = threshold;
+  const int ppa = __builtin_amdgcn_readfirstlane(P.ppa);
+  const int nslot = (d.nzp + kWave - 1) / kWave;
+#define MI_BPB(PP, S) bp_column<MODE, PP, S, false>(col, P.list_off, P.entries, subset, subset + 1, P.bp_in, P.bp_out, d, bd, lane)
+#define MI_BPB_S(PP)                 \
+  switch (nslot) {                   \
+    case 1: MI_BPB(PP, 1); break;    \
+    case 2: MI_BPB(PP, 2); break;    \
+    case 3: MI_BPB(PP, 3); break;    \
+    default: MI_BPB(PP, 4); break;   \
+  }
+  if (ppa == 1) { MI_BPB_S(1) }
+  else if (ppa == 2) { MI_BPB_S(2) }
+  else { MI_BPB_S(0) }
+#undef MI_BPB_S
+#undef MI_BPB
 }
 
 // dst[c * dst_ld + r] = src[r * src_ld + c], 32x32 tiles through LDS
@@ -347,6 +420,26 @@ void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const 
 #undef MI_BP_SLOTS
 #undef MI_BP_SKIP
 #undef MI_BP_LAUNCH
+}
+
+void launch_fp_batch(FpMode mode, const ProblemDesc* d_descs, const FpTask* const* d_task_lists, const WorkItem* d_work,
+                     int nwork, hipStream_t stream) {
+  if (nwork <= 0) return;
+  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  if (mode == FP_RATIO)
+    hipLaunchKernelGGL(fp_batch_kernel<FP_RATIO>, grid, block, 0, stream, d_descs, d_task_lists, d_work, nwork);
+  else
+    hipLaunchKernelGGL(fp_batch_kernel<FP_PLAIN>, grid, block, 0, stream, d_descs, d_task_lists, d_work, nwork);
+}
+
+void launch_bp_batch(BpMode mode, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork, int subset,
+                     int threshold, hipStream_t stream) {
+  if (nwork <= 0) return;
+  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  if (mode == BP_OSEM_UPDATE)
+    hipLaunchKernelGGL(bp_batch_kernel<BP_OSEM_UPDATE>, grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold);
+  else
+    hipLaunchKernelGGL(bp_batch_kernel<BP_PLAIN>, grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold);
 }
 
 void launch_transpose(const float* src, float* dst, int rows, int cols, int src_ld, int dst_ld, hipStream_t stream) {

@@ -164,10 +164,14 @@ __device__ void fp_tof_task(const Family& f, const FamilyLor& lor, const int2* _
           // a predicate, not a break: the loop must stay fully unrolled so that
           // acc and tw remain in registers
           if (k < T.ntof) {
+            // elements whose TOF weight is 0 are not in the reference's TOF row:
+            // skip them (wave-uniform), so an infinite image value does not give 0 * inf
             const float wk = bcast(tw[k], j + u);
+            if (wk != 0.F) {
 #pragma unroll
-            for (int q = 0; q < NSLOT; ++q)
-              if (ok[u][q]) acc[q][k] = acc[q][k] + v[u][q] * wk;
+              for (int q = 0; q < NSLOT; ++q)
+                if (ok[u][q]) acc[q][k] = acc[q][k] + v[u][q] * wk;
+            }
           }
         }
     }
@@ -187,9 +191,11 @@ __device__ void fp_tof_task(const Family& f, const FamilyLor& lor, const int2* _
       for (int k = 0; k < NT; ++k) {
         if (k < T.ntof) {
           const float wk = bcast(tw[k], j);
+          if (wk != 0.F) {
 #pragma unroll
-          for (int q = 0; q < NSLOT; ++q)
-            if (ok[q]) acc[q][k] = acc[q][k] + v[q] * wk;
+            for (int q = 0; q < NSLOT; ++q)
+              if (ok[q]) acc[q][k] = acc[q][k] + v[q] * wk;
+          }
         }
       }
     }
@@ -283,6 +289,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_kernel(const int
         for (int k = 0; k < NT; ++k) {
           if (k < kl || k > kh) continue;
           const float wk = bcast(tw[k], j);
+          if (wk == 0.F) continue;  // not in the reference's TOF row
           const float* run = proj + static_cast<int64_t>(k) * T.block + E.sino_base;
 #pragma unroll
           for (int q = 0; q < NSLOT; ++q) {
