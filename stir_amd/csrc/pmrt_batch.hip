@@ -61,16 +61,16 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_group_kernel(const P
   const ProblemDesc& P = descs[W.problem];
   const ImageDims d = P.d;
   Family f{};
-  if (item) f = P.fams[W.index];
+  if (item) f = as_const(P.fams)[W.index];
   const bool valid = item && a < f.nax;
   int ca, cb, cc, ce;
   inplane_coeffs(f.inplane, ca, cb, cc, ce);
   const int zs = f.zsign;
   const int zb = f.zadd0 + f.ppa * a - d.min_z;
-  const int2* row = P.pool + f.row_start;
+  cptr<int2> row = as_const(P.pool) + f.row_start;
   const int n = item ? f.row_len : 0;
   const int nmax = wave_max(n);
-  const float* img = P.fp_in;
+  cptr<float> img = as_const(P.fp_in);
   float acc = 0.F;
   for (int e0 = 0; e0 < nmax; e0 += kGroupUnroll) {
     float v[kGroupUnroll], w[kGroupUnroll];
@@ -116,14 +116,15 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_group_kernel(const P
   const ImageDims d = P.d;
   int64_t beg = 0, n = 0;
   if (item) {
-    beg = P.list_off[static_cast<int64_t>(subset) * d.ncols + W.index];
-    n = P.list_off[static_cast<int64_t>(subset) * d.ncols + W.index + 1] - beg;
+    cptr<int64_t> lo = as_const(P.list_off);
+    beg = lo[static_cast<int64_t>(subset) * d.ncols + W.index];
+    n = lo[static_cast<int64_t>(subset) * d.ncols + W.index + 1] - beg;
   }
   const int nmax = wave_max(static_cast<int>(n));
   const int zi = d.min_z + zl;
   const bool zin = item && zl < d.nz;
-  const BpEntry* ent = P.entries + beg;
-  const float* proj = P.bp_in;
+  cptr<BpEntry> ent = as_const(P.entries) + beg;
+  cptr<float> proj = as_const(P.bp_in);
   float acc = 0.F;
   for (int e0 = 0; e0 < nmax; e0 += kGroupUnroll) {
     float p[kGroupUnroll], w[kGroupUnroll];
@@ -159,19 +160,19 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_group_kernel(const P
   if (!item || zl >= d.nzp) return;
   const int64_t idx = static_cast<int64_t>(W.index) * d.nzp + zl;
   if constexpr (MODE == BP_PLAIN) {
-    P.bp_out[idx] = zl < d.nz ? acc : 0.F;
+    as_global(P.bp_out)[idx] = zl < d.nz ? acc : 0.F;
   } else {
     if (zl < d.nz) {
       // divide(update, sens, 0) (numerics/divide.inl:26-47), threshold, multiply
       float u = acc;
-      const float sv = P.sens[subset][idx];
+      const float sv = as_const(as_const(P.sens)[subset])[idx];
       if (fabsf(sv) <= 0.F && fabsf(u) <= 0.F) u = 0;
       else u = u / sv;
       if (threshold) {
         if (u > FLT_MAX) u = FLT_MAX;
         else if (0.F > u) u = 0.F;
       }
-      P.estimate[idx] = P.estimate[idx] * u;
+      as_global(P.estimate)[idx] = as_global(P.estimate)[idx] * u;
     }
   }
 }

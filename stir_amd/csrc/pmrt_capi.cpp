@@ -1194,12 +1194,14 @@ struct mi_batch {
   std::vector<std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 4>> fp_work;
   std::array<int, 4> bp_n{};
   std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 4> bp_work;
-  // full-wave items: forward (problem, task) per subset, back projection (problem, column)
+  // full-wave items: forward (problem, task) per subset and slot count,
+  // back projection (problem, column) per planes-per-axial-position class
+  // (ppa 0 = any, 1, 2) and slot count: [class * 4 + nslot - 1]
   DBuf<const mi::FpTask*> task_lists;
-  std::vector<int> fpw_n;
-  std::vector<std::unique_ptr<DBuf<mi::WorkItem>>> fpw_work;
-  int bpw_n = 0;
-  std::unique_ptr<DBuf<mi::WorkItem>> bpw_work;
+  std::vector<std::array<int, 4>> fpw_n;
+  std::vector<std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 4>> fpw_work;
+  std::array<int, 12> bpw_n{};
+  std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 12> bpw_work;
   bool lists_ready = false;
   hipGraphExec_t graph_exec = nullptr;
   int graph_key[3] = {0, 0, 0};  // start subiteration, count, start subset of the captured graph
@@ -1280,11 +1282,12 @@ void batch_prepare(mi_batch* b) {
   b->fp_n.assign(b->nsub, {0, 0, 0, 0});
   b->fp_work.clear();
   b->fp_work.resize(b->nsub);
-  b->fpw_n.assign(b->nsub, 0);
+  b->fpw_n.assign(b->nsub, {0, 0, 0, 0});
   b->fpw_work.clear();
+  b->fpw_work.resize(b->nsub);
   for (int s = 0; s < b->nsub; ++s) {
     std::array<std::vector<mi::WorkItem>, 4> lists;
-    std::vector<mi::WorkItem> wide;
+    std::array<std::vector<mi::WorkItem>, 4> wide;
     for (int p = 0; p < np; ++p) {
       const mi_ctx* c = b->probs[p];
       std::vector<mi::FpTask> ht(c->task_off[s + 1] - c->task_off[s]);
@@ -1294,7 +1297,8 @@ void batch_prepare(mi_batch* b) {
         const mi::FpTask& t = ht[i];
         const int total = c->h_fams[t.first].nax + (t.second >= 0 ? c->h_fams[t.second].nax : 0);
         if (total > b->fp_group_max) {
-          wide.push_back({p, c->task_off[s] + static_cast<int>(i)});
+          const int nslot = std::min((total + 63) / 64, 4);
+          wide[nslot - 1].push_back({p, c->task_off[s] + static_cast<int>(i)});
           continue;
         }
         for (int f : {t.first, t.second}) {
@@ -1308,17 +1312,21 @@ void batch_prepare(mi_batch* b) {
       b->fp_n[s][gi] = static_cast<int>(lists[gi].size());
       b->fp_work[s][gi] = upload(lists[gi]);
     }
-    b->fpw_n[s] = static_cast<int>(wide.size());
-    b->fpw_work.push_back(upload(wide));
+    for (int k = 0; k < 4; ++k) {
+      b->fpw_n[s][k] = static_cast<int>(wide[k].size());
+      b->fpw_work[s][k] = upload(wide[k]);
+    }
   }
   // back-projection work: the columns of every problem in tile order; images
   // of at most 32 planes in lane groups (when packing), the others one column per wave
   std::array<std::vector<mi::WorkItem>, 4> lists;
-  std::vector<mi::WorkItem> wide;
+  std::array<std::vector<mi::WorkItem>, 12> wide;
   for (int p = 0; p < np; ++p) {
     const mi::ImageDims& d = b->probs[p]->dims;
     const int gi = group_index_at_least(d.nzp);
     const bool grouped = b->bp_pack && gi >= 0 && gi <= 2;
+    const int ppa = b->probs[p]->uniform_ppa;
+    const int wclass = (ppa == 1 || ppa == 2 ? ppa : 0) * 4 + std::min((d.nzp + 63) / 64, 4) - 1;
     const int tx = std::max(b->tile_x, 1), ty = std::max(b->tile_y, 1);
     for (int y0 = 0; y0 < d.ny; y0 += ty)
       for (int x0 = 0; x0 < d.nx; x0 += tx)
@@ -1327,25 +1335,29 @@ void batch_prepare(mi_batch* b) {
             if (y < d.ny && x < d.nx) {
               const mi::WorkItem w{p, y * d.nx + x};
               if (grouped) lists[std::max(gi, 1)].push_back(w);
-              else wide.push_back(w);
+              else wide[wclass].push_back(w);
             }
   }
   for (int gi = 0; gi < 4; ++gi) {
     b->bp_n[gi] = static_cast<int>(lists[gi].size());
     b->bp_work[gi] = upload(lists[gi]);
   }
-  b->bpw_n = static_cast<int>(wide.size());
-  b->bpw_work = upload(wide);
+  for (int k = 0; k < 12; ++k) {
+    b->bpw_n[k] = static_cast<int>(wide[k].size());
+    b->bpw_work[k] = upload(wide[k]);
+  }
   b->lists_ready = true;
 }
 
 void batch_fp(mi_batch* b, mi::FpMode mode, int s) {
-  mi::launch_fp_batch(mode, b->descs.p, b->task_lists.p, b->fpw_work[s]->p, b->fpw_n[s], b->stream);
+  for (int k = 0; k < 4; ++k)
+    mi::launch_fp_batch(mode, k + 1, b->descs.p, b->task_lists.p, b->fpw_work[s][k]->p, b->fpw_n[s][k], b->stream);
   for (int gi = 0; gi < 4; ++gi)
     mi::launch_fp_group(mode, kGroupSizes[gi], b->descs.p, b->fp_work[s][gi]->p, b->fp_n[s][gi], b->stream);
 }
 void batch_bp(mi_batch* b, mi::BpMode mode, int s, int threshold) {
-  mi::launch_bp_batch(mode, b->descs.p, b->bpw_work->p, b->bpw_n, s, threshold, b->stream);
+  for (int k = 0; k < 12; ++k)
+    mi::launch_bp_batch(mode, k / 4, k % 4 + 1, b->descs.p, b->bpw_work[k]->p, b->bpw_n[k], s, threshold, b->stream);
   for (int gi = 1; gi < 4; ++gi)
     mi::launch_bp_group(mode, kGroupSizes[gi], b->descs.p, b->bp_work[gi]->p, b->bp_n[gi], s, threshold, b->stream);
 }

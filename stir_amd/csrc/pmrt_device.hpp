@@ -168,11 +168,14 @@ void launch_fp_group(FpMode mode, int G, const ProblemDesc* d_descs, const WorkI
 void launch_bp_group(BpMode mode, int G, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork, int subset,
                      int threshold, hipStream_t stream);
 // One work item per wave with the per-problem kernels' bodies (pmrt_project.hip):
-// forward items index each problem's task list (± pairs), back-projection items are columns.
-void launch_fp_batch(FpMode mode, const ProblemDesc* d_descs, const FpTask* const* d_task_lists, const WorkItem* d_work,
-                     int nwork, hipStream_t stream);
-void launch_bp_batch(BpMode mode, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork, int subset,
-                     int threshold, hipStream_t stream);
+// forward items index each problem's task list (± pairs) and every task of a
+// launch has at most 64 * nslot bins; back-projection items are columns of
+// images with at most 64 * nslot planes and `ppa` planes per axial position
+// (1, 2, or 0 for any).
+void launch_fp_batch(FpMode mode, int nslot, const ProblemDesc* d_descs, const FpTask* const* d_task_lists,
+                     const WorkItem* d_work, int nwork, hipStream_t stream);
+void launch_bp_batch(BpMode mode, int ppa, int nslot, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork,
+                     int subset, int threshold, hipStream_t stream);
 
 // ---- BP table build (pmrt_table.hip)
 void launch_count_entries(const Family* d_fams, int fam_begin, int fam_end, const int2* d_pool, int subset,

@@ -26,11 +26,44 @@ __device__ inline int xcd_remap(int bid, int nblocks) {
   return start + idx;
 }
 
+// Address spaces. Tables and inputs that a kernel only reads are accessed
+// through the constant address space, so that reads at wave-uniform
+// addresses become scalar loads; outputs through the global one. Kernel
+// arguments get this from their declarations already; pointers read out of
+// device descriptors (the batch kernels) carry no address space and would
+// otherwise be accessed with per-lane flat loads.
+// (The host pass of the compiler only checks these device functions and
+// has no such address spaces.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MI_AS_CONST __attribute__((address_space(4)))
+#define MI_AS_GLOBAL __attribute__((address_space(1)))
+#else
+#define MI_AS_CONST
+#define MI_AS_GLOBAL
+#endif
+template <class T>
+using cptr = const MI_AS_CONST T*;
+template <class T>
+using gptr = MI_AS_GLOBAL T*;
+template <class T>
+__device__ inline cptr<T> as_const(const T* p) {
+  return (cptr<T>)p;
+}
+template <class T>
+__device__ inline gptr<T> as_global(T* p) {
+  return (gptr<T>)p;
+}
+
 // base[idx] with a wave-uniform base and a 32-bit unsigned lane offset, so
 // the address is formed as scalar base + 32-bit vector offset
 __device__ inline float load_u32off(const float* base, unsigned idx) {
   return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (idx << 2));
 }
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline float load_u32off(cptr<float> base, unsigned idx) {
+  return *(cptr<float>)((const MI_AS_CONST char*)base + (idx << 2));
+}
+#endif
 
 template <int IP>
 __device__ inline void inplane_op(int x, int y, int& xo, int& yo) {
@@ -104,13 +137,13 @@ constexpr bool fp_accumulates_loglik(int m) { return m == FP_LOGLIK || m == FP_G
 template <int MODE>
 __device__ inline void fp_epilogue(float a, int64_t i, int vg, const FpData& data, float* __restrict__ out, double& ll) {
   if constexpr (MODE == FP_PLAIN) {
-    out[i] = a;
+    as_global(out)[i] = a;
   } else if constexpr (MODE == FP_SCALE) {
-    out[i] = a * data.scale[i];
+    as_global(out)[i] = a * as_const(data.scale)[i];
   } else {
-    const float ybar = a + (data.additive ? data.additive[i] : data.additive_constant);
-    const float yv = data.measured[i];
-    const float small_value = data.vg_small[vg];
+    const float ybar = a + (data.additive ? as_const(data.additive)[i] : data.additive_constant);
+    const float yv = as_const(data.measured)[i];
+    const float small_value = as_const(data.vg_small)[vg];
     const float max_quotient = 10000.F;
     if constexpr (fp_writes_ratio(MODE)) {
       // estimated += additive; divide_and_truncate (recon_array_functions.cxx:172-266)
@@ -121,8 +154,8 @@ __device__ inline void fp_epilogue(float a, int64_t i, int vg, const FpData& dat
         if (num > max_quotient * ybar) num = max_quotient;
         else num = num / ybar;
       }
-      if constexpr (MODE == FP_GRADIENT || MODE == FP_GRADIENT_LOGLIK) num = num - (data.mult ? data.mult[i] : 1.F);
-      out[i] = num;
+      if constexpr (MODE == FP_GRADIENT || MODE == FP_GRADIENT_LOGLIK) num = num - (data.mult ? as_const(data.mult)[i] : 1.F);
+      as_global(out)[i] = num;
     }
     if constexpr (MODE == FP_DRATIO) {
       // derivative of the quotient above: -y/ybar^2 where y/ybar is used, 0 where truncated
@@ -131,12 +164,12 @@ __device__ inline void fp_epilogue(float a, int64_t i, int vg, const FpData& dat
         const float r = yv / ybar;
         dr = -r / ybar;
       }
-      out[i] = dr;
+      as_global(out)[i] = dr;
     }
     if constexpr (fp_accumulates_loglik(MODE)) {
       // accumulate_loglikelihood (recon_array_functions.cxx:336-377)
       float est = ybar;
-      if (data.mult) est = est * data.mult[i];
+      if (data.mult) est = est * as_const(data.mult)[i];
       const float ne = (est < yv / 10000.F) ? yv / 10000.F : est;
       if (yv <= small_value) ll += -static_cast<double>(ne);
       else ll += yv * log(static_cast<double>(ne)) - static_cast<double>(ne);

@@ -36,8 +36,8 @@ constexpr int kUnroll = 8;      // matrix elements whose gathers are in flight t
 // the image read a clamped (valid) address and skip the add, so each sum is
 // the reference's, term by term.
 template <int IP, int NSLOT>
-__device__ inline void fp_row(const int2* __restrict__ row, int n, const LaneBins<NSLOT>& L,
-                              const float* __restrict__ img, const ImageDims& d, float (&acc)[NSLOT]) {
+__device__ inline void fp_row(cptr<int2> row, int n, const LaneBins<NSLOT>& L, cptr<float> img, const ImageDims& d,
+                              float (&acc)[NSLOT]) {
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
   int e = 0;
@@ -51,7 +51,7 @@ __device__ inline void fp_row(const int2* __restrict__ row, int n, const LaneBin
       decode(el.x, x, y, z);
       inplane_op<IP>(x, y, xo, yo);
       const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-      const float* colp = img + static_cast<int64_t>(col) * d.nzp;  // wave-uniform
+      cptr<float> colp = img + static_cast<int64_t>(col) * d.nzp;  // wave-uniform
       w[j] = __int_as_float(el.y);
 #pragma unroll
       for (int q = 0; q < NSLOT; ++q) {
@@ -73,7 +73,7 @@ __device__ inline void fp_row(const int2* __restrict__ row, int n, const LaneBin
     decode(el.x, x, y, z);
     inplane_op<IP>(x, y, xo, yo);
     const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-    const float* colp = img + static_cast<int64_t>(col) * d.nzp;
+    cptr<float> colp = img + static_cast<int64_t>(col) * d.nzp;
 #pragma unroll
     for (int q = 0; q < NSLOT; ++q) {
       const int zz = L.zs[q] * z + L.zb[q];
@@ -84,9 +84,8 @@ __device__ inline void fp_row(const int2* __restrict__ row, int n, const LaneBin
 }
 
 template <int MODE, int NSLOT>
-__device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, const int2* __restrict__ pool,
-                               const float* __restrict__ img, float* __restrict__ out, const ImageDims& d,
-                               const FpData& data, int t, int lane) {
+__device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, cptr<int2> pool, cptr<float> img,
+                               float* __restrict__ out, const ImageDims& d, const FpData& data, int t, int lane) {
   LaneBins<NSLOT> L;
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) {
@@ -102,7 +101,7 @@ __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, co
     L.vg[q] = in1 ? f1.viewgram : f2.viewgram;
     (void)f;
   }
-  const int2* row = pool + f1.row_start;
+  cptr<int2> row = pool + f1.row_start;
   float acc[NSLOT];
   switch (f1.inplane) {
     case IP_ID: fp_row<IP_ID, NSLOT>(row, f1.row_len, L, img, d, acc); break;
@@ -120,7 +119,7 @@ __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, co
     if (L.valid[q]) fp_epilogue<MODE>(acc[q], L.out[q], L.vg[q], data, out, ll);
   if constexpr (fp_accumulates_loglik(MODE)) {
     for (int off = kWave / 2; off > 0; off >>= 1) ll += __shfl_xor(ll, off, kWave);
-    if (lane == 0) data.partial[t] = ll;
+    if (lane == 0) as_global(data.partial)[t] = ll;
   }
 }
 
@@ -143,10 +142,10 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask*
   const int total = f1.nax + (pair ? f2.nax : 0);
   const int nslot = (total + kWave - 1) / kWave;
   switch (nslot) {
-    case 1: fp_task<MODE, 1>(f1, f2, pair, pool, img, out, d, data, t, lane); break;
-    case 2: fp_task<MODE, 2>(f1, f2, pair, pool, img, out, d, data, t, lane); break;
-    case 3: fp_task<MODE, 3>(f1, f2, pair, pool, img, out, d, data, t, lane); break;
-    default: fp_task<MODE, 4>(f1, f2, pair, pool, img, out, d, data, t, lane); break;
+    case 1: fp_task<MODE, 1>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane); break;
+    case 2: fp_task<MODE, 2>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane); break;
+    case 3: fp_task<MODE, 3>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane); break;
+    default: fp_task<MODE, 4>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane); break;
   }
 }
 
@@ -157,9 +156,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask*
 // together and are accumulated in table order.
 // Back projection of one image column (wave-uniform col) over subsets [sb, se).
 template <int MODE, int PPA, int NSLOT, bool SKIP>
-__device__ inline void bp_column(int col, const int64_t* __restrict__ list_off, const BpEntry* __restrict__ ent,
-                                 int sb, int se, const float* __restrict__ proj, float* __restrict__ img,
-                                 const ImageDims& d, const BpData& data, int lane) {
+__device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> ent, int sb, int se, cptr<float> proj,
+                                 float* __restrict__ img, const ImageDims& d, const BpData& data, int lane) {
   float acc[NSLOT];
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
@@ -173,7 +171,7 @@ __device__ inline void bp_column(int col, const int64_t* __restrict__ list_off, 
 #pragma unroll
       for (int j = 0; j < kUnroll; ++j) {
         const BpEntry E = ent[e + j];
-        const float* run = proj + E.sino_base;  // wave-uniform
+        cptr<float> run = proj + E.sino_base;  // wave-uniform
         wt[j] = E.w;
         // z range the entry's run can reach (wave-uniform): [k, k + ppa*(nax-1)]
         const int ppa_e = PPA > 0 ? PPA : (E.nax_ppa >> 16);
@@ -197,7 +195,7 @@ __device__ inline void bp_column(int col, const int64_t* __restrict__ list_off, 
     }
     for (; e < end; ++e) {
       const BpEntry E = ent[e];
-      const float* run = proj + E.sino_base;
+      cptr<float> run = proj + E.sino_base;
 #pragma unroll
       for (int q = 0; q < NSLOT; ++q) {
         bool ok;
@@ -206,7 +204,7 @@ __device__ inline void bp_column(int col, const int64_t* __restrict__ list_off, 
       }
     }
   }
-  float* out = img + static_cast<int64_t>(col) * d.nzp;
+  gptr<float> out = as_global(img) + static_cast<int64_t>(col) * d.nzp;
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) {
     const int z = lane + kWave * q;
@@ -218,15 +216,15 @@ __device__ inline void bp_column(int col, const int64_t* __restrict__ list_off, 
         // divide(update, sens, 0) (numerics/divide.inl:26-47), threshold, multiply
         const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
         float u = acc[q];
-        const float sv = data.sens[idx];
+        const float sv = as_const(data.sens)[idx];
         if (fabsf(sv) <= 0.F && fabsf(u) <= 0.F) u = 0;
         else u = u / sv;
         if (data.threshold) {
           if (u > FLT_MAX) u = FLT_MAX;
           else if (0.F > u) u = 0.F;
         }
-        data.estimate[idx] = data.estimate[idx] * u;
-        if (data.update) data.update[idx] = u;
+        as_global(data.estimate)[idx] = as_global(data.estimate)[idx] * u;
+        if (data.update) as_global(data.update)[idx] = u;
       }
     }
   }
@@ -244,14 +242,18 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t
   const int col = tiled_column(w, d, tile);
   if (col < 0) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
-  bp_column<MODE, PPA, NSLOT, SKIP>(col, list_off, ent, sb, se, proj, img, d, data, lane);
+  bp_column<MODE, PPA, NSLOT, SKIP>(col, as_const(list_off), as_const(ent), sb, se, as_const(proj), img, d, data, lane);
 }
 
 // ---- batches: one work item per wave, the problem's descriptor read
-// uniformly, the same per-family / per-column bodies as above.
+// uniformly, the same per-family / per-column bodies as above. The host
+// sorts the items by slot count (and back projection by planes per axial
+// position), so each launch instantiates one body only and keeps that
+// body's register use.
 
-// forward: work item = (problem, task of that problem's task list)
-template <int MODE>
+// forward: work item = (problem, task of that problem's task list); every
+// task of the launch has at most 64 * NSLOT bins
+template <int MODE, int NSLOT>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_batch_kernel(const ProblemDesc* __restrict__ descs,
                                                                          const FpTask* const* __restrict__ task_lists,
                                                                          const WorkItem* __restrict__ work, int nwork) {
@@ -262,22 +264,18 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_batch_kernel(const P
   const int p = __builtin_amdgcn_readfirstlane(work[wi].problem);
   const int ti = __builtin_amdgcn_readfirstlane(work[wi].index);
   const ProblemDesc& P = descs[p];
-  const FpTask task = task_lists[p][ti];
-  const Family f1 = P.fams[task.first];
+  const FpTask task = as_const(task_lists[p])[ti];
+  cptr<Family> fams = as_const(P.fams);
+  const Family f1 = fams[task.first];
   const bool pair = task.second >= 0;
-  const Family f2 = P.fams[pair ? task.second : task.first];
-  const ImageDims d = P.d;
-  const int total = f1.nax + (pair ? f2.nax : 0);
-  switch ((total + kWave - 1) / kWave) {
-    case 1: fp_task<MODE, 1>(f1, f2, pair, P.pool, P.fp_in, P.fp_out, d, P.fd, wi, lane); break;
-    case 2: fp_task<MODE, 2>(f1, f2, pair, P.pool, P.fp_in, P.fp_out, d, P.fd, wi, lane); break;
-    case 3: fp_task<MODE, 3>(f1, f2, pair, P.pool, P.fp_in, P.fp_out, d, P.fd, wi, lane); break;
-    default: fp_task<MODE, 4>(f1, f2, pair, P.pool, P.fp_in, P.fp_out, d, P.fd, wi, lane); break;
-  }
+  const Family f2 = fams[pair ? task.second : task.first];
+  fp_task<MODE, NSLOT>(f1, f2, pair, as_const(P.pool), as_const(P.fp_in), P.fp_out, P.d, P.fd, wi, lane);
 }
 
-// back projection: work item = (problem, image column), subset `subset`
-template <int MODE>
+// back projection: work item = (problem, image column), subset `subset`;
+// every image of the launch has at most 64 * NSLOT planes and PPA planes per
+// axial position (0: any)
+template <int MODE, int PPA, int NSLOT>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const ProblemDesc* __restrict__ descs,
                                                                          const WorkItem* __restrict__ work, int nwork,
                                                                          int subset, int threshold) {
@@ -289,27 +287,13 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const P
   const int col = __builtin_amdgcn_readfirstlane(work[wi].index);
   if (col < 0) return;
   const ProblemDesc& P = descs[p];
-  const ImageDims d = P.d;
   BpData bd{};
-  bd.sens = P.sens ? P.sens[subset] : nullptr;
+  bd.sens = P.sens ? as_const(P.sens)[subset] : nullptr;
   bd.estimate = P.estimate;
   bd.update = nullptr;
   bd.threshold = threshold;
-  const int ppa = __builtin_amdgcn_readfirstlane(P.ppa);
-  const int nslot = (d.nzp + kWave - 1) / kWave;
-#define MI_BPB(PP, S) bp_column<MODE, PP, S, false>(col, P.list_off, P.entries, subset, subset + 1, P.bp_in, P.bp_out, d, bd, lane)
-#define MI_BPB_S(PP)                 \
-  switch (nslot) {                   \
-    case 1: MI_BPB(PP, 1); break;    \
-    case 2: MI_BPB(PP, 2); break;    \
-    case 3: MI_BPB(PP, 3); break;    \
-    default: MI_BPB(PP, 4); break;   \
-  }
-  if (ppa == 1) { MI_BPB_S(1) }
-  else if (ppa == 2) { MI_BPB_S(2) }
-  else { MI_BPB_S(0) }
-#undef MI_BPB_S
-#undef MI_BPB
+  bp_column<MODE, PPA, NSLOT, false>(col, as_const(P.list_off), as_const(P.entries), subset, subset + 1,
+                                     as_const(P.bp_in), P.bp_out, P.d, bd, lane);
 }
 
 // dst[c * dst_ld + r] = src[r * src_ld + c], 32x32 tiles through LDS
@@ -422,24 +406,47 @@ void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const 
 #undef MI_BP_LAUNCH
 }
 
-void launch_fp_batch(FpMode mode, const ProblemDesc* d_descs, const FpTask* const* d_task_lists, const WorkItem* d_work,
-                     int nwork, hipStream_t stream) {
+void launch_fp_batch(FpMode mode, int nslot, const ProblemDesc* d_descs, const FpTask* const* d_task_lists,
+                     const WorkItem* d_work, int nwork, hipStream_t stream) {
   if (nwork <= 0) return;
   const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
-  if (mode == FP_RATIO)
-    hipLaunchKernelGGL(fp_batch_kernel<FP_RATIO>, grid, block, 0, stream, d_descs, d_task_lists, d_work, nwork);
-  else
-    hipLaunchKernelGGL(fp_batch_kernel<FP_PLAIN>, grid, block, 0, stream, d_descs, d_task_lists, d_work, nwork);
+#define MI_FPB(M, S) \
+  hipLaunchKernelGGL((fp_batch_kernel<M, S>), grid, block, 0, stream, d_descs, d_task_lists, d_work, nwork)
+#define MI_FPB_S(M)              \
+  switch (nslot) {               \
+    case 1: MI_FPB(M, 1); break; \
+    case 2: MI_FPB(M, 2); break; \
+    case 3: MI_FPB(M, 3); break; \
+    default: MI_FPB(M, 4); break; \
+  }
+  if (mode == FP_RATIO) { MI_FPB_S(FP_RATIO) }
+  else { MI_FPB_S(FP_PLAIN) }
+#undef MI_FPB_S
+#undef MI_FPB
 }
 
-void launch_bp_batch(BpMode mode, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork, int subset,
-                     int threshold, hipStream_t stream) {
+void launch_bp_batch(BpMode mode, int ppa, int nslot, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork,
+                     int subset, int threshold, hipStream_t stream) {
   if (nwork <= 0) return;
   const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
-  if (mode == BP_OSEM_UPDATE)
-    hipLaunchKernelGGL(bp_batch_kernel<BP_OSEM_UPDATE>, grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold);
-  else
-    hipLaunchKernelGGL(bp_batch_kernel<BP_PLAIN>, grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold);
+#define MI_BPB(M, P, S)                                                                                         \
+  hipLaunchKernelGGL((bp_batch_kernel<M, P, S>), grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold)
+#define MI_BPB_S(M, P)              \
+  switch (nslot) {                  \
+    case 1: MI_BPB(M, P, 1); break; \
+    case 2: MI_BPB(M, P, 2); break; \
+    case 3: MI_BPB(M, P, 3); break; \
+    default: MI_BPB(M, P, 4); break; \
+  }
+#define MI_BPB_P(M)                  \
+  if (ppa == 1) { MI_BPB_S(M, 1) }   \
+  else if (ppa == 2) { MI_BPB_S(M, 2) } \
+  else { MI_BPB_S(M, 0) }
+  if (mode == BP_OSEM_UPDATE) { MI_BPB_P(BP_OSEM_UPDATE) }
+  else { MI_BPB_P(BP_PLAIN) }
+#undef MI_BPB_P
+#undef MI_BPB_S
+#undef MI_BPB
 }
 
 void launch_transpose(const float* src, float* dst, int rows, int cols, int src_ld, int dst_ld, hipStream_t stream) {
