@@ -179,6 +179,21 @@ int mi_obj_compute_sensitivities(mi_ctx* ctx);
  * (:533-536,648-670), with 1 TOF back projections of TOF ones. Takes effect at
  * the next mi_obj_compute_sensitivities. */
 int mi_obj_set_use_tof_sensitivities(mi_ctx* ctx, int use);
+/* Multiplicative factors m of the model ybar = m (A x + b): the
+ * normalisation's undo() applied to ones ("Bin Normalisation type",
+ * ...AndProjData.cxx:158,551-565; get_viewgrams, distributable.cxx:197-223).
+ * Host array, ProjDataInMemory order, all TOF bins; NULL removes them. They
+ * enter the sensitivities (BP of m), the gradient (BP(y/(A x + b) - m),
+ * RPC at ...AndProjData.cxx:1450-1465) and the log-likelihood
+ * ((A x + b) m, :1484-1515); "zero end planes of segment 0" zeroes their end
+ * sinograms. With TOF data and non-TOF sensitivities every TOF bin must carry
+ * the same factors (:636-646). Sensitivities must be recomputed. */
+int mi_obj_set_multiplicative(mi_ctx* ctx, const float* mult);
+/* "use subset sensitivities" (default 1, ...ForMean.cxx:48). 0: every subset
+ * uses the total sensitivity (accumulated subset by subset) divided by the
+ * number of subsets (:374-455); subsets must be balanced. Takes effect at the
+ * next mi_obj_compute_sensitivities. */
+int mi_obj_set_use_subset_sensitivities(mi_ctx* ctx, int use);
 int mi_obj_get_sensitivity(mi_ctx* ctx, int subset, float* image);
 /* Current estimate (host, STIR layout). mi_obj_set_estimate applies
  * threshold_min_to_small_positive_value(x, 1e-6) when enforce_positivity
@@ -189,6 +204,34 @@ int mi_obj_get_estimate(mi_ctx* ctx, float* image);
  * OSMAPOSLReconstruction.cxx:373-516): x *= clamp(BP_s(y/(FP_s(x)+b))/sens_s).
  * subiteration_num is 1-based (thresholding is skipped for subiteration 1). */
 int mi_osem_subset_step(mi_ctx* ctx, int subset, int subiteration_num);
+/* OSMAPOSL keys (OSMAPOSLReconstruction.cxx:95-121): "MAP_model" (0 additive,
+ * the default; 1 multiplicative) and "minimum/maximum relative change"
+ * (defaults 0 and FLT_MAX) for mi_osem_subset_step / mi_osem_run. */
+int mi_osem_set_options(mi_ctx* ctx, int map_model_multiplicative, float minimum_relative_change,
+                        float maximum_relative_change);
+
+/* Prior of OSMAPOSL's one-step-late update ("prior type" of
+ * GeneralisedObjectiveFunction; QuadraticPrior.cxx, RelativeDifferencePrior.cxx). */
+#define MI_PRIOR_NONE 0
+#define MI_PRIOR_QUADRATIC 1
+#define MI_PRIOR_RELATIVE_DIFFERENCE 2
+typedef struct {
+  int32_t type;               /* MI_PRIOR_* */
+  float penalisation_factor;  /* beta */
+  float gamma;                /* relative difference prior "gamma value" (reference default 2) */
+  float epsilon;              /* relative difference prior "epsilon value" (default 0) */
+  int32_t only_2D;            /* no neighbours in other planes */
+  const float* weights;       /* 27 floats [dz+1][dy+1][dx+1], or NULL for vox_x / distance */
+  const float* kappa;         /* host image (STIR layout) or NULL */
+} mi_prior;
+/* With a prior (type != NONE and beta != 0) mi_osem_subset_step / mi_osem_run
+ * run the one-step-late update of OSMAPOSLReconstruction::update_estimate
+ * (:403-516): u = BP_s(y/ybar); denominator from the prior's gradient g at
+ * the current estimate -- additive: clamp(g/num_subsets + sens_s, sens_s/10,
+ * 10 sens_s); multiplicative: sens_s clamp(1 + g, 0.1, 10) --; divide(u, d,
+ * 1e-6); threshold; x *= u. NULL or type NONE removes the prior. */
+int mi_obj_set_prior(mi_ctx* ctx, const mi_prior* prior);
+
 /* num_subiterations subiterations starting at start_subiteration_num,
  * subset = (k + start_subset - 1) % num_subsets
  * (IterativeReconstruction.cxx:384-428,630-639). */
@@ -228,11 +271,20 @@ int mi_obj_value_and_gradient_dev(mi_ctx* ctx, const float* d_image, int subset,
 int mi_quadratic_prior_dev(mi_ctx* ctx, const float* d_image, const float* d_kappa, const float* weights, float beta,
                            int only_2D, float* d_gradient, double* d_value, void* stream);
 
+/* RelativeDifferencePrior::compute_value and compute_gradient
+ * (RelativeDifferencePrior.cxx:250-433) of a device image, arguments as
+ * mi_quadratic_prior_dev plus gamma and epsilon. */
+int mi_relative_difference_prior_dev(mi_ctx* ctx, const float* d_image, const float* d_kappa, const float* weights,
+                                     float beta, float gamma, float epsilon, int only_2D, float* d_gradient,
+                                     double* d_value, void* stream);
+
 /* One OSMAPOSL subiteration on device images, out of place:
  * d_x_out = d_x_in * u, u = clamp(BP_s(y/(FP_s(x)+b)) / sens_s)
  * (OSMAPOSLReconstruction::update_estimate, OSMAPOSLReconstruction.cxx:373-516,
  * as mi_osem_subset_step); d_update (if not NULL) receives u. Needs
- * mi_obj_set_data and mi_obj_compute_sensitivities. d_x_out may equal d_x_in. */
+ * mi_obj_set_data and mi_obj_compute_sensitivities. d_x_out may equal d_x_in.
+ * Prior-free with the default relative-change bounds (an error when a prior
+ * is set). */
 int mi_osem_subset_step_dev(mi_ctx* ctx, const float* d_x_in, float* d_x_out, float* d_update, int subset,
                             int subiteration_num, void* stream);
 /* No counterpart in the reference (it has no derivatives of the OSEM map):
@@ -269,6 +321,10 @@ int mi_batch_get_estimate(mi_batch* batch, int problem, float* image);
  * gives every column a full wave, tile_x x tile_y orders the columns.
  * Defaults 32, 0, 1, 32, 8. */
 int mi_batch_set_launch(mi_batch* batch, int fp_group_max, int fp_widen, int bp_pack, int tile_x, int tile_y);
+/* Forward projection in row groups: the families of one basic row with at
+ * most max_axial axial positions (0: none) run up to 64 axial positions per
+ * wave with the row's elements loaded once (default 32). */
+int mi_batch_set_row_groups(mi_batch* batch, int max_axial);
 /* Times the candidate launch shapes on subset 0 (no side effects on the
  * estimates) and keeps the fastest; *choice receives its index. */
 int mi_batch_tune(mi_batch* batch, int* choice);

@@ -75,6 +75,14 @@ def lib():
         L.or_quadratic_prior.restype = ctypes.c_double
         L.or_quadratic_prior.argtypes = [_f32p, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int,
                                          ctypes.c_float, vp]
+        L.or_relative_difference_prior.restype = ctypes.c_double
+        L.or_relative_difference_prior.argtypes = [_f32p, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p,
+                                                   ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]
+        L.or_set_multiplicative.argtypes = [vp, vp]
+        L.or_set_use_subset_sensitivities.argtypes = [vp, ctypes.c_int]
+        L.or_set_osem_options.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float]
+        L.or_set_prior.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, vp, vp,
+                                   ctypes.c_int]
         _lib = L
     return _lib
 
@@ -95,6 +103,20 @@ def quadratic_prior(image, vox_zyx, beta, kappa=None, only_2D=False, gradient=Tr
     g = np.zeros(img.shape, np.float32) if gradient else None
     v = lib().or_quadratic_prior(img.reshape(-1), kp[0] if kp else None, nz, ny, nx, w, int(only_2D), float(beta),
                                  g.ctypes.data_as(ctypes.c_void_p) if g is not None else None)
+    return float(v), g
+
+
+def relative_difference_prior(image, vox_zyx, beta, gamma=2.0, epsilon=0.0, kappa=None, only_2D=False,
+                              gradient=True, weights=None):
+    """RelativeDifferencePrior value (float) and gradient (float32 image, or None)."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    nz, ny, nx = img.shape
+    w = (quadratic_prior_weights(vox_zyx) if weights is None else np.asarray(weights, np.float32)).reshape(-1).copy()
+    kp = _opt_ptr(None if kappa is None else np.ascontiguousarray(kappa, np.float32).reshape(img.shape))
+    g = np.zeros(img.shape, np.float32) if gradient else None
+    v = lib().or_relative_difference_prior(img.reshape(-1), kp[0] if kp else None, nz, ny, nx, w, int(only_2D),
+                                           float(beta), float(gamma), float(epsilon),
+                                           g.ctypes.data_as(ctypes.c_void_p) if g is not None else None)
     return float(v), g
 
 
@@ -239,6 +261,28 @@ class OraclePlan:
 
     def set_use_tof_sensitivities(self, use: bool) -> None:
         lib().or_set_use_tof_sensitivities(self._p, int(use))
+
+    def set_multiplicative(self, mult) -> None:
+        """Multiplicative factors (kept alive here) or None."""
+        self._mult = None if mult is None else self._sino(mult)
+        lib().or_set_multiplicative(self._p, None if self._mult is None else self._mult.ctypes.data_as(ctypes.c_void_p))
+
+    def set_use_subset_sensitivities(self, use: bool) -> None:
+        lib().or_set_use_subset_sensitivities(self._p, int(use))
+
+    def set_osem_options(self, map_model="additive", minimum_relative_change=0.0,
+                         maximum_relative_change=float(np.finfo(np.float32).max)) -> None:
+        lib().or_set_osem_options(self._p, int(map_model == "multiplicative"), float(minimum_relative_change),
+                                  float(maximum_relative_change))
+
+    def set_prior(self, kind=None, beta=0.0, gamma=2.0, epsilon=0.0, kappa=None, weights=None, only_2D=False) -> None:
+        """kind: None, "quadratic" or "relative difference"."""
+        t = {None: 0, "quadratic": 1, "relative difference": 2}[kind]
+        self._kappa = None if kappa is None else np.ascontiguousarray(kappa, np.float32).reshape(self.img.shape)
+        self._w27 = None if weights is None else np.ascontiguousarray(weights, np.float32).reshape(27)
+        lib().or_set_prior(self._p, t, float(beta), float(gamma), float(epsilon),
+                           None if self._kappa is None else self._kappa.ctypes.data_as(ctypes.c_void_p),
+                           None if self._w27 is None else self._w27.ctypes.data_as(ctypes.c_void_p), int(only_2D))
 
     def subset_sensitivity(self, subset, num_subsets, zero_end_planes=False):
         out = np.zeros(self.img.shape, np.float32)

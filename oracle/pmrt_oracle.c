@@ -150,6 +150,16 @@ struct or_plan {
   double* erf_tab;  /* FastErf samples */
   int tofsens;      /* "use time-of-flight sensitivities" (default 0) */
   int nt_sym[4];    /* 90, 180, swap segment, swap s before the TOF restriction */
+  /* objective function and OSMAPOSL settings */
+  const float* mult;        /* multiplicative factors per bin (caller-owned), NULL = 1 */
+  int no_subset_sens;       /* "use subset sensitivities" := 0 */
+  int map_multiplicative;   /* "MAP_model" := multiplicative (default additive) */
+  float min_rel, max_rel;   /* "minimum/maximum relative change" */
+  int prior_type;           /* 0 none, 1 quadratic, 2 relative difference */
+  float beta, gamma, eps;   /* penalisation factor; RDP gamma and epsilon */
+  int only_2D;
+  const float* kappa;       /* caller-owned image or NULL */
+  float w27[27];
 };
 
 #define SEGI(p, s) ((s) - (p)->g.min_segment_num)
@@ -646,6 +656,8 @@ or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
 {
   or_plan* p = (or_plan*)calloc(1, sizeof(or_plan));
   p->g = *g;
+  p->min_rel = 0.F;      /* OSMAPOSLReconstruction.cxx:95-104 */
+  p->max_rel = FLT_MAX;
   p->nseg = g->max_segment_num - g->min_segment_num + 1;
   if (p->nseg <= 0 || p->nseg > OR_MAX_SEGMENTS) { snprintf(err, errlen, "bad segment range"); free(p); return NULL; }
   /* ProjDataInfoCylindrical ctor -- ProjDataInfoCylindrical.cxx:56-117 */
@@ -1187,10 +1199,45 @@ static int is_end_plane(const or_plan* p, int seg, int ax)
   return seg == 0 && (ax == 0 || ax == p->g.num_axial_poss[SEGI(p, 0)] - 1);
 }
 
-/* RPC_process_related_viewgrams_sensitivity_computation -- ...AndProjData.cxx:1517-1539,
- * with mult viewgrams = ones and zeroed seg-0 end planes when requested
- * (distributable.cxx:200-223). */
 void or_set_use_tof_sensitivities(or_plan* p, int use) { p->tofsens = use != 0; }
+
+/* Multiplicative factors m of the model ybar = m (A x + b): the normalisation's
+ * undo() applied to ones (distributable.cxx:197-215); NULL = none. The array
+ * (all TOF bins, measured order) stays owned by the caller. */
+void or_set_multiplicative(or_plan* p, const float* mult) { p->mult = mult; }
+/* "use subset sensitivities" (...ForMean.cxx:48,374-455) */
+void or_set_use_subset_sensitivities(or_plan* p, int use) { p->no_subset_sens = use == 0; }
+/* OSMAPOSL keys: "MAP_model" (0 additive, 1 multiplicative), "minimum/maximum
+ * relative change" (OSMAPOSLReconstruction.cxx:95-121) */
+void or_set_osem_options(or_plan* p, int map_multiplicative, float min_rel, float max_rel)
+{
+  p->map_multiplicative = map_multiplicative != 0;
+  p->min_rel = min_rel;
+  p->max_rel = max_rel;
+}
+/* Prior of OSMAPOSL's one-step-late update: type 0 none, 1 QuadraticPrior,
+ * 2 RelativeDifferencePrior; w27 NULL = default weights; kappa caller-owned. */
+void or_set_prior(or_plan* p, int type, float beta, float gamma, float eps, const float* kappa, const float* w27,
+                  int only_2D)
+{
+  p->prior_type = type;
+  p->beta = beta;
+  p->gamma = gamma;
+  p->eps = eps;
+  p->kappa = kappa;
+  p->only_2D = only_2D;
+  if (w27) memcpy(p->w27, w27, sizeof(p->w27));
+  else or_quadratic_prior_weights(p->g.vox_z, p->g.vox_y, p->g.vox_x, p->w27);
+}
+
+/* The multiplicative factor of TOF bin index i: mult viewgrams (1 without a
+ * normalisation) with the seg-0 end planes zeroed on request
+ * (distributable.cxx:197-223). */
+static float mult_at(const or_plan* p, int64_t i, int seg, int ax, int zero_end_planes)
+{
+  if (zero_end_planes && is_end_plane(p, seg, ax)) return 0.F;
+  return p->mult ? p->mult[i] : 1.F;
+}
 
 /* With TOF data and "use time-of-flight sensitivities" off (the default,
  * ...AndProjData.cxx:99,533-536,648-670) the sensitivity is the non-TOF back
@@ -1208,7 +1255,7 @@ static void subset_sensitivity_nontof(const or_plan* p, int subset, int nsub, in
     const int nax = p->g.num_axial_poss[SEGI(p, seg)];
     for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t)
       for (int a = 0; a < nax; ++a)
-        bp_bin(p, img, (zero_end_planes && is_end_plane(p, seg, a)) ? 0.F : 1.F, seg, view, a, t, -1);
+        bp_bin(p, img, mult_at(p, tbin(p, seg, view, a, t, 0), seg, a, zero_end_planes), seg, view, a, t, -1);
   }
   free(vs);
 }
@@ -1219,9 +1266,11 @@ void or_subset_sensitivity(const or_plan* p, int subset, int nsub, int zero_end_
     subset_sensitivity_nontof(p, subset, nsub, zero_end_planes, img);
     return;
   }
+  /* back projection of the mult viewgrams (RPC ...sensitivity_computation,
+   * ...AndProjData.cxx:1517-1539) */
   const int64_t nb = p->num_bins * p->ntof;
   float* ones = (float*)malloc(sizeof(float) * (size_t)nb);
-  for (int64_t i = 0; i < nb; ++i) ones[i] = 1.F;
+  for (int64_t i = 0; i < nb; ++i) ones[i] = p->mult ? p->mult[i] : 1.F;
   if (zero_end_planes) {
     for (int kk = 0; kk < p->ntof; ++kk)
       for (int v = 0; v < p->g.num_views; ++v)
@@ -1297,8 +1346,7 @@ void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, cons
         for (int a = 0; a < nax; ++a)
           for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
             const int64_t i = tbin(p, seg, view, a, t, kk);
-            const float m = (zero_end_planes && is_end_plane(p, seg, a)) ? 0.F : 1.F;
-            q[i] -= m;
+            q[i] -= mult_at(p, i, seg, a, zero_end_planes);
           }
     }
   }
@@ -1336,7 +1384,7 @@ double or_log_likelihood(const or_plan* p, const float* y, const float* bkg, con
         for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
           const int64_t i = tbin(p, seg, view, a, t, kk);
           float est = ybar[i] + (bkg ? (ep ? 0.F : bkg[i]) : 0.F);
-          if (ep) est = est * 0.F;
+          if (p->mult || zero_end_planes) est = est * mult_at(p, i, seg, a, zero_end_planes);
           const float yv = ep ? 0.F : y[i];
           const float ne = maxf_(est, yv / 10000.F);
           if (yv <= small_value) sub += -(double)ne;
@@ -1366,6 +1414,46 @@ static void threshold_min_to_small_positive(float* x, size_t n, float small_numb
   }
 }
 
+/* One-step-late denominator and division of OSMAPOSL with a prior
+ * (OSMAPOSLReconstruction.cxx:403-463): d = grad/num_subsets + sens clamped to
+ * [sens/10, sens*10] ("additive" MAP model) or sens * clamp(1 + grad, 0.1, 10)
+ * ("multiplicative"), grad the prior's gradient at the current estimate;
+ * then divide(u, d, 1e-6) (divide.inl:26-47). */
+static void osl_divide(const or_plan* p, const float* x, const float* sens, int nsub, float* u, size_t nv)
+{
+  float* grad = (float*)malloc(sizeof(float) * nv);
+  if (p->prior_type == 1)
+    or_quadratic_prior(x, p->kappa, p->g.nz, p->g.ny, p->g.nx, p->w27, p->only_2D, p->beta, grad);
+  else
+    or_relative_difference_prior(x, p->kappa, p->g.nz, p->g.ny, p->g.nx, p->w27, p->only_2D, p->beta, p->gamma,
+                                 p->eps, grad);
+  for (size_t i = 0; i < nv; ++i) {
+    float d = grad[i];
+    /* std::max(std::min(d, hi), lo) */
+    if (!p->map_multiplicative) {
+      d = d / nsub + sens[i];
+      const float hi = sens[i] * 10, lo = sens[i] / 10;
+      d = (hi < d) ? hi : d;
+      d = (d < lo) ? lo : d;
+    } else {
+      d += 1;
+      d = (10.F < d) ? 10.F : d;
+      d = (d < 1 / 10.F) ? 1 / 10.F : d;
+      d *= sens[i];
+    }
+    grad[i] = d;
+  }
+  float mx = -FLT_MAX;
+  for (size_t i = 0; i < nv; ++i) if (u[i] > mx) mx = u[i];
+  float small_value = mx * 0.000001F;
+  small_value = small_value > 0 ? small_value : 0;
+  for (size_t i = 0; i < nv; ++i) {
+    if (fabsf(grad[i]) <= small_value && fabsf(u[i]) <= small_value) u[i] = 0;
+    else u[i] /= grad[i];
+  }
+  free(grad);
+}
+
 void or_osem(const or_plan* p, const float* y, const float* bkg, float* x, int nsub, int nsubiter,
              int start_subset, int zero_end_planes, int use_omp)
 {
@@ -1375,6 +1463,15 @@ void or_osem(const or_plan* p, const float* y, const float* bkg, float* x, int n
     sens[s] = (float*)malloc(sizeof(float) * nv);
     or_subset_sensitivity(p, s, nsub, zero_end_planes, sens[s]);
   }
+  if (p->no_subset_sens) {
+    /* "use subset sensitivities" := 0 (...ForMean.cxx:374-455): the total,
+     * accumulated subset by subset in float, divided by num_subsets */
+    for (int s = 1; s < nsub; ++s)
+      for (size_t i = 0; i < nv; ++i) sens[0][i] += sens[s][i];
+    for (size_t i = 0; i < nv; ++i) sens[0][i] /= nsub;
+    for (int s = 1; s < nsub; ++s) memcpy(sens[s], sens[0], sizeof(float) * nv);
+  }
+  const int with_prior = p->prior_type != 0 && p->beta != 0;
   float* u = (float*)malloc(sizeof(float) * nv);
   threshold_min_to_small_positive(x, nv, 0.000001F);
   for (int k = 1; k <= nsubiter; ++k) {
@@ -1399,8 +1496,10 @@ void or_osem(const or_plan* p, const float* y, const float* bkg, float* x, int n
     } else {
       or_subset_gradient(p, y, bkg, x, subset, nsub, zero_end_planes, 1, u);
     }
-    /* divide(update, subsens, 0.F) -- numerics/divide.inl:26-47 */
-    {
+    if (with_prior) {
+      osl_divide(p, x, sens[subset], nsub, u, nv);
+    } else {
+      /* divide(update, subsens, 0.F) -- numerics/divide.inl:26-47 */
       float mx = -FLT_MAX;
       for (size_t i = 0; i < nv; ++i) if (u[i] > mx) mx = u[i];
       float small_value = mx * 0.F;
@@ -1410,11 +1509,12 @@ void or_osem(const or_plan* p, const float* y, const float* bkg, float* x, int n
         else u[i] /= sens[subset][i];
       }
     }
-    /* threshold_upper_lower(0, FLT_MAX) except subiteration 1 (OSMAPOSLReconstruction.cxx:491-507) */
+    /* threshold_upper_lower(min_rel, max_rel) except subiteration 1
+     * (OSMAPOSLReconstruction.cxx:491-507, thresholding.h:45-54) */
     if (k != 1)
       for (size_t i = 0; i < nv; ++i) {
-        if (u[i] > FLT_MAX) u[i] = FLT_MAX;
-        else if (0.F > u[i]) u[i] = 0.F;
+        if (u[i] > p->max_rel) u[i] = p->max_rel;
+        else if (p->min_rel > u[i]) u[i] = p->min_rel;
       }
     for (size_t i = 0; i < nv; ++i) x[i] *= u[i];
   }
@@ -1473,6 +1573,72 @@ double or_quadratic_prior(const float* img, const float* kappa, int nz, int ny, 
               const float d = img[v] - img[n];
               double cv = (double)(w * (d * d) / 4);
               double cg = (double)(w * d);
+              if (kappa) {
+                const float kk = kappa[v] * kappa[n];
+                cv *= kk;
+                cg *= kk;
+              }
+              result += cv;
+              g += cg;
+            }
+        if (grad) grad[v] = (float)(g * beta);
+      }
+    }
+  }
+  return result * beta;
+}
+
+/* ------------------------------------------------------------------ */
+/* RelativeDifferencePrior                                              */
+/* ------------------------------------------------------------------ */
+/* compute_value (RelativeDifferencePrior.cxx:303-373) and compute_gradient
+ * (:375-433) over an image [nz][ny][nx] with the weights of compute_weights
+ * (:250-278, the same as the quadratic prior's); neighbours outside the image
+ * are skipped; kappa may be NULL.
+ *   value(x, y) = 0.5 * (double)(x-y)^2 / (float)(x + y + gamma|x-y| + eps)  (:280-284)
+ *   derivative_10(x, y) = (float)((double)(x-y) * (float)(gamma|x-y| + x + 3y + 2eps)
+ *                         / ((double)(x+y) + gamma|x-y| + eps)^2), 0 at x = y = 0, eps = 0  (:286-300)
+ * Terms are summed in double in (dz, dy, dx) order per voxel; the value's
+ * terms at x = y = 0 (eps = 0) are 0. Returns beta * value; grad (if not NULL)
+ * = (float)(beta * gradient). */
+double or_relative_difference_prior(const float* img, const float* kappa, int nz, int ny, int nx, const float* w27,
+                                    int only_2D, float beta, float gamma, float eps, float* grad)
+{
+  double result = 0.;
+  const int wz0 = only_2D ? 0 : -1, wz1 = only_2D ? 0 : 1;
+  for (int z = 0; z < nz; ++z) {
+    const int min_dz = wz0 > -z ? wz0 : -z, max_dz = wz1 < nz - 1 - z ? wz1 : nz - 1 - z;
+    for (int y = 0; y < ny; ++y) {
+      const int min_dy = -1 > -y ? -1 : -y, max_dy = 1 < ny - 1 - y ? 1 : ny - 1 - y;
+      for (int x = 0; x < nx; ++x) {
+        const int min_dx = -1 > -x ? -1 : -x, max_dx = 1 < nx - 1 - x ? 1 : nx - 1 - x;
+        const int64_t v = ((int64_t)z * ny + y) * nx + x;
+        double g = 0.;
+        for (int dz = min_dz; dz <= max_dz; ++dz)
+          for (int dy = min_dy; dy <= max_dy; ++dy)
+            for (int dx = min_dx; dx <= max_dx; ++dx) {
+              const int64_t n = ((int64_t)(z + dz) * ny + (y + dy)) * nx + (x + dx);
+              const float w = w27[((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)];
+              const float a = img[v], b = img[n];
+              const float ad = fabsf(a - b);
+              double cv;
+              if (eps == 0.F && a == 0.F && b == 0.F) {
+                cv = 0.0;
+              } else {
+                const float den = a + b + gamma * ad + eps;
+                const double dd = (double)(a - b);
+                cv = (double)w * (0.5 * (dd * dd / den));
+              }
+              float der;
+              if (eps == 0.F && a == 0.F && b == 0.F) {
+                der = 0.F;
+              } else {
+                const float t = gamma * ad + a + 3 * b + 2 * eps;
+                const double num = (double)(a - b) * t;
+                const double ds = (double)(a + b) + gamma * ad + eps;
+                der = (float)(num / (ds * ds));
+              }
+              double cg = (double)(w * der);
               if (kappa) {
                 const float kk = kappa[v] * kappa[n];
                 cv *= kk;

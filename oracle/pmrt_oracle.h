@@ -162,8 +162,22 @@ void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, cons
                         float* grad);
 double or_log_likelihood(const or_plan* p, const float* y, const float* bkg, const float* x,
                          int subset, int num_subsets, int zero_end_planes);
-/* OSMAPOSL (OSMAPOSLReconstruction.cxx:266-516, IterativeReconstruction.cxx:384-428,630-639),
- * prior-free, subset sensitivities, initial positivity enforced on x. */
+/* Multiplicative factors m of the model ybar = m (A x + b) (normalisation
+ * undo() of ones, distributable.cxx:197-215), all TOF bins in measured order,
+ * caller-owned; NULL removes them. Used by the sensitivities (BP of m), the
+ * gradient (BP(y/(Ax+b) - m)) and the log-likelihood (ybar = (Ax+b) m). */
+void or_set_multiplicative(or_plan* p, const float* mult);
+/* "use subset sensitivities" (default 1); 0: every subset uses total / num_subsets. */
+void or_set_use_subset_sensitivities(or_plan* p, int use);
+/* OSMAPOSL "MAP_model" (0 additive, 1 multiplicative) and "minimum/maximum
+ * relative change" (defaults 0, FLT_MAX). */
+void or_set_osem_options(or_plan* p, int map_multiplicative, float min_rel, float max_rel);
+/* Prior of the one-step-late update: type 0 none, 1 quadratic, 2 relative
+ * difference (gamma, eps); w27 NULL = default weights; kappa caller-owned or NULL. */
+void or_set_prior(or_plan* p, int type, float beta, float gamma, float eps, const float* kappa, const float* w27,
+                  int only_2D);
+/* OSMAPOSL (OSMAPOSLReconstruction.cxx:266-516, IterativeReconstruction.cxx:384-428,630-639)
+ * with the settings above, initial positivity enforced on x. */
 void or_osem(const or_plan* p, const float* y, const float* bkg, float* x,
              int num_subsets, int num_subiterations, int start_subset, int zero_end_planes,
              int use_omp);
@@ -175,6 +189,11 @@ void or_osem(const or_plan* p, const float* y, const float* bkg, float* x,
 void or_quadratic_prior_weights(float vox_z, float vox_y, float vox_x, float* w27);
 double or_quadratic_prior(const float* img, const float* kappa, int nz, int ny, int nx, const float* w27,
                           int only_2D, float beta, float* grad);
+/* RelativeDifferencePrior (RelativeDifferencePrior.cxx:250-433): value =
+ * beta * sum w 0.5 (x_v - x_n)^2 / (x_v + x_n + gamma |x_v - x_n| + eps)
+ * (returned), grad_v = beta * sum_n w derivative_10(x_v, x_n). */
+double or_relative_difference_prior(const float* img, const float* kappa, int nz, int ny, int nx, const float* w27,
+                                    int only_2D, float beta, float gamma, float eps, float* grad);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,24 @@ EXPORTED_SYMBOLS = (
     "mi_osem_subset_step_dev", "mi_osem_subset_step_vjp_dev", "mi_obj_set_use_tof_sensitivities",
     "mi_batch_create", "mi_batch_destroy", "mi_batch_context", "mi_batch_set_data", "mi_batch_set_estimate",
     "mi_batch_get_estimate", "mi_batch_set_launch", "mi_batch_tune", "mi_batch_osem_run", "mi_batch_synchronize",
+    "mi_obj_set_multiplicative", "mi_obj_set_use_subset_sensitivities", "mi_osem_set_options", "mi_obj_set_prior",
+    "mi_relative_difference_prior_dev", "mi_batch_set_row_groups",
 )
+
+MI_PRIOR_NONE, MI_PRIOR_QUADRATIC, MI_PRIOR_RELATIVE_DIFFERENCE = 0, 1, 2
+
+
+class PriorStruct(ctypes.Structure):
+    """mi_prior"""
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("penalisation_factor", ctypes.c_float),
+        ("gamma", ctypes.c_float),
+        ("epsilon", ctypes.c_float),
+        ("only_2D", ctypes.c_int32),
+        ("weights", ctypes.c_void_p),
+        ("kappa", ctypes.c_void_p),
+    ]
 
 MI_TUNE_BP_TILE_X, MI_TUNE_BP_TILE_Y, MI_TUNE_BP_XCD_REMAP, MI_TUNE_BP_SLOT_SKIP = 1, 2, 3, 4
 
@@ -115,6 +132,13 @@ def load():
     L.mi_batch_tune.argtypes = [vp, ctypes.POINTER(c_int)]
     L.mi_batch_osem_run.argtypes = [vp, c_int, c_int, c_int, c_int]
     L.mi_batch_synchronize.argtypes = [vp]
+    L.mi_batch_set_row_groups.argtypes = [vp, c_int]
+    L.mi_obj_set_multiplicative.argtypes = [vp, vp]
+    L.mi_obj_set_use_subset_sensitivities.argtypes = [vp, c_int]
+    L.mi_osem_set_options.argtypes = [vp, c_int, ctypes.c_float, ctypes.c_float]
+    L.mi_obj_set_prior.argtypes = [vp, ctypes.POINTER(PriorStruct)]
+    L.mi_relative_difference_prior_dev.argtypes = [vp, vp, vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                                   c_int, vp, vp, vp]
     for name in EXPORTED_SYMBOLS:
         fn = getattr(L, name)
         if name not in ("mi_last_error", "mi_ctx_destroy", "mi_batch_destroy"):

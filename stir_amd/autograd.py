@@ -151,21 +151,17 @@ class PoissonLogLikelihoodLoss:
         return _NegLogLikelihood.apply(x, self.objective, self.subset, want_grad)
 
 
-class _QuadraticPrior(torch.autograd.Function):
+class _NeighbourhoodPrior(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x: torch.Tensor, prior: "QuadraticPriorLoss", want_grad: bool):
+    def forward(ctx, x: torch.Tensor, prior, want_grad: bool):
         pair = prior.pair
         nvox = pair.img.num_voxels
         xs = _as_batch(x, nvox, "image")
         values = torch.empty(xs.shape[0], dtype=torch.float64, device=x.device)
         grads = torch.empty_like(xs) if want_grad else None
-        L, st = _native.load(), _stream(x)
-        w = prior._weights_arg()
+        st = _stream(x)
         for b in range(xs.shape[0]):
-            _native.check(L.mi_quadratic_prior_dev(pair.context.handle, _ptr(xs[b]), _ptr(prior.kappa), w,
-                                                   ctypes.c_float(prior.beta), int(prior.only_2D),
-                                                   _ptr(grads[b]) if want_grad else None, _ptr(values[b:b + 1]), st),
-                          "mi_quadratic_prior_dev")
+            prior._native(xs[b], grads[b] if want_grad else None, values[b:b + 1], st)
         ctx.in_shape = x.shape
         if want_grad:
             ctx.save_for_backward(grads)
@@ -199,9 +195,35 @@ class QuadraticPriorLoss:
     def _weights_arg(self):
         return None if self._w is None else self._w.ctypes.data_as(ctypes.c_void_p)
 
+    def _native(self, x, grad, value, st):
+        _native.check(_native.load().mi_quadratic_prior_dev(self.pair.context.handle, _ptr(x), _ptr(self.kappa),
+                                                            self._weights_arg(), ctypes.c_float(self.beta),
+                                                            int(self.only_2D), _ptr(grad), _ptr(value), st),
+                      "mi_quadratic_prior_dev")
+
     def __call__(self, x: torch.Tensor) -> torch.Tensor:
         want_grad = torch.is_grad_enabled() and x.requires_grad
-        return _QuadraticPrior.apply(x, self, want_grad)
+        return _NeighbourhoodPrior.apply(x, self, want_grad)
+
+
+class RelativeDifferencePriorLoss(QuadraticPriorLoss):
+    """``RelativeDifferencePrior``: beta * sum_v sum_n w_vn k_v k_n 0.5 (x_v - x_n)^2 /
+    (x_v + x_n + gamma |x_v - x_n| + epsilon) (``RelativeDifferencePrior.cxx:280-373``),
+    gradient beta * sum_n w_vn k_v k_n derivative_10(x_v, x_n) (``:286-300,375-433``);
+    reference defaults gamma 2, epsilon 0 (``:140-147``)."""
+
+    def __init__(self, pair: ProjectorByBinPairMI355X, beta: float, gamma: float = 2.0, epsilon: float = 0.0,
+                 kappa: Optional[torch.Tensor] = None, only_2D: bool = False,
+                 weights: Optional[Sequence[float]] = None):
+        super().__init__(pair, beta, kappa, only_2D, weights)
+        self.gamma = float(gamma)
+        self.epsilon = float(epsilon)
+
+    def _native(self, x, grad, value, st):
+        _native.check(_native.load().mi_relative_difference_prior_dev(
+            self.pair.context.handle, _ptr(x), _ptr(self.kappa), self._weights_arg(), ctypes.c_float(self.beta),
+            ctypes.c_float(self.gamma), ctypes.c_float(self.epsilon), int(self.only_2D), _ptr(grad), _ptr(value), st),
+            "mi_relative_difference_prior_dev")
 
 
 # ---------------------------------------------------------------------------

@@ -163,17 +163,92 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_group_kernel(const P
     as_global(P.bp_out)[idx] = zl < d.nz ? acc : 0.F;
   } else {
     if (zl < d.nz) {
-      // divide(update, sens, 0) (numerics/divide.inl:26-47), threshold, multiply
-      float u = acc;
-      const float sv = as_const(as_const(P.sens)[subset])[idx];
-      if (fabsf(sv) <= 0.F && fabsf(u) <= 0.F) u = 0;
-      else u = u / sv;
-      if (threshold) {
-        if (u > FLT_MAX) u = FLT_MAX;
-        else if (0.F > u) u = 0.F;
-      }
+      const float u = osem_factor(acc, as_const(as_const(P.sens)[subset])[idx], threshold, 0.F, FLT_MAX);
       as_global(P.estimate)[idx] = as_global(P.estimate)[idx] * u;
     }
+  }
+}
+
+// Row groups: the families of one basic row (its symmetry-related
+// (segment, view, tangential) positions, all in the same subset) share every
+// row element; a wave runs up to 64 axial positions of several such families,
+// lanes laid out family after family. Row elements are wave-uniform (scalar
+// loads) and each lane applies its own family's in-plane operation, so a
+// short family no longer leaves most of a wave idle. Each bin sums its row in
+// row order, as fp_kernel does, so results are identical to it.
+template <int MODE>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_rowgroup_kernel(const ProblemDesc* __restrict__ descs,
+                                                                            const RowGroupItem* __restrict__ items,
+                                                                            const int32_t* __restrict__ item_fams,
+                                                                            int nitems) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int wi = static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
+  if (wi >= nitems) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  const RowGroupItem it = items[wi];
+  const int p = __builtin_amdgcn_readfirstlane(it.problem);
+  const ProblemDesc& P = descs[p];
+  const ImageDims d = P.d;
+  cptr<Family> fams = as_const(P.fams);
+  cptr<int32_t> list = as_const(item_fams) + it.first;
+  // this lane's family and axial position
+  int ca = 0, cb = 0, cc = 0, ce = 0, zs = 1, zb = 0, vg = 0;
+  int64_t out = 0;
+  bool valid = false;
+  int off = 0, row_start = 0, row_len = 0;
+  for (int k = 0; k < it.count; ++k) {
+    const Family f = fams[list[k]];
+    if (k == 0) {
+      row_start = f.row_start;
+      row_len = f.row_len;
+    }
+    const int a = lane - off;
+    if (a >= 0 && a < f.nax) {
+      inplane_coeffs(f.inplane, ca, cb, cc, ce);
+      zs = f.zsign;
+      zb = f.zadd0 + f.ppa * a - d.min_z;
+      out = static_cast<int64_t>(f.sino_base) + a;
+      vg = f.viewgram;
+      valid = true;
+    }
+    off += f.nax;
+  }
+  cptr<int2> row = as_const(P.pool) + row_start;
+  cptr<float> img = as_const(P.fp_in);
+  float acc = 0.F;
+  int e = 0;
+  for (; e + kGroupUnroll <= row_len; e += kGroupUnroll) {
+    float v[kGroupUnroll], w[kGroupUnroll];
+    bool ok[kGroupUnroll];
+#pragma unroll
+    for (int j = 0; j < kGroupUnroll; ++j) {
+      const int2 el = row[e + j];
+      int x, y, z;
+      decode(el.x, x, y, z);
+      const int xo = ca * x + cb * y, yo = cc * x + ce * y;
+      const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
+      const int zz = zs * z + zb;
+      ok[j] = valid && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
+      w[j] = __int_as_float(el.y);
+      v[j] = img[static_cast<int64_t>(valid ? col : 0) * d.nzp + min(max(zz, 0), d.nz - 1)];
+    }
+#pragma unroll
+    for (int j = 0; j < kGroupUnroll; ++j)
+      if (ok[j]) acc = acc + v[j] * w[j];
+  }
+  for (; e < row_len; ++e) {
+    const int2 el = row[e];
+    int x, y, z;
+    decode(el.x, x, y, z);
+    const int xo = ca * x + cb * y, yo = cc * x + ce * y;
+    const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
+    const int zz = zs * z + zb;
+    if (valid && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz))
+      acc = acc + img[static_cast<int64_t>(col) * d.nzp + zz] * __int_as_float(el.y);
+  }
+  if (valid) {
+    double ll = 0.0;
+    fp_epilogue<MODE>(acc, out, vg, P.fd, P.fp_out, ll);
   }
 }
 
@@ -200,6 +275,16 @@ void launch_fp_group(FpMode mode, int G, const ProblemDesc* d_descs, const WorkI
   else { MI_FPG_G(FP_PLAIN) }
 #undef MI_FPG_G
 #undef MI_FPG
+}
+
+void launch_fp_rowgroup(FpMode mode, const ProblemDesc* d_descs, const RowGroupItem* d_items, const int32_t* d_item_fams,
+                        int nitems, hipStream_t stream) {
+  if (nitems <= 0) return;
+  const dim3 grid((nitems + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  if (mode == FP_RATIO)
+    hipLaunchKernelGGL(fp_rowgroup_kernel<FP_RATIO>, grid, block, 0, stream, d_descs, d_items, d_item_fams, nitems);
+  else
+    hipLaunchKernelGGL(fp_rowgroup_kernel<FP_PLAIN>, grid, block, 0, stream, d_descs, d_items, d_item_fams, nitems);
 }
 
 void launch_bp_group(BpMode mode, int G, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork, int subset,

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -146,6 +147,30 @@ struct mi_ctx {
   bool have_data = false, have_sens = false, have_x = false;
   float add_const = 0.F;
   int zero_end_planes = 0;
+  // multiplicative factors given by the caller (device layout), and whether
+  // every TOF bin carries the same factors
+  DBuf<float> mult_user;
+  bool mult_tof_uniform = true;
+  bool use_subset_sens = true;  // "use subset sensitivities"
+  // OSMAPOSL: "MAP_model", "minimum/maximum relative change", the prior of the
+  // one-step-late update (kind 0 none, 1 quadratic, 2 relative difference)
+  int map_multiplicative = 0;
+  float min_rel = 0.F, max_rel = FLT_MAX;
+  int prior_kind = 0;
+  float prior_beta = 0.F, prior_gamma = 2.F, prior_eps = 0.F;
+  int prior_only_2D = 0;
+  mi::PriorWeights prior_w{};
+  DBuf<float> prior_kappa, prior_grad, small_part, small_val;
+  bool prior_active() const { return prior_kind != 0 && prior_beta != 0.F; }
+  mi::PriorImage device_prior_image() const {
+    const mi_geometry& g = plan->geometry();
+    return mi::PriorImage{g.nz, g.ny, g.nx, dims.nzp, 1};
+  }
+  mi::PriorImage stir_prior_image() const {
+    const mi_geometry& g = plan->geometry();
+    return mi::PriorImage{g.nz, g.ny, g.nx, g.nz, 0};
+  }
+  void rebuild_mult(hipStream_t st);
 
   void require_setup() const {
     if (!plan) throw ApiError(MI_ERR_STATE, "mi_set_up has not been called");
@@ -540,7 +565,7 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   ratio.alloc(total_bins());
   stage_img.alloc(plan->num_voxels());
   stage_sino.alloc(total_bins());
-  prior_partial.alloc(mi::prior_num_partials(g.nz, g.ny, g.nx));
+  prior_partial.alloc(std::max(mi::prior_num_partials(stir_prior_image()), mi::prior_num_partials(device_prior_image())));
   // viewgram extents in the device layout
   {
     // TOF viewgram (segment, view, TOF bin k) has index k * num_viewgrams + viewgram_index
@@ -768,6 +793,35 @@ int mi_back_project(mi_ctx* ctx, const float* proj_data, float* image, int subse
   });
 }
 
+}  // extern "C"
+
+// The multiplicative viewgrams of the model (distributable.cxx:197-223): the
+// caller's factors, or ones when only "zero end planes of segment 0" asks for
+// them, with the end sinograms of segment 0 zeroed on request; none otherwise.
+void mi_ctx::rebuild_mult(hipStream_t st) {
+  const int64_t nb = total_bins();
+  if (!mult_user.p && !zero_end_planes) {
+    mult.release();
+    return;
+  }
+  mult.alloc(nb);
+  if (mult_user.p) HIPCHECK(hipMemcpyAsync(mult.p, mult_user.p, nb * sizeof(float), hipMemcpyDeviceToDevice, st));
+  else mi::launch_fill(mult.p, nb, 1.F, st);
+  if (zero_end_planes) {
+    const mi_geometry& g = plan->geometry();
+    DBuf<int64_t> off;
+    off.alloc(ntof);
+    std::vector<int64_t> o(ntof);
+    for (int k = 0; k < ntof; ++k) o[k] = plan->segment_offset(0) + k * block_bins;
+    HIPCHECK(hipMemcpyAsync(off.p, o.data(), o.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    const int runs = g.num_views * g.num_tangential_poss, run = plan->num_axial(0);
+    for (int k = 0; k < ntof; ++k) mi::launch_zero_end_planes(mult.p, off.p + k, runs, run, 0, st);
+  }
+  HIPCHECK(hipStreamSynchronize(st));
+}
+
+extern "C" {
+
 int mi_obj_set_data(mi_ctx* ctx, const float* measured, const float* additive, float additive_constant,
                     int zero_end_planes) {
   return guarded([&] {
@@ -792,25 +846,22 @@ int mi_obj_set_data(mi_ctx* ctx, const float* measured, const float* additive, f
     ctx->ones.alloc(nb);
     mi::launch_fill(ctx->ones.p, nb, 1.F, st);
     if (zero_end_planes) {
-      // "zero end planes of segment 0" (distributable.cxx:218-223): y, b and the
-      // multiplicative viewgrams get their first and last sinogram zeroed
+      // "zero end planes of segment 0" (distributable.cxx:218-223): y and b get
+      // their first and last sinogram zeroed (the multiplicative viewgrams too,
+      // in rebuild_mult)
       DBuf<int64_t> off;
       off.alloc(ctx->ntof);
       std::vector<int64_t> o(ctx->ntof);
       for (int k = 0; k < ctx->ntof; ++k) o[k] = ctx->plan->segment_offset(0) + k * ctx->block_bins;
       HIPCHECK(hipMemcpyAsync(off.p, o.data(), o.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
       const int runs = g.num_views * g.num_tangential_poss, run = ctx->plan->num_axial(0);
-      ctx->mult.alloc(nb);
-      mi::launch_fill(ctx->mult.p, nb, 1.F, st);
       for (int k = 0; k < ctx->ntof; ++k) {
         mi::launch_zero_end_planes(ctx->y.p, off.p + k, runs, run, 0, st);
         if (ctx->add.p) mi::launch_zero_end_planes(ctx->add.p, off.p + k, runs, run, 0, st);
-        mi::launch_zero_end_planes(ctx->mult.p, off.p + k, runs, run, 0, st);
       }
       HIPCHECK(hipStreamSynchronize(st));
-    } else {
-      ctx->mult.release();
     }
+    ctx->rebuild_mult(st);
     const int nvg = ctx->plan->num_viewgrams() * ctx->ntof;
     ctx->vg_small.alloc(nvg);
     mi::launch_viewgram_max(ctx->y.p, ctx->vg_off.p, ctx->vg_len.p, nvg, ctx->vg_small.p, st);
@@ -830,6 +881,12 @@ int mi_obj_compute_sensitivities(mi_ctx* ctx) {
     const int nsub = ctx->plan->num_subsets();
     ctx->sens.clear();
     const float* in = ctx->mult.p ? ctx->mult.p : ctx->ones.p;
+    if (!ctx->use_subset_sens && !ctx->plan->subsets_balanced())
+      // compute_sensitivities (...ForMean.cxx:377-384)
+      throw ApiError(MI_ERR_STATE, "subsets are not balanced: use subset sensitivities");
+    if (ctx->ntof > 1 && !ctx->tof_sensitivities && ctx->mult_user.p && !ctx->mult_tof_uniform)
+      // the non-TOF sensitivity needs a non-TOF normalisation (...AndProjData.cxx:636-646)
+      throw ApiError(MI_ERR_STATE, "multiplicative factors differ between TOF bins: use TOF sensitivities");
     if (ctx->ntof > 1 && !ctx->tof_sensitivities) {
       // TOF data: the non-TOF sensitivity (...AndProjData.cxx:533-536,648-670),
       // computed by the non-TOF projector, whose symmetries decide the
@@ -865,6 +922,15 @@ int mi_obj_compute_sensitivities(mi_ctx* ctx) {
         ctx->bp(mi::BP_PLAIN, in, b->p, s, mi::BpData{}, ctx->stream);
         ctx->sens.push_back(std::move(b));
       }
+    }
+    if (!ctx->use_subset_sens) {
+      // "use subset sensitivities" := 0 (...ForMean.cxx:386-455): the total,
+      // accumulated subset by subset, divided by num_subsets for every subset
+      for (int s = 1; s < nsub; ++s) mi::launch_accumulate(ctx->sens[0]->p, ctx->sens[s]->p, ctx->img.n, ctx->stream);
+      mi::launch_divide_scalar(ctx->sens[0]->p, static_cast<float>(nsub), ctx->img.n, ctx->stream);
+      for (int s = 1; s < nsub; ++s)
+        HIPCHECK(hipMemcpyAsync(ctx->sens[s]->p, ctx->sens[0]->p, ctx->sens[0]->bytes(), hipMemcpyDeviceToDevice,
+                                ctx->stream));
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -931,11 +997,27 @@ int mi_obj_get_estimate(mi_ctx* ctx, float* image) {
 
 static void osem_step(mi_ctx* ctx, int subset, int subiteration_num) {
   ctx->fp(mi::FP_RATIO, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
-  mi::BpData bd{};
-  bd.sens = ctx->sens[subset]->p;
-  bd.estimate = ctx->x.p;
-  bd.threshold = subiteration_num != 1;
-  ctx->bp(mi::BP_OSEM_UPDATE, ctx->ratio.p, nullptr, subset, bd, ctx->stream);
+  if (!ctx->prior_active()) {
+    // prior-free: divide, threshold and multiply fused into the back projection
+    mi::BpData bd{};
+    bd.sens = ctx->sens[subset]->p;
+    bd.estimate = ctx->x.p;
+    bd.threshold = subiteration_num != 1;
+    bd.thr_lo = ctx->min_rel;
+    bd.thr_hi = ctx->max_rel;
+    ctx->bp(mi::BP_OSEM_UPDATE, ctx->ratio.p, nullptr, subset, bd, ctx->stream);
+    return;
+  }
+  // one-step-late MAP update (OSMAPOSLReconstruction.cxx:403-516): u = BP(y/ybar),
+  // the prior's gradient at the current estimate, divide(u, denominator, 1e-6)
+  ctx->bp(mi::BP_PLAIN, ctx->ratio.p, ctx->img2.p, subset, mi::BpData{}, ctx->stream);
+  mi::launch_prior(ctx->prior_kind, ctx->x.p, ctx->prior_kappa.p, ctx->device_prior_image(), ctx->prior_w,
+                   ctx->prior_only_2D, ctx->prior_beta, ctx->prior_gamma, ctx->prior_eps, ctx->prior_grad.p,
+                   ctx->prior_partial.p, ctx->stream);
+  mi::launch_small_value(ctx->img2.p, ctx->img2.n, 0.000001F, ctx->small_part.p, ctx->small_val.p, ctx->stream);
+  mi::launch_osl_update(ctx->x.p, ctx->img2.p, ctx->prior_grad.p, ctx->sens[subset]->p, ctx->small_val.p,
+                        ctx->plan->num_subsets(), ctx->map_multiplicative, subiteration_num != 1, ctx->min_rel,
+                        ctx->max_rel, ctx->dims.nz, ctx->dims.nzp, ctx->img.n, ctx->stream);
 }
 
 int mi_osem_subset_step(mi_ctx* ctx, int subset, int subiteration_num) {
@@ -1044,10 +1126,10 @@ int mi_quadratic_prior_dev(mi_ctx* ctx, const float* d_image, const float* d_kap
       mi::PriorWeights W;
       if (weights) std::memcpy(W.w, weights, sizeof(W.w));
       else default_prior_weights(g, W.w);
-      mi::launch_quadratic_prior(d_image, d_kappa, g.nz, g.ny, g.nx, W, only_2D, beta, d_gradient, ctx->prior_partial.p,
-                                 st);
+      mi::launch_prior(1, d_image, d_kappa, ctx->stir_prior_image(), W, only_2D, beta, 0.F, 0.F, d_gradient,
+                       ctx->prior_partial.p, st);
       if (d_value)
-        mi::launch_sum_partials(ctx->prior_partial.p, mi::prior_num_partials(g.nz, g.ny, g.nx),
+        mi::launch_sum_partials(ctx->prior_partial.p, mi::prior_num_partials(ctx->stir_prior_image()),
                                 static_cast<double>(beta), d_value, st);
     }
     HIPCHECK(hipGetLastError());
@@ -1058,6 +1140,7 @@ int mi_osem_subset_step_dev(mi_ctx* ctx, const float* d_x_in, float* d_x_out, fl
                             int subiteration_num, void* stream) {
   return guarded([&] {
     if (!ctx || !d_x_in || !d_x_out) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (ctx->prior_active()) throw ApiError(MI_ERR_STATE, "the out-of-place OSEM step is prior-free");
     ctx->require_setup();
     if (!ctx->have_data || !ctx->have_sens) throw ApiError(MI_ERR_STATE, "data and sensitivities must be set first");
     if (subset < 0 || subset >= ctx->plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
@@ -1110,6 +1193,115 @@ int mi_obj_set_use_tof_sensitivities(mi_ctx* ctx, int use) {
     if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     ctx->tof_sensitivities = use != 0;
     ctx->have_sens = false;
+  });
+}
+
+int mi_obj_set_multiplicative(mi_ctx* ctx, const float* mult) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    if (!mult) {
+      ctx->mult_user.release();
+      ctx->mult_tof_uniform = true;
+    } else {
+      const int64_t nb = ctx->total_bins();
+      bool uniform = true;
+      for (int k = 1; k < ctx->ntof && uniform; ++k)
+        uniform = std::memcmp(mult, mult + k * ctx->block_bins, ctx->block_bins * sizeof(float)) == 0;
+      ctx->mult_tof_uniform = uniform;
+      ctx->mult_user.alloc(nb);
+      HIPCHECK(hipMemcpyAsync(ctx->stage_sino.p, mult, nb * sizeof(float), hipMemcpyHostToDevice, st));
+      ctx->proj_to_device(ctx->stage_sino.p, ctx->mult_user.p, st);
+    }
+    if (ctx->have_data) ctx->rebuild_mult(st);
+    HIPCHECK(hipStreamSynchronize(st));
+    ctx->have_sens = false;
+  });
+}
+
+int mi_obj_set_use_subset_sensitivities(mi_ctx* ctx, int use) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->use_subset_sens = use != 0;
+    ctx->have_sens = false;
+  });
+}
+
+int mi_osem_set_options(mi_ctx* ctx, int map_model_multiplicative, float minimum_relative_change,
+                        float maximum_relative_change) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->map_multiplicative = map_model_multiplicative != 0;
+    ctx->min_rel = minimum_relative_change;
+    ctx->max_rel = maximum_relative_change;
+  });
+}
+
+int mi_obj_set_prior(mi_ctx* ctx, const mi_prior* prior) {
+  return guarded([&] {
+    if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    if (!prior || prior->type == MI_PRIOR_NONE) {
+      ctx->prior_kind = 0;
+      ctx->prior_kappa.release();
+      ctx->prior_grad.release();
+      return;
+    }
+    if (prior->type != MI_PRIOR_QUADRATIC && prior->type != MI_PRIOR_RELATIVE_DIFFERENCE)
+      throw ApiError(MI_ERR_ARGUMENT, "unknown prior type");
+    const mi_geometry& g = ctx->plan->geometry();
+    ctx->prior_kind = prior->type;
+    ctx->prior_beta = prior->penalisation_factor;
+    ctx->prior_gamma = prior->gamma;
+    ctx->prior_eps = prior->epsilon;
+    ctx->prior_only_2D = prior->only_2D != 0;
+    if (prior->weights) std::memcpy(ctx->prior_w.w, prior->weights, sizeof(ctx->prior_w.w));
+    else default_prior_weights(g, ctx->prior_w.w);
+    hipStream_t st = ctx->stream;
+    if (prior->kappa) {
+      ctx->prior_kappa.alloc(ctx->img.n);
+      HIPCHECK(hipMemsetAsync(ctx->prior_kappa.p, 0, ctx->prior_kappa.bytes(), st));
+      HIPCHECK(hipMemcpyAsync(ctx->stage_img.p, prior->kappa, ctx->plan->num_voxels() * sizeof(float),
+                              hipMemcpyHostToDevice, st));
+      ctx->image_to_device(ctx->stage_img.p, ctx->prior_kappa.p, st);
+    } else {
+      ctx->prior_kappa.release();
+    }
+    ctx->prior_grad.alloc(ctx->img.n);
+    HIPCHECK(hipMemsetAsync(ctx->prior_grad.p, 0, ctx->prior_grad.bytes(), st));
+    ctx->small_part.alloc(mi::max_num_partials());
+    ctx->small_val.alloc(1);
+    HIPCHECK(hipStreamSynchronize(st));
+  });
+}
+
+int mi_relative_difference_prior_dev(mi_ctx* ctx, const float* d_image, const float* d_kappa, const float* weights,
+                                     float beta, float gamma, float epsilon, int only_2D, float* d_gradient,
+                                     double* d_value, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_image) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (!d_gradient && !d_value) throw ApiError(MI_ERR_ARGUMENT, "neither a gradient nor a value was asked for");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->pick(stream);
+    const mi_geometry& g = ctx->plan->geometry();
+    if (beta == 0.F) {  // RelativeDifferencePrior.cxx:307-310,380-384
+      if (d_gradient) mi::launch_fill(d_gradient, ctx->plan->num_voxels(), 0.F, st);
+      if (d_value) mi::launch_sum_partials(ctx->prior_partial.p, 0, 0.0, d_value, st);
+    } else {
+      mi::PriorWeights W;
+      if (weights) std::memcpy(W.w, weights, sizeof(W.w));
+      else default_prior_weights(g, W.w);
+      mi::launch_prior(2, d_image, d_kappa, ctx->stir_prior_image(), W, only_2D, beta, gamma, epsilon, d_gradient,
+                       ctx->prior_partial.p, st);
+      if (d_value)
+        mi::launch_sum_partials(ctx->prior_partial.p, mi::prior_num_partials(ctx->stir_prior_image()),
+                                static_cast<double>(beta), d_value, st);
+    }
+    HIPCHECK(hipGetLastError());
   });
 }
 
@@ -1185,8 +1377,9 @@ struct mi_batch {
   // launch choices (mi_batch_tune)
   int fp_widen = 0;             // forward groups: 0 = smallest group size holding the family, 1 = twice that
   int fp_group_max = 32;        // forward tasks with at most this many bins go to lane groups, the rest to full waves
-  int bp_pack = 1;              // back projection: columns of at most 32 planes in lane groups (1) or full waves (0)
+  int bp_pack = 0;              // back projection: columns of at most 32 planes in lane groups (1) or full waves (0)
   int tile_x = 32, tile_y = 8;  // back-projection column order
+  int fp_rowgroup_max = 32;     // families of at most this many axial positions run in row groups (0: off)
   // device descriptors and work lists
   DBuf<mi::ProblemDesc> descs;
   std::vector<std::unique_ptr<DBuf<const float*>>> sens_lists;
@@ -1202,6 +1395,10 @@ struct mi_batch {
   std::vector<std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 4>> fpw_work;
   std::array<int, 12> bpw_n{};
   std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 12> bpw_work;
+  // forward row groups per subset: items and their family lists
+  std::vector<int> rg_n;
+  std::vector<std::unique_ptr<DBuf<mi::RowGroupItem>>> rg_items;
+  std::vector<std::unique_ptr<DBuf<int32_t>>> rg_fams;
   bool lists_ready = false;
   hipGraphExec_t graph_exec = nullptr;
   int graph_key[3] = {0, 0, 0};  // start subiteration, count, start subset of the captured graph
@@ -1285,16 +1482,33 @@ void batch_prepare(mi_batch* b) {
   b->fpw_n.assign(b->nsub, {0, 0, 0, 0});
   b->fpw_work.clear();
   b->fpw_work.resize(b->nsub);
+  b->rg_n.assign(b->nsub, 0);
+  b->rg_items.clear();
+  b->rg_fams.clear();
   for (int s = 0; s < b->nsub; ++s) {
     std::array<std::vector<mi::WorkItem>, 4> lists;
     std::array<std::vector<mi::WorkItem>, 4> wide;
+    std::vector<mi::RowGroupItem> rg;
+    std::vector<int32_t> rg_fam;
     for (int p = 0; p < np; ++p) {
       const mi_ctx* c = b->probs[p];
       std::vector<mi::FpTask> ht(c->task_off[s + 1] - c->task_off[s]);
       if (!ht.empty())
         HIPCHECK(hipMemcpy(ht.data(), c->tasks.p + c->task_off[s], ht.size() * sizeof(mi::FpTask), hipMemcpyDeviceToHost));
+      // families sharing a basic row, in order of first appearance
+      std::unordered_map<int, int> group_of_row;
+      std::vector<std::vector<int>> groups;
       for (size_t i = 0; i < ht.size(); ++i) {
         const mi::FpTask& t = ht[i];
+        if (b->fp_rowgroup_max > 0 && c->h_fams[t.first].nax <= std::min(b->fp_rowgroup_max, 64)) {
+          for (int f : {t.first, t.second}) {
+            if (f < 0) continue;
+            auto it = group_of_row.emplace(c->h_fams[f].row_start, static_cast<int>(groups.size()));
+            if (it.second) groups.emplace_back();
+            groups[it.first->second].push_back(f);
+          }
+          continue;
+        }
         const int total = c->h_fams[t.first].nax + (t.second >= 0 ? c->h_fams[t.second].nax : 0);
         if (total > b->fp_group_max) {
           const int nslot = std::min((total + 63) / 64, 4);
@@ -1307,6 +1521,23 @@ void batch_prepare(mi_batch* b) {
           lists[gi].push_back({p, f});
         }
       }
+      // pack each row group into items of at most 64 axial positions
+      for (const auto& g : groups) {
+        mi::RowGroupItem cur{p, static_cast<int32_t>(rg_fam.size()), 0, 0};
+        int lanes = 0;
+        for (int f : g) {
+          const int n = c->h_fams[f].nax;
+          if (lanes + n > 64 && cur.count > 0) {
+            rg.push_back(cur);
+            cur = mi::RowGroupItem{p, static_cast<int32_t>(rg_fam.size()), 0, 0};
+            lanes = 0;
+          }
+          rg_fam.push_back(f);
+          ++cur.count;
+          lanes += n;
+        }
+        if (cur.count > 0) rg.push_back(cur);
+      }
     }
     for (int gi = 0; gi < 4; ++gi) {
       b->fp_n[s][gi] = static_cast<int>(lists[gi].size());
@@ -1316,6 +1547,17 @@ void batch_prepare(mi_batch* b) {
       b->fpw_n[s][k] = static_cast<int>(wide[k].size());
       b->fpw_work[s][k] = upload(wide[k]);
     }
+    b->rg_n[s] = static_cast<int>(rg.size());
+    auto di = std::make_unique<DBuf<mi::RowGroupItem>>();
+    auto df = std::make_unique<DBuf<int32_t>>();
+    if (!rg.empty()) {
+      di->alloc(rg.size());
+      HIPCHECK(hipMemcpy(di->p, rg.data(), rg.size() * sizeof(mi::RowGroupItem), hipMemcpyHostToDevice));
+      df->alloc(rg_fam.size());
+      HIPCHECK(hipMemcpy(df->p, rg_fam.data(), rg_fam.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    b->rg_items.push_back(std::move(di));
+    b->rg_fams.push_back(std::move(df));
   }
   // back-projection work: the columns of every problem in tile order; images
   // of at most 32 planes in lane groups (when packing), the others one column per wave
@@ -1350,6 +1592,7 @@ void batch_prepare(mi_batch* b) {
 }
 
 void batch_fp(mi_batch* b, mi::FpMode mode, int s) {
+  mi::launch_fp_rowgroup(mode, b->descs.p, b->rg_items[s]->p, b->rg_fams[s]->p, b->rg_n[s], b->stream);
   for (int k = 0; k < 4; ++k)
     mi::launch_fp_batch(mode, k + 1, b->descs.p, b->task_lists.p, b->fpw_work[s][k]->p, b->fpw_n[s][k], b->stream);
   for (int gi = 0; gi < 4; ++gi)
@@ -1447,13 +1690,22 @@ int mi_batch_set_launch(mi_batch* b, int fp_group_max, int fp_widen, int bp_pack
   });
 }
 
+int mi_batch_set_row_groups(mi_batch* b, int max_axial) {
+  return guarded([&] {
+    if (!b) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (max_axial < 0 || max_axial > 64) throw ApiError(MI_ERR_ARGUMENT, "max_axial must be in [0, 64]");
+    b->fp_rowgroup_max = max_axial;
+    batch_invalidate(b);
+  });
+}
+
 int mi_batch_tune(mi_batch* b, int* choice) {
   return guarded([&] {
     if (!b) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     HIPCHECK(hipSetDevice(b->device));
-    struct Cand { int gmax, widen, pack, tx, ty; };
-    const Cand cands[] = {{32, 0, 1, 32, 8}, {16, 0, 1, 32, 8}, {0, 0, 0, 32, 8}, {32, 1, 1, 32, 8},
-                          {32, 0, 0, 32, 8}, {32, 0, 1, 8, 8},  {32, 0, 1, 1, 1},  {32, 0, 1, 64, 4}};
+    struct Cand { int rg, gmax, widen, pack, tx, ty; };
+    const Cand cands[] = {{32, 32, 0, 0, 32, 8}, {32, 32, 0, 1, 32, 8}, {16, 32, 0, 0, 32, 8}, {0, 32, 0, 0, 32, 8},
+                          {0, 0, 0, 0, 32, 8},   {32, 32, 0, 0, 8, 8},  {32, 32, 0, 0, 64, 4}, {32, 32, 0, 0, 1, 1}};
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
@@ -1461,6 +1713,7 @@ int mi_batch_tune(mi_batch* b, int* choice) {
     int best_i = 0;
     const int ncand = static_cast<int>(sizeof(cands) / sizeof(cands[0]));
     for (int i = 0; i < ncand; ++i) {
+      b->fp_rowgroup_max = cands[i].rg;
       b->fp_group_max = cands[i].gmax;
       b->fp_widen = cands[i].widen;
       b->bp_pack = cands[i].pack;
@@ -1485,6 +1738,7 @@ int mi_batch_tune(mi_batch* b, int* choice) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    b->fp_rowgroup_max = cands[best_i].rg;
     b->fp_group_max = cands[best_i].gmax;
     b->fp_widen = cands[best_i].widen;
     b->bp_pack = cands[best_i].pack;

@@ -86,7 +86,9 @@ struct BpData {
   const float* sens;  // subset sensitivity, device image layout
   float* estimate;    // x, device image layout, updated in place
   float* update;      // if not null, receives the multiplicative update u
-  int threshold;      // apply threshold_upper_lower(0, FLT_MAX)
+  int threshold;      // apply threshold_upper_lower(thr_lo, thr_hi)
+  float thr_lo = 0.F;            // "minimum relative change"
+  float thr_hi = 3.40282347e38F;  // "maximum relative change" (FLT_MAX)
 };
 
 // ---- row builder (pmrt_trace.hip)
@@ -160,6 +162,17 @@ struct WorkItem {
   int32_t problem;
   int32_t index;
 };
+// a row group of a batched forward projection: item_fams[first .. first+count)
+// are families of one problem sharing one basic row, at most 64 axial
+// positions in total
+struct RowGroupItem {
+  int32_t problem;
+  int32_t first;
+  int32_t count;
+  int32_t pad;
+};
+void launch_fp_rowgroup(FpMode mode, const ProblemDesc* d_descs, const RowGroupItem* d_items, const int32_t* d_item_fams,
+                        int nitems, hipStream_t stream);
 // G lanes per work item (G in 8, 16, 32, 64): a forward-projection group
 // handles one family with at most G axial positions, a back-projection group
 // one image column with at most G planes.
@@ -199,12 +212,29 @@ void launch_zero_end_planes(float* d_proj, const int64_t* d_off, int count, int 
 struct PriorWeights {
   float w[27];
 };
-// Value partials (one double per workgroup, count returned by
-// prior_num_partials) and gradient (if d_grad) of the quadratic prior of a
-// [nz][ny][nx] image; kappa may be null.
-int prior_num_partials(int nz, int ny, int nx);
-void launch_quadratic_prior(const float* d_img, const float* d_kappa, int nz, int ny, int nx, const PriorWeights& w,
-                            int only_2D, float beta, float* d_grad, double* d_partial, hipStream_t stream);
+// Image of a prior kernel: [z][y][x] (zfast = 0) or the device layout
+// [y][x][zp] (zfast = 1, kappa and gradient in the same layout).
+struct PriorImage {
+  int nz, ny, nx, nzp;
+  int zfast;
+};
+// Value partials without beta (one double per workgroup, count returned by
+// prior_num_partials) and beta * gradient (if d_grad) of a neighbourhood
+// prior: kind 1 quadratic, 2 relative difference (gamma, eps); kappa may be null.
+int prior_num_partials(const PriorImage& im);
+void launch_prior(int kind, const float* d_img, const float* d_kappa, const PriorImage& im, const PriorWeights& w,
+                  int only_2D, float beta, float gamma, float eps, float* d_grad, double* d_partial,
+                  hipStream_t stream);
+// *d_small = max(max(v) * small_num, 0) over n floats (d_part: max_num_partials() floats)
+int max_num_partials();
+void launch_small_value(const float* d_v, int64_t n, float small_num, float* d_part, float* d_small, hipStream_t stream);
+// One-step-late OSMAPOSL update on the device layout (n = ncols * nzp elements)
+void launch_osl_update(float* d_x, const float* d_upd, const float* d_pgrad, const float* d_sens, const float* d_small,
+                       int nsub, int multiplicative, int threshold, float lo, float hi, int nz, int nzp, int64_t n,
+                       hipStream_t stream);
+// dst += src; dst /= d (element-wise, n elements)
+void launch_accumulate(float* d_dst, const float* d_src, int64_t n, hipStream_t stream);
+void launch_divide_scalar(float* d_dst, float d, int64_t n, hipStream_t stream);
 // *d_out = sum of n doubles in a fixed order (one workgroup); d_out may be null-checked by the caller.
 void launch_sum_partials(const double* d_partial, int n, double scale, double* d_out, hipStream_t stream);
 // v = g * x / sens (0 where sens == 0), device image layout, n elements; v may alias g or x

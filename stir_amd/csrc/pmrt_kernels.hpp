@@ -128,6 +128,21 @@ __device__ inline int tiled_column(int w, const ImageDims& d, int2 tile) {
   return (y < d.ny && x < d.nx) ? y * d.nx + x : -1;
 }
 
+// Multiplicative update of the prior-free OSMAPOSL step from u = BP(y/ybar)
+// and the subset sensitivity: divide(u, sens, 0) with 0/0 = 0
+// (numerics/divide.inl:26-47), then threshold_upper_lower(lo, hi)
+// (thresholding.h:45-54) except in subiteration 1
+// (OSMAPOSLReconstruction.cxx:491-507).
+__device__ inline float osem_factor(float u, float sv, int threshold, float lo, float hi) {
+  if (fabsf(sv) <= 0.F && fabsf(u) <= 0.F) u = 0;
+  else u = u / sv;
+  if (threshold) {
+    if (u > hi) u = hi;
+    else if (lo > u) u = lo;
+  }
+  return u;
+}
+
 constexpr bool fp_writes_ratio(int m) {
   return m == FP_RATIO || m == FP_GRADIENT || m == FP_GRADIENT_LOGLIK || m == FP_RATIO_LOGLIK;
 }

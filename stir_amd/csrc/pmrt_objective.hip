@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 
 #include "pmrt_device.hpp"
 
@@ -14,25 +15,35 @@ constexpr int kPriorTileX = 64;
 constexpr int kPriorTileY = 4;
 constexpr int kPriorThreads = kPriorTileX * kPriorTileY;
 
-// QuadraticPrior::compute_value / compute_gradient (QuadraticPrior.cxx:229-367)
-// for one voxel per thread, image [nz][ny][nx]. Neighbours outside the image
-// are skipped; each term is evaluated in float as the reference does
-// (w * (a-b)^2 / 4 and w * (a-b)) and accumulated in double in (dz, dy, dx)
-// order. Workgroup = 64 x 4 voxels of one plane; its value partial is written
-// to partial[block].
-template <bool KAPPA>
-__global__ void __launch_bounds__(kPriorThreads) quadratic_prior_kernel(const float* __restrict__ img,
-                                                                        const float* __restrict__ kappa, int nz, int ny,
-                                                                        int nx, PriorWeights W, int wz0, int wz1,
-                                                                        float beta, float* __restrict__ grad,
-                                                                        double* __restrict__ partial) {
-  const int x = blockIdx.x * kPriorTileX + static_cast<int>(threadIdx.x % kPriorTileX);
-  const int y = blockIdx.y * kPriorTileY + static_cast<int>(threadIdx.x / kPriorTileX);
-  const int z = blockIdx.z;
+// Neighbourhood priors, one voxel per thread, accumulated in double in
+// (dz, dy, dx) order; neighbours outside the image are skipped.
+//  KIND 1, QuadraticPrior::compute_value / compute_gradient
+//    (QuadraticPrior.cxx:229-367): float terms w * (a-b)^2 / 4 and w * (a-b).
+//  KIND 2, RelativeDifferencePrior::compute_value / compute_gradient
+//    (RelativeDifferencePrior.cxx:280-433): value term
+//    w * 0.5 * (double)(a-b)^2 / (float)(a + b + gamma|a-b| + eps), gradient term
+//    float w * derivative_10(a, b); both 0 at a = b = 0 when eps = 0.
+// ZFAST: the device layout [y][x][zp] (threads along z), else [z][y][x]
+// (threads along x). Workgroup = 64 x 4 voxels; its value partial (without
+// beta) is written to partial[block]; grad = beta * gradient.
+template <int KIND, bool KAPPA, bool ZFAST>
+__global__ void __launch_bounds__(kPriorThreads) prior_kernel(const float* __restrict__ img,
+                                                              const float* __restrict__ kappa, PriorImage im,
+                                                              PriorWeights W, int wz0, int wz1, float beta, float gamma,
+                                                              float eps, float* __restrict__ grad,
+                                                              double* __restrict__ partial) {
+  const int c = blockIdx.x * kPriorTileX + static_cast<int>(threadIdx.x % kPriorTileX);
+  const int b = blockIdx.y * kPriorTileY + static_cast<int>(threadIdx.x / kPriorTileX);
+  const int a = blockIdx.z;
+  const int nz = im.nz, ny = im.ny, nx = im.nx;
+  const int z = ZFAST ? c : a, y = ZFAST ? a : b, x = ZFAST ? b : c;
+  // strides of z, y, x
+  const int64_t sz = ZFAST ? 1 : static_cast<int64_t>(ny) * nx;
+  const int64_t sy = ZFAST ? static_cast<int64_t>(nx) * im.nzp : nx;
+  const int64_t sx = ZFAST ? im.nzp : 1;
   double value = 0.;
-  if (x < nx && y < ny) {
-    const int64_t plane = static_cast<int64_t>(ny) * nx;
-    const int64_t v = z * plane + static_cast<int64_t>(y) * nx + x;
+  if (x < nx && y < ny && z < nz) {
+    const int64_t v = z * sz + y * sy + x * sx;
     const float xv = img[v];
     const float kv = KAPPA ? kappa[v] : 1.F;
     const int min_dz = max(wz0, -z), max_dz = min(wz1, nz - 1 - z);
@@ -42,11 +53,29 @@ __global__ void __launch_bounds__(kPriorThreads) quadratic_prior_kernel(const fl
     for (int dz = min_dz; dz <= max_dz; ++dz)
       for (int dy = min_dy; dy <= max_dy; ++dy)
         for (int dx = min_dx; dx <= max_dx; ++dx) {
-          const int64_t n = v + dz * plane + static_cast<int64_t>(dy) * nx + dx;
+          const int64_t n = v + dz * sz + dy * sy + dx * sx;
           const float w = W.w[((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)];
-          const float d = xv - img[n];
-          double cv = static_cast<double>(w * (d * d) / 4);
-          double cg = static_cast<double>(w * d);
+          const float xn = img[n];
+          double cv, cg;
+          if constexpr (KIND == 1) {
+            const float d = xv - xn;
+            cv = static_cast<double>(w * (d * d) / 4);
+            cg = static_cast<double>(w * d);
+          } else {
+            const float ad = fabsf(xv - xn);
+            if (eps == 0.F && xv == 0.F && xn == 0.F) {
+              cv = 0.0;
+              cg = 0.0;
+            } else {
+              const float den = xv + xn + gamma * ad + eps;
+              const double dd = static_cast<double>(xv - xn);
+              cv = static_cast<double>(w) * (0.5 * (dd * dd / den));
+              const float t = gamma * ad + xv + 3 * xn + 2 * eps;
+              const double num = dd * t;
+              const double ds = static_cast<double>(xv + xn) + gamma * ad + eps;
+              cg = static_cast<double>(w * static_cast<float>(num / (ds * ds)));
+            }
+          }
           if (KAPPA) {
             const float kk = kv * kappa[n];
             cv *= kk;
@@ -66,6 +95,74 @@ __global__ void __launch_bounds__(kPriorThreads) quadratic_prior_kernel(const fl
     double s = 0.;
     for (int i = 0; i < kPriorThreads / 64; ++i) s += red[i];
     partial[(static_cast<int64_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// Maximum of n floats: per-workgroup maxima into part[blockIdx.x].
+__global__ void __launch_bounds__(256) max_partials_kernel(const float* __restrict__ v, int64_t n,
+                                                           float* __restrict__ part) {
+  float m = -FLT_MAX;
+  for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256)
+    m = fmaxf(m, v[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  __shared__ float red[4];
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// *small = max(max(part) * small_num, 0) -- the threshold of divide()
+// (numerics/divide.inl:33-34).
+__global__ void __launch_bounds__(256) small_value_kernel(const float* __restrict__ part, int n, float small_num,
+                                                          float* __restrict__ small) {
+  float m = -FLT_MAX;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmaxf(m, part[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  __shared__ float red[4];
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float sv = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])) * small_num;
+    *small = sv > 0 ? sv : 0.F;
+  }
+}
+
+// One-step-late update of OSMAPOSL with a prior (OSMAPOSLReconstruction.cxx:403-516)
+// on the device layout: denominator from the prior gradient g and the subset
+// sensitivity s -- "additive": min/max-clamped g / num_subsets + s into
+// [s/10, 10 s]; "multiplicative": s * clamp(1 + g, 0.1, 10) -- then
+// divide(u, d, 1e-6) with the precomputed *small, threshold_upper_lower(lo, hi)
+// unless the first subiteration, and x *= u. Clamps use std::min / std::max
+// semantics.
+__global__ void osl_update_kernel(float* __restrict__ x, const float* __restrict__ upd,
+                                  const float* __restrict__ pgrad, const float* __restrict__ sens,
+                                  const float* __restrict__ small, int nsub, int multiplicative, int threshold,
+                                  float lo, float hi, int nz, int nzp, int64_t n) {
+  const float sm = *small;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    if (static_cast<int>(i % nzp) >= nz) continue;
+    const float s = sens[i];
+    float d = pgrad[i];
+    if (!multiplicative) {
+      d = d / nsub + s;
+      const float hi_d = s * 10, lo_d = s / 10;
+      d = (hi_d < d) ? hi_d : d;
+      d = (d < lo_d) ? lo_d : d;
+    } else {
+      d += 1;
+      d = (10.F < d) ? 10.F : d;
+      d = (d < 1 / 10.F) ? 1 / 10.F : d;
+      d *= s;
+    }
+    float u = upd[i];
+    if (fabsf(d) <= sm && fabsf(u) <= sm) u = 0;
+    else u = u / d;
+    if (threshold) {
+      if (u > hi) u = hi;
+      else if (lo > u) u = lo;
+    }
+    x[i] = x[i] * u;
   }
 }
 
@@ -102,24 +199,80 @@ __global__ void add_product_kernel(float* __restrict__ out, const float* __restr
     out[i] = out[i] + g[i] * u[i];
 }
 
+__global__ void accumulate_kernel(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = dst[i] + src[i];
+}
+
+__global__ void divide_scalar_kernel(float* __restrict__ dst, float d, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = dst[i] / d;
+}
+
 int elementwise_blocks(int64_t n) { return static_cast<int>(std::min<int64_t>((n + 255) / 256, 8192)); }
 
 }  // namespace
 
-int prior_num_partials(int nz, int ny, int nx) {
-  return ((nx + kPriorTileX - 1) / kPriorTileX) * ((ny + kPriorTileY - 1) / kPriorTileY) * nz;
+namespace {
+dim3 prior_grid(const PriorImage& im) {
+  const int nc = im.zfast ? im.nz : im.nx, nb = im.zfast ? im.nx : im.ny, na = im.zfast ? im.ny : im.nz;
+  return dim3((nc + kPriorTileX - 1) / kPriorTileX, (nb + kPriorTileY - 1) / kPriorTileY, na);
+}
+}  // namespace
+
+int prior_num_partials(const PriorImage& im) {
+  const dim3 g = prior_grid(im);
+  return static_cast<int>(g.x * g.y * g.z);
 }
 
-void launch_quadratic_prior(const float* d_img, const float* d_kappa, int nz, int ny, int nx, const PriorWeights& w,
-                            int only_2D, float beta, float* d_grad, double* d_partial, hipStream_t stream) {
-  const dim3 grid((nx + kPriorTileX - 1) / kPriorTileX, (ny + kPriorTileY - 1) / kPriorTileY, nz);
+void launch_prior(int kind, const float* d_img, const float* d_kappa, const PriorImage& im, const PriorWeights& w,
+                  int only_2D, float beta, float gamma, float eps, float* d_grad, double* d_partial,
+                  hipStream_t stream) {
+  const dim3 grid = prior_grid(im);
   const int wz0 = only_2D ? 0 : -1, wz1 = only_2D ? 0 : 1;
-  if (d_kappa)
-    hipLaunchKernelGGL(quadratic_prior_kernel<true>, grid, dim3(kPriorThreads), 0, stream, d_img, d_kappa, nz, ny, nx, w,
-                       wz0, wz1, beta, d_grad, d_partial);
-  else
-    hipLaunchKernelGGL(quadratic_prior_kernel<false>, grid, dim3(kPriorThreads), 0, stream, d_img, d_kappa, nz, ny, nx,
-                       w, wz0, wz1, beta, d_grad, d_partial);
+#define MI_PRIOR(K, KA, ZF)                                                                                         \
+  hipLaunchKernelGGL((prior_kernel<K, KA, ZF>), grid, dim3(kPriorThreads), 0, stream, d_img, d_kappa, im, w, wz0, wz1, \
+                     beta, gamma, eps, d_grad, d_partial)
+#define MI_PRIOR_Z(K, KA)                 \
+  if (im.zfast) { MI_PRIOR(K, KA, true); } \
+  else { MI_PRIOR(K, KA, false); }
+#define MI_PRIOR_K(K)                        \
+  if (d_kappa) { MI_PRIOR_Z(K, true) }        \
+  else { MI_PRIOR_Z(K, false) }
+  if (kind == 2) { MI_PRIOR_K(2) }
+  else { MI_PRIOR_K(1) }
+#undef MI_PRIOR_K
+#undef MI_PRIOR_Z
+#undef MI_PRIOR
+}
+
+int max_num_partials() { return 1024; }
+
+void launch_small_value(const float* d_v, int64_t n, float small_num, float* d_part, float* d_small,
+                        hipStream_t stream) {
+  const int blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, max_num_partials())));
+  hipLaunchKernelGGL(max_partials_kernel, dim3(blocks), dim3(256), 0, stream, d_v, n, d_part);
+  hipLaunchKernelGGL(small_value_kernel, dim3(1), dim3(256), 0, stream, d_part, blocks, small_num, d_small);
+}
+
+void launch_osl_update(float* d_x, const float* d_upd, const float* d_pgrad, const float* d_sens, const float* d_small,
+                       int nsub, int multiplicative, int threshold, float lo, float hi, int nz, int nzp, int64_t n,
+                       hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(osl_update_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_x, d_upd, d_pgrad, d_sens,
+                     d_small, nsub, multiplicative, threshold, lo, hi, nz, nzp, n);
+}
+
+void launch_accumulate(float* d_dst, const float* d_src, int64_t n, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(accumulate_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_dst, d_src, n);
+}
+
+void launch_divide_scalar(float* d_dst, float d, int64_t n, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(divide_scalar_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_dst, d, n);
 }
 
 void launch_sum_partials(const double* d_partial, int n, double scale, double* d_out, hipStream_t stream) {

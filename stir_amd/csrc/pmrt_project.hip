@@ -213,16 +213,8 @@ __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> 
       out[z] = z < d.nz ? acc[q] : 0.F;
     } else {
       if (z < d.nz) {
-        // divide(update, sens, 0) (numerics/divide.inl:26-47), threshold, multiply
         const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
-        float u = acc[q];
-        const float sv = as_const(data.sens)[idx];
-        if (fabsf(sv) <= 0.F && fabsf(u) <= 0.F) u = 0;
-        else u = u / sv;
-        if (data.threshold) {
-          if (u > FLT_MAX) u = FLT_MAX;
-          else if (0.F > u) u = 0.F;
-        }
+        const float u = osem_factor(acc[q], as_const(data.sens)[idx], data.threshold, data.thr_lo, data.thr_hi);
         as_global(data.estimate)[idx] = as_global(data.estimate)[idx] * u;
         if (data.update) as_global(data.update)[idx] = u;
       }

@@ -309,16 +309,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_kernel(const int
       outc[z] = z < d.nz ? acc[q] : 0.F;
     } else {
       if (z < d.nz) {
-        // divide(update, sens, 0) (numerics/divide.inl:26-47), threshold, multiply
         const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
-        float u = acc[q];
-        const float sv = data.sens[idx];
-        if (fabsf(sv) <= 0.F && fabsf(u) <= 0.F) u = 0;
-        else u = u / sv;
-        if (data.threshold) {
-          if (u > FLT_MAX) u = FLT_MAX;
-          else if (0.F > u) u = 0.F;
-        }
+        const float u = osem_factor(acc[q], data.sens[idx], data.threshold, data.thr_lo, data.thr_hi);
         data.estimate[idx] = data.estimate[idx] * u;
         if (data.update) data.update[idx] = u;
       }

@@ -230,12 +230,21 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
         self._ctx = projector.context
 
     def set_up(self, measured: np.ndarray, additive=0.0, zero_end_planes: bool = False,
-               compute_sensitivities: bool = True, use_tof_sensitivities: bool = False) -> None:
+               compute_sensitivities: bool = True, use_tof_sensitivities: bool = False,
+               multiplicative: Optional[np.ndarray] = None, use_subset_sensitivities: bool = True) -> None:
         """use_tof_sensitivities: "use time-of-flight sensitivities" (default off:
-        TOF data get non-TOF sensitivities, ``...AndProjData.cxx:533-536``)."""
+        TOF data get non-TOF sensitivities, ``...AndProjData.cxx:533-536``).
+        multiplicative: the factors m of ybar = m (A x + b) -- the
+        normalisation's ``undo`` of ones (``distributable.cxx:197-215``), e.g.
+        1 / the "From ProjData" normalisation factors -- or None.
+        use_subset_sensitivities: the reference's key (default true); false
+        divides every subset's update by total sensitivity / num_subsets."""
         L = _native.load()
         _native.check(L.mi_obj_set_use_tof_sensitivities(self._ctx.handle, int(use_tof_sensitivities)),
                       "mi_obj_set_use_tof_sensitivities")
+        _native.check(L.mi_obj_set_use_subset_sensitivities(self._ctx.handle, int(use_subset_sensitivities)),
+                      "mi_obj_set_use_subset_sensitivities")
+        self.set_multiplicative(multiplicative)
         y = np.ascontiguousarray(measured, dtype=np.float32).reshape(-1)
         if isinstance(additive, np.ndarray):
             ap, keep = _native.float_ptr_or_none(additive)
@@ -247,6 +256,29 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
                                             int(zero_end_planes)), "mi_obj_set_data")
         if compute_sensitivities:
             _native.check(L.mi_obj_compute_sensitivities(self._ctx.handle), "mi_obj_compute_sensitivities")
+
+    def set_multiplicative(self, multiplicative: Optional[np.ndarray]) -> None:
+        """Multiplicative factors (all TOF bins, measured order) or None; the
+        sensitivities must be recomputed (``set_up`` does)."""
+        if multiplicative is None:
+            _native.check(_native.load().mi_obj_set_multiplicative(self._ctx.handle, None), "mi_obj_set_multiplicative")
+            return
+        m = np.ascontiguousarray(multiplicative, dtype=np.float32).reshape(-1)
+        _native.check(_native.load().mi_obj_set_multiplicative(self._ctx.handle, m.ctypes.data_as(ctypes.c_void_p)),
+                      "mi_obj_set_multiplicative")
+
+    def compute_sensitivities(self) -> None:
+        _native.check(_native.load().mi_obj_compute_sensitivities(self._ctx.handle), "mi_obj_compute_sensitivities")
+
+    def set_prior(self, prior: Optional["QuadraticPrior"]) -> None:
+        """Prior of OSMAPOSL's one-step-late update (``GeneralisedObjectiveFunction::set_prior_sptr``)."""
+        self.prior = prior
+        if prior is None:
+            _native.check(_native.load().mi_obj_set_prior(self._ctx.handle, None), "mi_obj_set_prior")
+            return
+        st, keep = prior._struct()
+        _native.check(_native.load().mi_obj_set_prior(self._ctx.handle, ctypes.byref(st)), "mi_obj_set_prior")
+        del keep
 
     def get_subset_sensitivity(self, subset: int) -> np.ndarray:
         out = np.empty(self.projector.img.shape, np.float32)
@@ -280,25 +312,110 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
         return float(v.value)
 
 
+class QuadraticPrior:
+    """``QuadraticPrior`` (``QuadraticPrior.cxx``) as a prior of the objective
+    function: penalisation factor, optional 3x3x3 weights (default vox_x /
+    distance), kappa image and only_2D. compute_value / compute_gradient run on
+    the device of the projector the prior is used with."""
+
+    kind = _native.MI_PRIOR_QUADRATIC
+
+    def __init__(self, penalisation_factor: float = 0.0, only_2D: bool = False, weights=None, kappa=None):
+        self.penalisation_factor = float(penalisation_factor)
+        self.only_2D = bool(only_2D)
+        self.weights = None if weights is None else np.ascontiguousarray(np.asarray(weights, np.float32).reshape(27))
+        self.kappa = None if kappa is None else np.ascontiguousarray(kappa, dtype=np.float32)
+        self.gamma, self.epsilon = 2.0, 0.0
+
+    def _struct(self):
+        st = _native.PriorStruct(self.kind, self.penalisation_factor, self.gamma, self.epsilon, int(self.only_2D),
+                                 None if self.weights is None else self.weights.ctypes.data,
+                                 None if self.kappa is None else self.kappa.ctypes.data)
+        return st, (self.weights, self.kappa)
+
+    def _loss(self, projector: "ProjectorByBinPairMI355X"):
+        import torch
+        from .autograd import QuadraticPriorLoss
+
+        kappa = None if self.kappa is None else torch.from_numpy(self.kappa.reshape(-1)).cuda()
+        return QuadraticPriorLoss(projector, self.penalisation_factor, kappa, self.only_2D, self.weights)
+
+    def compute_value(self, image: np.ndarray, projector: "ProjectorByBinPairMI355X") -> float:
+        import torch
+
+        x = torch.from_numpy(np.ascontiguousarray(image, dtype=np.float32)).cuda()
+        return float(self._loss(projector)(x).item())
+
+    def compute_gradient(self, image: np.ndarray, projector: "ProjectorByBinPairMI355X") -> np.ndarray:
+        import torch
+
+        x = torch.from_numpy(np.ascontiguousarray(image, dtype=np.float32)).cuda().requires_grad_(True)
+        self._loss(projector)(x).backward()
+        return x.grad.cpu().numpy()
+
+
+class RelativeDifferencePrior(QuadraticPrior):
+    """``RelativeDifferencePrior`` (``RelativeDifferencePrior.cxx``): gamma
+    (default 2) and epsilon (default 0) besides the quadratic prior's keys."""
+
+    kind = _native.MI_PRIOR_RELATIVE_DIFFERENCE
+
+    def __init__(self, penalisation_factor: float = 0.0, gamma: float = 2.0, epsilon: float = 0.0,
+                 only_2D: bool = False, weights=None, kappa=None):
+        super().__init__(penalisation_factor, only_2D, weights, kappa)
+        self.gamma, self.epsilon = float(gamma), float(epsilon)
+
+    def _loss(self, projector):
+        import torch
+        from .autograd import RelativeDifferencePriorLoss
+
+        kappa = None if self.kappa is None else torch.from_numpy(self.kappa.reshape(-1)).cuda()
+        return RelativeDifferencePriorLoss(projector, self.penalisation_factor, self.gamma, self.epsilon, kappa,
+                                           self.only_2D, self.weights)
+
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
 class OSMAPOSLReconstruction:
-    """Prior-free OSMAPOSL (``OSMAPOSLReconstruction.cxx:266-516``) on the device."""
+    """OSMAPOSL (``OSMAPOSLReconstruction.cxx:266-516``) on the device: prior-free
+    EM updates, or one-step-late MAP updates with the objective's prior.
+    MAP_model "additive" (default) or "multiplicative"; minimum / maximum
+    relative change bound the multiplicative update after subiteration 1."""
 
     def __init__(self, objective: PoissonLogLikelihoodWithLinearModelForMeanAndProjData,
-                 num_subiterations: int, start_subset: int = 0, enforce_initial_positivity: bool = True):
+                 num_subiterations: int, start_subset: int = 0, enforce_initial_positivity: bool = True,
+                 MAP_model: str = "additive", minimum_relative_change: float = 0.0,
+                 maximum_relative_change: float = FLT_MAX, start_subiteration_num: int = 1):
+        if MAP_model not in ("additive", "multiplicative"):
+            raise ValueError(f"MAP model should have as value 'additive' or 'multiplicative', while it is {MAP_model!r}")
         self.objective = objective
         self.num_subiterations = num_subiterations
         self.start_subset = start_subset
+        self.start_subiteration_num = start_subiteration_num
         self.enforce_initial_positivity = enforce_initial_positivity
+        self.MAP_model = MAP_model
+        self.minimum_relative_change = float(minimum_relative_change)
+        self.maximum_relative_change = float(maximum_relative_change)
+
+    def _set_options(self):
+        _native.check(_native.load().mi_osem_set_options(self.objective._ctx.handle,
+                                                         int(self.MAP_model == "multiplicative"),
+                                                         ctypes.c_float(self.minimum_relative_change),
+                                                         ctypes.c_float(self.maximum_relative_change)),
+                      "mi_osem_set_options")
 
     def reconstruct(self, initial_image: np.ndarray) -> np.ndarray:
         proj = self.objective.projector
         if not proj.subsets_are_approximately_balanced():
             raise RuntimeError("OSMAPOSL cannot handle unbalanced subsets")
+        self._set_options()
         self.objective.set_estimate(initial_image, enforce_positivity=self.enforce_initial_positivity)
-        _native.check(_native.load().mi_osem_run(self.objective._ctx.handle, 1, self.num_subiterations,
-                                                 self.start_subset), "mi_osem_run")
+        _native.check(_native.load().mi_osem_run(self.objective._ctx.handle, self.start_subiteration_num,
+                                                 self.num_subiterations, self.start_subset), "mi_osem_run")
         return self.objective.get_estimate()
 
     def update_estimate(self, subset: int, subiteration_num: int) -> None:
+        self._set_options()
         _native.check(_native.load().mi_osem_subset_step(self.objective._ctx.handle, subset, subiteration_num),
                       "mi_osem_subset_step")

@@ -8,7 +8,8 @@ radius 350 mm, 8 subsets, 2 iterations, cylinder-and-spheres phantom with
 Poisson data (mean 5/bin + 0.1).
 
 Bars: batched images identical to the per-problem path (both sum in the same
-order), HIP-graph replays identical to eager launches, and one problem
+order) for every launch shape (row groups, lane groups, full waves, column
+tiles), HIP-graph replays identical to eager launches, and one problem
 within 1e-4 relative of the oracle's OSEM.
 """
 import numpy as np
@@ -84,8 +85,8 @@ def test_graph_replays_equal_eager(batch):
             assert np.array_equal(a, b), f"replay {rep}"
 
 
-@pytest.mark.parametrize("launch", [(32, True, True, 32, 8), (0, False, False, 1, 1), (16, False, True, 8, 8),
-                                    (64, False, True, 32, 8)])
+@pytest.mark.parametrize("launch", [(32, True, True, 32, 8, 0), (0, False, False, 1, 1, 0), (16, False, True, 8, 8, 16),
+                                    (64, False, True, 32, 8, 64), (32, False, False, 32, 8, 32)])
 def test_launch_shapes_do_not_change_results(batch, launch):
     B, problems, data = batch
     B.set_launch(32, False, True, 32, 8)
