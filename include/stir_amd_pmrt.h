@@ -194,6 +194,9 @@ int mi_obj_set_multiplicative(mi_ctx* ctx, const float* mult);
  * number of subsets (:374-455); subsets must be balanced. Takes effect at the
  * next mi_obj_compute_sensitivities. */
 int mi_obj_set_use_subset_sensitivities(mi_ctx* ctx, int use);
+/* Subset sensitivity (host image, STIR layout); subset -1: the total
+ * sensitivity, accumulated subset by subset (get_sensitivity,
+ * ...ForMean.cxx:147-153,419-455). */
 int mi_obj_get_sensitivity(mi_ctx* ctx, int subset, float* image);
 /* Current estimate (host, STIR layout). mi_obj_set_estimate applies
  * threshold_min_to_small_positive_value(x, 1e-6) when enforce_positivity

@@ -150,6 +150,7 @@ struct mi_ctx {
   // multiplicative factors given by the caller (device layout), and whether
   // every TOF bin carries the same factors
   DBuf<float> mult_user;
+  DBuf<float> sens_total;  // total sensitivity when subsets use total / num_subsets
   bool mult_tof_uniform = true;
   bool use_subset_sens = true;  // "use subset sensitivities"
   // OSMAPOSL: "MAP_model", "minimum/maximum relative change", the prior of the
@@ -927,6 +928,9 @@ int mi_obj_compute_sensitivities(mi_ctx* ctx) {
       // "use subset sensitivities" := 0 (...ForMean.cxx:386-455): the total,
       // accumulated subset by subset, divided by num_subsets for every subset
       for (int s = 1; s < nsub; ++s) mi::launch_accumulate(ctx->sens[0]->p, ctx->sens[s]->p, ctx->img.n, ctx->stream);
+      ctx->sens_total.alloc(ctx->img.n);
+      HIPCHECK(hipMemcpyAsync(ctx->sens_total.p, ctx->sens[0]->p, ctx->sens_total.bytes(), hipMemcpyDeviceToDevice,
+                              ctx->stream));
       mi::launch_divide_scalar(ctx->sens[0]->p, static_cast<float>(nsub), ctx->img.n, ctx->stream);
       for (int s = 1; s < nsub; ++s)
         HIPCHECK(hipMemcpyAsync(ctx->sens[s]->p, ctx->sens[0]->p, ctx->sens[0]->bytes(), hipMemcpyDeviceToDevice,
@@ -943,9 +947,21 @@ int mi_obj_get_sensitivity(mi_ctx* ctx, int subset, float* image) {
     if (!ctx || !image) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     ctx->require_setup();
     if (!ctx->have_sens) throw ApiError(MI_ERR_STATE, "sensitivities have not been computed");
-    if (subset < 0 || subset >= ctx->plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
+    if (subset < -1 || subset >= ctx->plan->num_subsets()) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
     HIPCHECK(hipSetDevice(ctx->device));
-    ctx->image_from_device(ctx->sens[subset]->p, ctx->stage_img.p, ctx->stream);
+    const float* src = nullptr;
+    if (subset >= 0) {
+      src = ctx->sens[subset]->p;
+    } else if (!ctx->use_subset_sens) {
+      src = ctx->sens_total.p;
+    } else {
+      // total = sens_0 + sens_1 + ... in subset order (...ForMean.cxx:419-438)
+      HIPCHECK(hipMemcpyAsync(ctx->img2.p, ctx->sens[0]->p, ctx->img2.bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+      for (size_t s = 1; s < ctx->sens.size(); ++s)
+        mi::launch_accumulate(ctx->img2.p, ctx->sens[s]->p, ctx->img2.n, ctx->stream);
+      src = ctx->img2.p;
+    }
+    ctx->image_from_device(src, ctx->stage_img.p, ctx->stream);
     HIPCHECK(hipMemcpyAsync(image, ctx->stage_img.p, ctx->stage_img.bytes(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
   });

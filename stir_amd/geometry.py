@@ -248,6 +248,15 @@ class ProjDataInfoCylindrical:
     def max_tangential_pos_num(self) -> int:
         return self.min_tangential_pos_num + self.num_tangential_poss - 1
 
+    def get_s(self, tang: int) -> float:
+        """Signed distance of a tangential position from the centre in mm:
+        tang * bin_size (arc-corrected, ``ProjDataInfoCylindricalArcCorr.inl:31-34``)
+        or R sin(tang * pi / num_detectors) (``ProjDataInfoCylindricalNoArcCorr.inl:79-82``)."""
+        if self.arc_corrected:
+            return float(np.float32(tang) * np.float32(self.bin_size))
+        dalpha = np.float32(math.pi / self.scanner.num_detectors_per_ring)
+        return float(np.float32(self.scanner.effective_ring_radius) * np.float32(math.sin(np.float32(tang) * dalpha)))
+
     def num_axial_poss_of(self, seg: int) -> int:
         return self.num_axial_poss[seg - self.min_segment_num]
 
@@ -349,9 +358,13 @@ class VoxelsOnCartesianGrid:
             z_size = (pdi.num_axial_poss[seg0] if pdi.max_ring_diff[seg0] > pdi.min_ring_diff[seg0]
                       else 2 * pdi.num_axial_poss[seg0] - 1)
         s_sampling = np.float32(pdi.scanner.default_bin_size)
+        if s_sampling <= 0:  # sampling in s of the central bin (VoxelsOnCartesianGrid.cxx:113-126)
+            s_sampling = np.float32((pdi.get_s(1) - pdi.get_s(-1)) / np.float32(2))
         vxy = s_sampling / np.float32(zoom)
         if xy_size < 0:
-            raise ValueError("explicit xy size required")
+            # cover the FOV: 2 ceil(R_fov / voxel) + 1 (VoxelsOnCartesianGrid.cxx:252-267)
+            r_fov = max(abs(pdi.get_s(pdi.max_tangential_pos_num)), abs(pdi.get_s(pdi.min_tangential_pos_num)))
+            xy_size = 2 * int(math.ceil(float(np.float32(r_fov) / vxy))) + 1
         return VoxelsOnCartesianGrid(z_size, xy_size, xy_size, float(z_sampling), float(vxy), float(vxy))
 
 

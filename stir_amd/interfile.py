@@ -45,10 +45,25 @@ def _norm_key(k: str) -> str:
 
 
 def _parse_value(v: str):
+    """Scalar, or (nested) list for {a, b} / {{a, b}, {c, d}}."""
     v = v.strip()
-    if v.startswith("{") and v.endswith("}"):
-        return [_parse_scalar(x) for x in v[1:-1].split(",") if x.strip()]
-    return _parse_scalar(v)
+    if not v.startswith("{"):
+        return _parse_scalar(v)
+    stack: List[list] = [[]]
+    tok = ""
+    for ch in v:
+        if ch == "{":
+            stack.append([])
+        elif ch in ",}":
+            if tok.strip():
+                stack[-1].append(_parse_scalar(tok))
+            tok = ""
+            if ch == "}":
+                inner = stack.pop()
+                stack[-1].append(inner)
+        else:
+            tok += ch
+    return stack[0][0] if stack[0] else []
 
 
 def _parse_scalar(v: str):
@@ -90,15 +105,20 @@ def parse_par(path_or_text: str) -> Dict[str, Any]:
     if os.path.exists(path_or_text):
         with open(path_or_text) as f:
             text = f.read()
+    lines = list(_lines(text))
+    # a key without value opens a block when a matching "End <key>" follows or
+    # its name ends in "parameters" (closed by "End ..." or the final "END")
+    closers = {k[4:] for k, v in lines if k.startswith("end ") and v == ""}
     root: Dict[str, Any] = {}
     stack: List[Dict[str, Any]] = [root]
-    for k, v in _lines(text):
-        if k.startswith("end") and v == "" and len(stack) > 1:
-            stack.pop()
-            continue
+    for k, v in lines:
         if k == "end":
             break
-        if v == "" and (k.endswith("parameters") or k.endswith("parameters :=")):
+        if k.startswith("end ") and v == "":
+            if len(stack) > 1:
+                stack.pop()
+            continue
+        if v == "" and (k in closers or k.endswith("parameters")):
             child: Dict[str, Any] = {}
             stack[-1][k] = child
             stack.append(child)
@@ -352,6 +372,23 @@ def read_proj_data(path: str) -> Tuple[ProjDataInfoCylindrical, np.ndarray]:
                 sino = chunk.reshape(nv, na, nt).transpose(1, 0, 2)
             out[k * block + offs[s]:k * block + offs[s] + na * nv * nt] = sino.reshape(-1)
     return pdi, out
+
+
+def restrict_proj_data(pdi: ProjDataInfoCylindrical, data: np.ndarray, max_abs_segment: int):
+    """(restricted geometry, its data): the segments |s| <= max_abs_segment of
+    flat data in ProjDataInMemory order ("maximum absolute segment number to
+    process")."""
+    sub = pdi.restrict_segments(max_abs_segment)
+    if sub == pdi:
+        return pdi, data
+    full = np.asarray(data, np.float32).reshape(pdi.num_tof_bins, -1)
+    offs, subo = pdi.segment_offsets(), sub.segment_offsets()
+    out = np.empty((sub.num_tof_bins, sub.num_bins // sub.num_tof_bins), np.float32)
+    vt = pdi.num_views * pdi.num_tangential_poss
+    for seg in sub.segments:
+        n = pdi.num_axial_poss_of(seg) * vt
+        out[:, subo[seg]:subo[seg] + n] = full[:, offs[seg]:offs[seg] + n]
+    return sub, out.reshape(-1)
 
 
 def write_proj_data(path: str, pdi: ProjDataInfoCylindrical, data: np.ndarray) -> None:

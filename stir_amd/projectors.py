@@ -280,6 +280,11 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
         _native.check(_native.load().mi_obj_set_prior(self._ctx.handle, ctypes.byref(st)), "mi_obj_set_prior")
         del keep
 
+    def get_sensitivity(self) -> np.ndarray:
+        """Total sensitivity: the subset sensitivities accumulated in subset
+        order (``get_sensitivity``, ``...ForMean.cxx:147-153,419-455``)."""
+        return self.get_subset_sensitivity(-1)
+
     def get_subset_sensitivity(self, subset: int) -> np.ndarray:
         out = np.empty(self.projector.img.shape, np.float32)
         _native.check(_native.load().mi_obj_get_sensitivity(self._ctx.handle, subset, out.reshape(-1)),

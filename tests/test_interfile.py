@@ -222,3 +222,30 @@ data rescale slope := 0.5
     img, data = I.read_image(str(tmp_path / "s.hv"))
     assert img.shape == (2, 3, 4) and (img.vox_x, img.vox_z) == (2.0, 3.0)
     assert np.array_equal(data, vals.astype(np.float32) * np.float32(0.5))
+
+
+def test_par_nesting_of_blocks_closed_by_end_keys():
+    """Blocks whose start key does not end in "parameters" ("Bin Normalisation
+    From ProjData") are closed by their own End key, so later keys stay in the
+    enclosing block."""
+    from tests.test_gpu_par import PAR
+    t = I.parse_par(PAR)
+    obj = t["osmaposlparameters"]["poissonloglikelihoodwithlinearmodelformeanandprojdata parameters"]
+    assert obj["bin normalisation from projdata"] == {"normalisation projdata filename": "norm.hs"}
+    assert obj["projector pair type"] == "Separate Projectors"
+    assert obj["quadratic prior parameters"] == {"penalisation factor": 0.02, "kappa filename": "kappa.hv"}
+    assert t["osmaposlparameters"]["map model"] == "multiplicative"
+    assert I.find_key(t, "forward projector type") == "MI355X Ray Tracing"
+
+
+def test_nested_list_values():
+    assert I._parse_value("{{{0,1,0},{1,0,1},{0,1,0}}}") == [[[0, 1, 0], [1, 0, 1], [0, 1, 0]]]
+    assert I._parse_value("{ 12,13,14}") == [12, 13, 14]
+
+
+def test_default_image_size_covers_the_fov():
+    """xy size -1: 2 ceil(R_fov / voxel) + 1 (VoxelsOnCartesianGrid.cxx:252-267):
+    mMR 344 tangential positions at R = 328 mm reach 288.0 mm -> 2*139+1."""
+    from stir_amd.geometry import VoxelsOnCartesianGrid
+    pdi = config1()[0]
+    assert VoxelsOnCartesianGrid.from_proj_data_info(pdi).shape == (127, 279, 279)
