@@ -320,9 +320,11 @@ int mi_batch_set_data(mi_batch* batch, int problem, const float* measured, const
 int mi_batch_set_estimate(mi_batch* batch, int problem, const float* image, int enforce_positivity);
 int mi_batch_get_estimate(mi_batch* batch, int problem, float* image);
 /* Launch shape: forward tasks of at most fp_group_max bins run in lane
- * groups (the others one per wave), fp_widen 1 doubles those groups, bp_pack 0
- * gives every column a full wave, tile_x x tile_y orders the columns.
- * Defaults 32, 0, 1, 32, 8. */
+ * groups (the others one per wave; row groups, below, take precedence),
+ * fp_widen 1 doubles those groups; columns of images of at most 32 planes run
+ * one per full wave (bp_pack 0), packed in lane groups (1), or one per wave
+ * split into 2 or 4 parts that sum different table entries (2); tile_x x
+ * tile_y orders the columns. Defaults 32, 0, 2, 32, 8. */
 int mi_batch_set_launch(mi_batch* batch, int fp_group_max, int fp_widen, int bp_pack, int tile_x, int tile_y);
 /* Forward projection in row groups: the families of one basic row with at
  * most max_axial axial positions (0: none) run up to 64 axial positions per

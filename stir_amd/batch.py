@@ -75,11 +75,12 @@ class BatchedOSMAPOSL:
         _native.check(_native.load().mi_batch_get_estimate(self._h, i, out.reshape(-1)), "mi_batch_get_estimate")
         return out
 
-    def set_launch(self, fp_group_max: int = 32, fp_widen: bool = False, bp_pack: bool = False, tile_x: int = 32,
+    def set_launch(self, fp_group_max: int = 32, fp_widen: bool = False, bp_pack: int = 2, tile_x: int = 32,
                    tile_y: int = 8, row_groups: int = 32) -> None:
         """row_groups: families of at most this many axial positions run in row
         groups (0: off); the others in lane groups of fp_group_max bins or
-        full waves; bp_pack: columns of short images in lane groups."""
+        full waves; bp_pack: columns of images of at most 32 planes in full
+        waves (0), lane groups (1) or waves split over table entries (2)."""
         L = _native.load()
         _native.check(L.mi_batch_set_launch(self._h, fp_group_max, int(fp_widen), int(bp_pack), tile_x, tile_y),
                       "mi_batch_set_launch")

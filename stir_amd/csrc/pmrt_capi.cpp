@@ -1393,7 +1393,8 @@ struct mi_batch {
   // launch choices (mi_batch_tune)
   int fp_widen = 0;             // forward groups: 0 = smallest group size holding the family, 1 = twice that
   int fp_group_max = 32;        // forward tasks with at most this many bins go to lane groups, the rest to full waves
-  int bp_pack = 0;              // back projection: columns of at most 32 planes in lane groups (1) or full waves (0)
+  int bp_pack = 2;              // back projection of images of at most 32 planes: full waves (0), lane groups (1),
+                                // waves split into parts over table entries (2)
   int tile_x = 32, tile_y = 8;  // back-projection column order
   int fp_rowgroup_max = 32;     // families of at most this many axial positions run in row groups (0: off)
   // device descriptors and work lists
@@ -1409,8 +1410,9 @@ struct mi_batch {
   DBuf<const mi::FpTask*> task_lists;
   std::vector<std::array<int, 4>> fpw_n;
   std::vector<std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 4>> fpw_work;
-  std::array<int, 12> bpw_n{};
-  std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 12> bpw_work;
+  // [ppa class * 6 + variant], variants: 4 parts, 2 parts, 1..4 slots
+  std::array<int, 18> bpw_n{};
+  std::array<std::unique_ptr<DBuf<mi::WorkItem>>, 18> bpw_work;
   // forward row groups per subset: items and their family lists
   std::vector<int> rg_n;
   std::vector<std::unique_ptr<DBuf<mi::RowGroupItem>>> rg_items;
@@ -1578,13 +1580,16 @@ void batch_prepare(mi_batch* b) {
   // back-projection work: the columns of every problem in tile order; images
   // of at most 32 planes in lane groups (when packing), the others one column per wave
   std::array<std::vector<mi::WorkItem>, 4> lists;
-  std::array<std::vector<mi::WorkItem>, 12> wide;
+  std::array<std::vector<mi::WorkItem>, 18> wide;
   for (int p = 0; p < np; ++p) {
     const mi::ImageDims& d = b->probs[p]->dims;
     const int gi = group_index_at_least(d.nzp);
-    const bool grouped = b->bp_pack && gi >= 0 && gi <= 2;
+    const bool grouped = b->bp_pack == 1 && gi >= 0 && gi <= 2;
     const int ppa = b->probs[p]->uniform_ppa;
-    const int wclass = (ppa == 1 || ppa == 2 ? ppa : 0) * 4 + std::min((d.nzp + 63) / 64, 4) - 1;
+    int variant = 1 + std::min((d.nzp + 63) / 64, 4);  // slots
+    if (b->bp_pack == 2 && d.nzp <= 16) variant = 0;
+    else if (b->bp_pack == 2 && d.nzp <= 32) variant = 1;
+    const int wclass = (ppa == 1 || ppa == 2 ? ppa : 0) * 6 + variant;
     const int tx = std::max(b->tile_x, 1), ty = std::max(b->tile_y, 1);
     for (int y0 = 0; y0 < d.ny; y0 += ty)
       for (int x0 = 0; x0 < d.nx; x0 += tx)
@@ -1600,7 +1605,7 @@ void batch_prepare(mi_batch* b) {
     b->bp_n[gi] = static_cast<int>(lists[gi].size());
     b->bp_work[gi] = upload(lists[gi]);
   }
-  for (int k = 0; k < 12; ++k) {
+  for (int k = 0; k < 18; ++k) {
     b->bpw_n[k] = static_cast<int>(wide[k].size());
     b->bpw_work[k] = upload(wide[k]);
   }
@@ -1615,8 +1620,10 @@ void batch_fp(mi_batch* b, mi::FpMode mode, int s) {
     mi::launch_fp_group(mode, kGroupSizes[gi], b->descs.p, b->fp_work[s][gi]->p, b->fp_n[s][gi], b->stream);
 }
 void batch_bp(mi_batch* b, mi::BpMode mode, int s, int threshold) {
-  for (int k = 0; k < 12; ++k)
-    mi::launch_bp_batch(mode, k / 4, k % 4 + 1, b->descs.p, b->bpw_work[k]->p, b->bpw_n[k], s, threshold, b->stream);
+  static const int kVariantSlots[6] = {-4, -2, 1, 2, 3, 4};
+  for (int k = 0; k < 18; ++k)
+    mi::launch_bp_batch(mode, k / 6, kVariantSlots[k % 6], b->descs.p, b->bpw_work[k]->p, b->bpw_n[k], s, threshold,
+                        b->stream);
   for (int gi = 1; gi < 4; ++gi)
     mi::launch_bp_group(mode, kGroupSizes[gi], b->descs.p, b->bp_work[gi]->p, b->bp_n[gi], s, threshold, b->stream);
 }
@@ -1699,7 +1706,8 @@ int mi_batch_set_launch(mi_batch* b, int fp_group_max, int fp_widen, int bp_pack
     if (fp_group_max < 0 || fp_group_max > 64) throw ApiError(MI_ERR_ARGUMENT, "fp_group_max must be in [0, 64]");
     b->fp_group_max = fp_group_max;
     b->fp_widen = fp_widen ? 1 : 0;
-    b->bp_pack = bp_pack ? 1 : 0;
+    if (bp_pack < 0 || bp_pack > 2) throw ApiError(MI_ERR_ARGUMENT, "bp_pack must be 0, 1 or 2");
+    b->bp_pack = bp_pack;
     b->tile_x = std::max(tile_x, 1);
     b->tile_y = std::max(tile_y, 1);
     batch_invalidate(b);
@@ -1720,8 +1728,8 @@ int mi_batch_tune(mi_batch* b, int* choice) {
     if (!b) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     HIPCHECK(hipSetDevice(b->device));
     struct Cand { int rg, gmax, widen, pack, tx, ty; };
-    const Cand cands[] = {{32, 32, 0, 0, 32, 8}, {32, 32, 0, 1, 32, 8}, {16, 32, 0, 0, 32, 8}, {0, 32, 0, 0, 32, 8},
-                          {0, 0, 0, 0, 32, 8},   {32, 32, 0, 0, 8, 8},  {32, 32, 0, 0, 64, 4}, {32, 32, 0, 0, 1, 1}};
+    const Cand cands[] = {{32, 32, 0, 2, 32, 8}, {32, 32, 0, 0, 32, 8}, {16, 32, 0, 2, 32, 8}, {0, 32, 0, 2, 32, 8},
+                          {0, 0, 0, 0, 32, 8},   {32, 32, 0, 2, 8, 8},  {32, 32, 0, 2, 64, 4}, {32, 32, 0, 1, 32, 8}};
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));

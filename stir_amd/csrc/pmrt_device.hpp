@@ -184,7 +184,8 @@ void launch_bp_group(BpMode mode, int G, const ProblemDesc* d_descs, const WorkI
 // forward items index each problem's task list (± pairs) and every task of a
 // launch has at most 64 * nslot bins; back-projection items are columns of
 // images with at most 64 * nslot planes and `ppa` planes per axial position
-// (1, 2, or 0 for any).
+// (1, 2, or 0 for any); nslot = -2 or -4: images of at most 32 or 16 planes,
+// a wave split into that many parts working on different table entries.
 void launch_fp_batch(FpMode mode, int nslot, const ProblemDesc* d_descs, const FpTask* const* d_task_lists,
                      const WorkItem* d_work, int nwork, hipStream_t stream);
 void launch_bp_batch(BpMode mode, int ppa, int nslot, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork,

@@ -110,9 +110,19 @@ __device__ inline unsigned entry_axial(int k, int nax_ppa, int zi, bool& ok) {
     a = diff >> 1;
     ok = (diff & 1) == 0 && static_cast<unsigned>(a) < static_cast<unsigned>(nax);
   } else {
+    // families of one table mix 1 and 2 planes per axial position (span > 1
+    // with a last segment of ring difference span 1): no division for those
     const int ppa = nax_ppa >> 16;
-    a = diff / ppa;
-    ok = diff >= 0 && a * ppa == diff && a < nax;
+    if (ppa == 1) {
+      a = diff;
+      ok = static_cast<unsigned>(a) < static_cast<unsigned>(nax);
+    } else if (ppa == 2) {
+      a = diff >> 1;
+      ok = (diff & 1) == 0 && static_cast<unsigned>(a) < static_cast<unsigned>(nax);
+    } else {
+      a = diff / ppa;
+      ok = diff >= 0 && a * ppa == diff && a < nax;
+    }
   }
   return static_cast<unsigned>(min(max(a, 0), nax - 1));
 }

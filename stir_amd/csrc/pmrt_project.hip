@@ -222,6 +222,80 @@ __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> 
   }
 }
 
+// Back projection of one short image column (nzp <= 64 / NPART planes) with
+// the wave split into NPART parts of 64 / NPART lanes: part p takes the
+// entries p, p + NPART, ... of the column's list (wave-uniform loads, each
+// lane selecting its part's entry), so a column of 16 planes keeps all 64
+// lanes busy. Each part sums its entries in table order; the parts' sums are
+// added in part order (a fixed order, not the single-accumulator order of
+// bp_column).
+template <int MODE, int PPA, int NPART>
+__device__ inline void bp_column_parts(int col, cptr<int64_t> list_off, cptr<BpEntry> ent, int sb, int se,
+                                       cptr<float> proj, float* __restrict__ img, const ImageDims& d,
+                                       const BpData& data, int lane) {
+  constexpr int ZL = kWave / NPART;
+  constexpr int U = kUnroll / NPART > 0 ? kUnroll / NPART : 1;  // entry groups in flight
+  const int part = lane / ZL;
+  const int zi = d.min_z + lane % ZL;
+  float acc = 0.F;
+  for (int s = sb; s < se; ++s) {
+    const int64_t beg = list_off[static_cast<int64_t>(s) * d.ncols + col];
+    const int64_t end = list_off[static_cast<int64_t>(s) * d.ncols + col + 1];
+    int64_t e = beg;
+    for (; e + NPART * U <= end; e += NPART * U) {
+      float p[U], wt[U];
+      bool ok[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        BpEntry E = ent[e + j * NPART];
+#pragma unroll
+        for (int q = 1; q < NPART; ++q) {
+          const BpEntry Eq = ent[e + j * NPART + q];
+          if (part == q) E = Eq;
+        }
+        wt[j] = E.w;
+        const unsigned a = entry_axial<PPA>(E.k, E.nax_ppa, zi, ok[j]);
+        p[j] = proj[static_cast<int64_t>(E.sino_base) + a];
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        if (ok[j]) acc = acc + wt[j] * p[j];
+    }
+    for (; e < end; e += NPART) {
+      const int64_t n = end - e;  // entries left (wave-uniform)
+      BpEntry E = ent[e];
+#pragma unroll
+      for (int q = 1; q < NPART; ++q) {
+        if (q < n) {
+          const BpEntry Eq = ent[e + q];
+          if (part == q) E = Eq;
+        }
+      }
+      bool ok;
+      const unsigned a = entry_axial<PPA>(E.k, E.nax_ppa, zi, ok);
+      const float pv = proj[static_cast<int64_t>(E.sino_base) + a];
+      if (ok && part < n) acc = acc + E.w * pv;
+    }
+  }
+  // parts' sums in part order, on the lanes of part 0
+  float total = acc;
+#pragma unroll
+  for (int q = 1; q < NPART; ++q) total = total + __shfl(acc, lane + q * ZL, kWave);
+  const int z = lane;
+  if (part != 0 || z >= d.nzp) return;
+  gptr<float> out = as_global(img) + static_cast<int64_t>(col) * d.nzp;
+  if constexpr (MODE == BP_PLAIN) {
+    out[z] = z < d.nz ? total : 0.F;
+  } else {
+    if (z < d.nz) {
+      const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
+      const float u = osem_factor(total, as_const(data.sens)[idx], data.threshold, data.thr_lo, data.thr_hi);
+      as_global(data.estimate)[idx] = as_global(data.estimate)[idx] * u;
+      if (data.update) as_global(data.update)[idx] = u;
+    }
+  }
+}
+
 template <int MODE, int PPA, int NSLOT, bool SKIP>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t* __restrict__ list_off,
                                                                    const BpEntry* __restrict__ ent, int sb, int se,
@@ -267,7 +341,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_batch_kernel(const P
 // back projection: work item = (problem, image column), subset `subset`;
 // every image of the launch has at most 64 * NSLOT planes and PPA planes per
 // axial position (0: any)
-template <int MODE, int PPA, int NSLOT>
+template <int MODE, int PPA, int NSLOT, int NPART>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const ProblemDesc* __restrict__ descs,
                                                                          const WorkItem* __restrict__ work, int nwork,
                                                                          int subset, int threshold) {
@@ -284,8 +358,12 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const P
   bd.estimate = P.estimate;
   bd.update = nullptr;
   bd.threshold = threshold;
-  bp_column<MODE, PPA, NSLOT, false>(col, as_const(P.list_off), as_const(P.entries), subset, subset + 1,
-                                     as_const(P.bp_in), P.bp_out, P.d, bd, lane);
+  if constexpr (NPART > 1)
+    bp_column_parts<MODE, PPA, NPART>(col, as_const(P.list_off), as_const(P.entries), subset, subset + 1,
+                                      as_const(P.bp_in), P.bp_out, P.d, bd, lane);
+  else
+    bp_column<MODE, PPA, NSLOT, false>(col, as_const(P.list_off), as_const(P.entries), subset, subset + 1,
+                                       as_const(P.bp_in), P.bp_out, P.d, bd, lane);
 }
 
 // dst[c * dst_ld + r] = src[r * src_ld + c], 32x32 tiles through LDS
@@ -421,14 +499,17 @@ void launch_bp_batch(BpMode mode, int ppa, int nslot, const ProblemDesc* d_descs
                      int subset, int threshold, hipStream_t stream) {
   if (nwork <= 0) return;
   const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
-#define MI_BPB(M, P, S)                                                                                         \
-  hipLaunchKernelGGL((bp_batch_kernel<M, P, S>), grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold)
-#define MI_BPB_S(M, P)              \
-  switch (nslot) {                  \
-    case 1: MI_BPB(M, P, 1); break; \
-    case 2: MI_BPB(M, P, 2); break; \
-    case 3: MI_BPB(M, P, 3); break; \
-    default: MI_BPB(M, P, 4); break; \
+#define MI_BPB(M, P, S, NP)                                                                                         \
+  hipLaunchKernelGGL((bp_batch_kernel<M, P, S, NP>), grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold)
+  // nslot < 0: one slot, -nslot parts
+#define MI_BPB_S(M, P)                 \
+  switch (nslot) {                     \
+    case -4: MI_BPB(M, P, 1, 4); break; \
+    case -2: MI_BPB(M, P, 1, 2); break; \
+    case 1: MI_BPB(M, P, 1, 1); break;  \
+    case 2: MI_BPB(M, P, 2, 1); break;  \
+    case 3: MI_BPB(M, P, 3, 1); break;  \
+    default: MI_BPB(M, P, 4, 1); break; \
   }
 #define MI_BPB_P(M)                  \
   if (ppa == 1) { MI_BPB_S(M, 1) }   \

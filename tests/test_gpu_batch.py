@@ -8,9 +8,11 @@ radius 350 mm, 8 subsets, 2 iterations, cylinder-and-spheres phantom with
 Poisson data (mean 5/bin + 0.1).
 
 Bars: batched images identical to the per-problem path (both sum in the same
-order) for every launch shape (row groups, lane groups, full waves, column
-tiles), HIP-graph replays identical to eager launches, and one problem
-within 1e-4 relative of the oracle's OSEM.
+order) for every launch shape that keeps the table order (row groups, lane
+groups, full waves, column tiles), within 1e-5 relative for back projections
+split over table entries (parts summed in a fixed part order), HIP-graph
+replays identical to eager launches, and one problem within 1e-4 relative of
+the oracle's OSEM.
 """
 import numpy as np
 import pytest
@@ -59,21 +61,39 @@ def _images(B):
     return [B.get_estimate(i) for i in range(len(B))]
 
 
+def _max_rel(a, b):
+    m = a > 1e-3 * a.max()
+    return float((np.abs(a - b)[m] / a[m]).max())
+
+
 def test_batched_equals_per_problem(batch):
+    """Full waves and lane groups sum every voxel in table order, as the
+    per-problem kernels: identical images."""
     B, problems, data = batch
     _reset(B, problems)
     B.run_sequential(2 * NSUB)
     seq = _images(B)
+    for bp_pack in (0, 1):
+        B.set_launch(32, False, bp_pack, 32, 8)
+        _reset(B, problems)
+        B.run(2 * NSUB, use_graph=False)
+        bat = _images(B)
+        for i, (a, b) in enumerate(zip(seq, bat)):
+            print(f"bp_pack {bp_pack} problem {i}: max |batched - per-problem| {np.abs(a - b).max():.3e}")
+            assert np.array_equal(a, b)
+    # waves split over table entries: parts summed in part order
+    B.set_launch()
     _reset(B, problems)
     B.run(2 * NSUB, use_graph=False)
-    bat = _images(B)
-    for i, (a, b) in enumerate(zip(seq, bat)):
-        print(f"problem {i}: max |batched - per-problem| {np.abs(a - b).max():.3e} (max {a.max():.3e})")
-        assert np.array_equal(a, b)
+    for i, (a, b) in enumerate(zip(seq, _images(B))):
+        rel = _max_rel(a, b)
+        print(f"split waves problem {i}: max rel {rel:.2e}")
+        assert rel <= 1e-5
 
 
 def test_graph_replays_equal_eager(batch):
     B, problems, data = batch
+    B.set_launch()
     _reset(B, problems)
     B.run(2 * NSUB, use_graph=False)
     eager = _images(B)
@@ -85,11 +105,13 @@ def test_graph_replays_equal_eager(batch):
             assert np.array_equal(a, b), f"replay {rep}"
 
 
-@pytest.mark.parametrize("launch", [(32, True, True, 32, 8, 0), (0, False, False, 1, 1, 0), (16, False, True, 8, 8, 16),
-                                    (64, False, True, 32, 8, 64), (32, False, False, 32, 8, 32)])
+@pytest.mark.parametrize("launch", [(32, True, 1, 32, 8, 0), (0, False, 0, 1, 1, 0), (16, False, 1, 8, 8, 16),
+                                    (64, False, 1, 32, 8, 64), (32, False, 0, 32, 8, 32), (32, False, 2, 32, 8, 32),
+                                    (0, False, 2, 8, 8, 0)])
 def test_launch_shapes_do_not_change_results(batch, launch):
+    """Exact for shapes that keep the table order, 1e-5 for split waves."""
     B, problems, data = batch
-    B.set_launch(32, False, True, 32, 8)
+    B.set_launch(32, False, 0, 32, 8)
     _reset(B, problems)
     B.run(NSUB, use_graph=False)
     ref = _images(B)
@@ -97,9 +119,12 @@ def test_launch_shapes_do_not_change_results(batch, launch):
     _reset(B, problems)
     B.run(NSUB, use_graph=False)
     got = _images(B)
-    B.set_launch(32, False, True, 32, 8)
+    B.set_launch()
     for a, b in zip(ref, got):
-        assert np.array_equal(a, b)
+        if launch[2] == 2:
+            assert _max_rel(a, b) <= 1e-5
+        else:
+            assert np.array_equal(a, b)
 
 
 def test_tuner_and_oracle(batch):
