@@ -189,6 +189,23 @@ int mi_obj_set_use_tof_sensitivities(mi_ctx* ctx, int use);
  * sinograms. With TOF data and non-TOF sensitivities every TOF bin must carry
  * the same factors (:636-646). Sensitivities must be recomputed. */
 int mi_obj_set_multiplicative(mi_ctx* ctx, const float* mult);
+/* List-mode data of PoissonLogLikelihoodWithLinearModelForMeanAndListModeData-
+ * WithProjMatrixByBin: prompt events as num_events x 5 int32 (segment, view,
+ * axial, tangential position, timing position; timing 0 without TOF); events
+ * outside the projection data are dropped (...ListModeDataWithProjMatrixByBin.cxx:
+ * 495-511). additive: host projection data (ProjDataInMemory order) or NULL
+ * for additive_constant; with TOF data every event gets the additive term of
+ * the last timing position, as the reference's event cache (:570-596).
+ * Afterwards mi_obj_compute_sensitivities (BP of the multiplicative factors,
+ * no end-plane zeroing), mi_osem_*, mi_obj_subset_gradient (sum over the
+ * subset's events of row^T / fwd, minus the subset sensitivity unless
+ * add_sensitivity) and mi_obj_log_likelihood (the value the reference's
+ * actual_compute_objective_function_without_penalty returns: -sum over events
+ * of log fwd, log(1e-4) for capped events, :748-787,874-911) work on the
+ * events: fwd = row . x + additive, events with 1 > 1e4 fwd contribute no
+ * gradient. mi_obj_set_data switches back to projection data. */
+int mi_lm_set_events(mi_ctx* ctx, const int32_t* events, int64_t num_events, const float* additive,
+                     float additive_constant);
 /* "use subset sensitivities" (default 1, ...ForMean.cxx:48). 0: every subset
  * uses the total sensitivity (accumulated subset by subset) divided by the
  * number of subsets (:374-455); subsets must be balanced. Takes effect at the

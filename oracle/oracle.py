@@ -78,6 +78,9 @@ def lib():
         L.or_relative_difference_prior.restype = ctypes.c_double
         L.or_relative_difference_prior.argtypes = [_f32p, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p,
                                                    ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]
+        L.or_lm_gradient.restype = ctypes.c_double
+        L.or_lm_gradient.argtypes = [vp, _i32p, ctypes.c_int64, vp, _f32p, ctypes.c_int, ctypes.c_int, vp]
+        L.or_lm_osem.argtypes = [vp, _i32p, ctypes.c_int64, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_set_multiplicative.argtypes = [vp, vp]
         L.or_set_use_subset_sensitivities.argtypes = [vp, ctypes.c_int]
         L.or_set_osem_options.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float]
@@ -261,6 +264,26 @@ class OraclePlan:
 
     def set_use_tof_sensitivities(self, use: bool) -> None:
         lib().or_set_use_tof_sensitivities(self._p, int(use))
+
+    def lm_gradient(self, events, x, subset=0, num_subsets=1, additive=None, gradient=True):
+        """(value, gradient) of the list-mode objective's subset: events
+        int32 n x 5 (segment, view, axial, tangential, timing)."""
+        ev = np.ascontiguousarray(events, dtype=np.int32).reshape(-1, 5)
+        x = self._img(x)
+        add = None if additive is None else self._sino(additive)
+        g = np.zeros(self.img.shape, np.float32) if gradient else None
+        v = lib().or_lm_gradient(self._p, ev, ev.shape[0], None if add is None else add.ctypes.data_as(ctypes.c_void_p),
+                                 x.reshape(-1), subset, num_subsets,
+                                 None if g is None else g.ctypes.data_as(ctypes.c_void_p))
+        return float(v), g
+
+    def lm_osem(self, events, x0, num_subsets, num_subiterations, additive=None, start_subset=0):
+        ev = np.ascontiguousarray(events, dtype=np.int32).reshape(-1, 5)
+        x = np.array(self._img(x0), dtype=np.float32, copy=True)
+        add = None if additive is None else self._sino(additive)
+        lib().or_lm_osem(self._p, ev, ev.shape[0], None if add is None else add.ctypes.data_as(ctypes.c_void_p),
+                         x.reshape(-1), num_subsets, num_subiterations, start_subset)
+        return x
 
     def set_multiplicative(self, mult) -> None:
         """Multiplicative factors (kept alive here) or None."""

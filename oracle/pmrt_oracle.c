@@ -1653,3 +1653,94 @@ double or_relative_difference_prior(const float* img, const float* kappa, int nz
   }
   return result * beta;
 }
+
+/* ------------------------------------------------------------------ */
+/* List mode (PoissonLogLikelihoodWithLinearModelForMeanAndListModeData */
+/* WithProjMatrixByBin)                                                 */
+/* ------------------------------------------------------------------ */
+/* Whether an event's bin lies in the projection data (the cache keeps only
+ * those, ...ListModeDataWithProjMatrixByBin.cxx:500-511). */
+static int lm_event_in_range(const or_plan* p, const int32_t* e)
+{
+  const int seg = e[0], view = e[1], ax = e[2], tang = e[3], tof = e[4];
+  if (seg < p->g.min_segment_num || seg > p->g.max_segment_num) return 0;
+  if (view < 0 || view >= p->g.num_views) return 0;
+  if (tang < p->g.min_tangential_pos_num || tang > p->max_tang) return 0;
+  if (ax < 0 || ax >= p->g.num_axial_poss[SEGI(p, seg)]) return 0;
+  const int min_tof = -(p->ntof / 2);
+  if (p->ntof > 1 ? (tof < min_tof || tof > min_tof + p->ntof - 1) : tof != 0) return 0;
+  return 1;
+}
+
+/* LM_gradient_and_value over the prompt events of one subset, in event order
+ * (distributable.txx:99-140 with ...ListModeDataWithProjMatrixByBin.cxx:748-787):
+ * events n x (segment, view, axial, tangential, timing position); an event is
+ * in subset s when its basic bin's view % num_subsets == s (num_subsets > 1);
+ * fwd = row . x + additive (the additive term of the event's bin; for TOF
+ * data the cache keeps that of the last timing position, .cxx:570-596);
+ * 1 > 1e4 fwd: value -= log(1/1e4) and no gradient; otherwise grad += row^T
+ * (1/fwd) and value -= log(fwd). add may be NULL (no additive term). grad
+ * (if not NULL) is accumulated into. Returns the value. */
+double or_lm_gradient(const or_plan* p, const int32_t* ev, int64_t n, const float* add, const float* x, int subset,
+                      int nsub, float* grad)
+{
+  const float max_quotient = 10000.F;
+  double value = 0.;
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* e = ev + 5 * i;
+    if (!lm_event_in_range(p, e)) continue;
+    const int seg = e[0], view = e[1], ax = e[2], tang = e[3];
+    const int kk = p->ntof > 1 ? e[4] + p->ntof / 2 : 0;
+    if (nsub > 1) {
+      int bs = seg, bv = view, ba = ax, bt = tang;
+      (void)sym_op_for_bin(p, &bs, &bv, &ba, &bt);
+      if (bv % nsub != subset) continue;
+    }
+    const float add_term = add ? add[tbin(p, seg, view, ax, tang, p->ntof - 1)] : 0.F;
+    const float fwd = fp_bin(p, x, seg, view, ax, tang, p->ntof > 1 ? kk : -1) + add_term;
+    const float num = 1.F;
+    if (num > max_quotient * fwd) {
+      value -= num * log((double)(num / max_quotient));
+      continue;
+    }
+    if (grad) bp_bin(p, grad, num / fwd, seg, view, ax, tang, p->ntof > 1 ? kk : -1);
+    value -= num * log((double)fwd);
+  }
+  return value;
+}
+
+/* OSMAPOSL on list-mode data, prior-free: subset sensitivities BP(m or 1)
+ * without end-plane zeroing (add_subset_sensitivity, .cxx:661-720), update
+ * x *= divide(sum over the subset's events of row^T/fwd, sens_s, 0),
+ * thresholded except in subiteration 1. */
+void or_lm_osem(const or_plan* p, const int32_t* ev, int64_t n, const float* add, float* x, int nsub, int nsubiter,
+                int start_subset)
+{
+  const size_t nv = (size_t)p->g.nz * p->g.ny * p->g.nx;
+  float** sens = (float**)malloc(sizeof(float*) * (size_t)nsub);
+  for (int s = 0; s < nsub; ++s) {
+    sens[s] = (float*)malloc(sizeof(float) * nv);
+    or_subset_sensitivity(p, s, nsub, 0, sens[s]);
+  }
+  float* u = (float*)malloc(sizeof(float) * nv);
+  threshold_min_to_small_positive(x, nv, 0.000001F);
+  for (int k = 1; k <= nsubiter; ++k) {
+    const int subset = (k + start_subset - 1) % nsub;
+    memset(u, 0, sizeof(float) * nv);
+    (void)or_lm_gradient(p, ev, n, add, x, subset, nsub, u);
+    for (size_t i = 0; i < nv; ++i) {
+      if (fabsf(sens[subset][i]) <= 0.F && fabsf(u[i]) <= 0.F) u[i] = 0;
+      else u[i] /= sens[subset][i];
+    }
+    if (k != 1)
+      for (size_t i = 0; i < nv; ++i) {
+        if (u[i] > p->max_rel) u[i] = p->max_rel;
+        else if (p->min_rel > u[i]) u[i] = p->min_rel;
+      }
+    for (size_t i = 0; i < nv; ++i) x[i] *= u[i];
+  }
+  free(u);
+  for (int s = 0; s < nsub; ++s) free(sens[s]);
+  free(sens);
+}
+

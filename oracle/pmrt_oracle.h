@@ -195,6 +195,16 @@ double or_quadratic_prior(const float* img, const float* kappa, int nz, int ny, 
 double or_relative_difference_prior(const float* img, const float* kappa, int nz, int ny, int nx, const float* w27,
                                     int only_2D, float beta, float gamma, float eps, float* grad);
 
+/* List mode: events n x (segment, view, axial, tangential, timing position).
+ * or_lm_gradient accumulates sum over the subset's events of row^T (1/fwd)
+ * into grad (if not NULL) and returns the reference's accumulated value
+ * -sum log(fwd) (log(1/1e4) where 1 > 1e4 fwd); or_lm_osem runs prior-free
+ * OSMAPOSL on the events (...ListModeDataWithProjMatrixByBin.cxx). */
+double or_lm_gradient(const or_plan* p, const int32_t* ev, int64_t n, const float* add, const float* x, int subset,
+                      int num_subsets, float* grad);
+void or_lm_osem(const or_plan* p, const int32_t* ev, int64_t n, const float* add, float* x, int num_subsets,
+                int num_subiterations, int start_subset);
+
 #ifdef __cplusplus
 }
 #endif

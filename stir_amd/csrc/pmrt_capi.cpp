@@ -151,6 +151,8 @@ struct mi_ctx {
   // every TOF bin carries the same factors
   DBuf<float> mult_user;
   DBuf<float> sens_total;  // total sensitivity when subsets use total / num_subsets
+  bool lm = false;         // the data are histogrammed list-mode events (mi_lm_set_events)
+  mi::FpMode ratio_mode() const { return lm ? mi::FP_LM_RATIO : mi::FP_RATIO; }
   bool mult_tof_uniform = true;
   bool use_subset_sens = true;  // "use subset sensitivities"
   // OSMAPOSL: "MAP_model", "minimum/maximum relative change", the prior of the
@@ -844,6 +846,7 @@ int mi_obj_set_data(mi_ctx* ctx, const float* measured, const float* additive, f
     }
     ctx->add_const = additive_constant;
     ctx->zero_end_planes = zero_end_planes;
+    ctx->lm = false;
     ctx->ones.alloc(nb);
     mi::launch_fill(ctx->ones.p, nb, 1.F, st);
     if (zero_end_planes) {
@@ -1012,7 +1015,7 @@ int mi_obj_get_estimate(mi_ctx* ctx, float* image) {
 }
 
 static void osem_step(mi_ctx* ctx, int subset, int subiteration_num) {
-  ctx->fp(mi::FP_RATIO, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
+  ctx->fp(ctx->ratio_mode(), ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
   if (!ctx->prior_active()) {
     // prior-free: divide, threshold and multiply fused into the back projection
     mi::BpData bd{};
@@ -1072,8 +1075,20 @@ int mi_obj_subset_gradient(mi_ctx* ctx, int subset, int add_sensitivity, float* 
     ctx->check_subset(subset);
     if (!ctx->have_data || !ctx->have_x) throw ApiError(MI_ERR_STATE, "data and estimate must be set first");
     HIPCHECK(hipSetDevice(ctx->device));
-    ctx->fp(add_sensitivity ? mi::FP_RATIO : mi::FP_GRADIENT, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
-    ctx->bp(mi::BP_PLAIN, ctx->ratio.p, ctx->img2.p, subset, mi::BpData{}, ctx->stream);
+    if (ctx->lm) {
+      // list mode: sum of row^T/fwd over the subset's events, minus the subset
+      // sensitivity (...ListModeDataWithProjMatrixByBin.cxx:913-961)
+      if (!add_sensitivity && !ctx->have_sens) throw ApiError(MI_ERR_STATE, "sensitivities have not been computed");
+      ctx->fp(mi::FP_LM_RATIO, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
+      ctx->bp(mi::BP_PLAIN, ctx->ratio.p, ctx->img2.p, subset, mi::BpData{}, ctx->stream);
+      if (!add_sensitivity)
+        for (int s = ctx->subset_begin(subset); s < ctx->subset_end(subset); ++s)
+          mi::launch_subtract(ctx->img2.p, ctx->sens[s]->p, ctx->img2.n, ctx->stream);
+    } else {
+      ctx->fp(add_sensitivity ? mi::FP_RATIO : mi::FP_GRADIENT, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(),
+              ctx->stream);
+      ctx->bp(mi::BP_PLAIN, ctx->ratio.p, ctx->img2.p, subset, mi::BpData{}, ctx->stream);
+    }
     ctx->image_from_device(ctx->img2.p, ctx->stage_img.p, ctx->stream);
     HIPCHECK(hipMemcpyAsync(gradient, ctx->stage_img.p, ctx->stage_img.bytes(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -1088,7 +1103,7 @@ int mi_obj_log_likelihood(mi_ctx* ctx, int subset, double* value) {
     if (!ctx->have_data || !ctx->have_x) throw ApiError(MI_ERR_STATE, "data and estimate must be set first");
     HIPCHECK(hipSetDevice(ctx->device));
     const int tb = ctx->task_off[ctx->subset_begin(subset)], te = ctx->task_off[ctx->subset_end(subset)];
-    ctx->fp(mi::FP_LOGLIK, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
+    ctx->fp(ctx->lm ? mi::FP_LM_VALUE : mi::FP_LOGLIK, ctx->x.p, ctx->ratio.p, subset, ctx->fp_data(), ctx->stream);
     std::vector<double> h(te - tb);
     HIPCHECK(hipMemcpyAsync(h.data(), ctx->partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -1105,6 +1120,7 @@ int mi_obj_value_and_gradient_dev(mi_ctx* ctx, const float* d_image, int subset,
     if (!d_gradient && !d_value) throw ApiError(MI_ERR_ARGUMENT, "neither a gradient nor a value was asked for");
     ctx->require_setup();
     ctx->check_subset(subset);
+    if (ctx->lm) throw ApiError(MI_ERR_STATE, "the fused value and gradient work on projection data, not list mode");
     if (!ctx->have_data) throw ApiError(MI_ERR_STATE, "mi_obj_set_data has not been called");
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->pick(stream);
@@ -1156,6 +1172,7 @@ int mi_osem_subset_step_dev(mi_ctx* ctx, const float* d_x_in, float* d_x_out, fl
                             int subiteration_num, void* stream) {
   return guarded([&] {
     if (!ctx || !d_x_in || !d_x_out) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (ctx->lm) throw ApiError(MI_ERR_STATE, "the out-of-place OSEM step works on projection data, not list mode");
     if (ctx->prior_active()) throw ApiError(MI_ERR_STATE, "the out-of-place OSEM step is prior-free");
     ctx->require_setup();
     if (!ctx->have_data || !ctx->have_sens) throw ApiError(MI_ERR_STATE, "data and sensitivities must be set first");
@@ -1241,6 +1258,70 @@ int mi_obj_set_use_subset_sensitivities(mi_ctx* ctx, int use) {
   return guarded([&] {
     if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     ctx->use_subset_sens = use != 0;
+    ctx->have_sens = false;
+  });
+}
+
+int mi_lm_set_events(mi_ctx* ctx, const int32_t* events, int64_t num_events, const float* additive,
+                     float additive_constant) {
+  return guarded([&] {
+    if (!ctx || (!events && num_events > 0)) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (num_events < 0) throw ApiError(MI_ERR_ARGUMENT, "negative number of events");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    const mi_geometry& g = ctx->plan->geometry();
+    const int64_t nb = ctx->total_bins();
+    // histogram of the prompts, events outside the projection data dropped
+    // (...ListModeDataWithProjMatrixByBin.cxx:495-511): y events of one bin
+    // contribute as y events of value 1 do
+    std::vector<float> counts(nb, 0.F);
+    const int min_tof = -(ctx->ntof / 2);
+    const int ntang = g.num_tangential_poss, min_tang = -(ntang / 2);
+    int64_t kept = 0;
+    for (int64_t i = 0; i < num_events; ++i) {
+      const int32_t* e = events + 5 * i;
+      const int seg = e[0], view = e[1], ax = e[2], tang = e[3], tof = e[4];
+      if (seg < g.min_segment_num || seg > g.max_segment_num || view < 0 || view >= g.num_views ||
+          tang < min_tang || tang >= min_tang + ntang || ax < 0 || ax >= ctx->plan->num_axial(seg))
+        continue;
+      if (ctx->ntof > 1 ? (tof < min_tof || tof >= min_tof + ctx->ntof) : tof != 0) continue;
+      const int64_t k = ctx->ntof > 1 ? tof - min_tof : 0;
+      const int64_t idx = k * ctx->block_bins + ctx->plan->segment_offset(seg) +
+                          (static_cast<int64_t>(ax) * g.num_views + view) * ntang + (tang - min_tang);
+      counts[idx] += 1.F;
+      ++kept;
+    }
+    (void)kept;
+    hipStream_t st = ctx->stream;
+    ctx->y.alloc(nb);
+    HIPCHECK(hipMemcpyAsync(ctx->stage_sino.p, counts.data(), nb * sizeof(float), hipMemcpyHostToDevice, st));
+    ctx->proj_to_device(ctx->stage_sino.p, ctx->y.p, st);
+    if (additive) {
+      // the cache keeps the additive term of the last timing position for
+      // every event (...ListModeDataWithProjMatrixByBin.cxx:570-596)
+      std::vector<float> a(nb);
+      for (int k = 0; k < ctx->ntof; ++k)
+        std::memcpy(a.data() + k * ctx->block_bins, additive + (ctx->ntof - 1) * ctx->block_bins,
+                    ctx->block_bins * sizeof(float));
+      ctx->add.alloc(nb);
+      HIPCHECK(hipMemcpyAsync(ctx->stage_sino.p, a.data(), nb * sizeof(float), hipMemcpyHostToDevice, st));
+      ctx->proj_to_device(ctx->stage_sino.p, ctx->add.p, st);
+      HIPCHECK(hipStreamSynchronize(st));
+    } else {
+      ctx->add.release();
+    }
+    ctx->add_const = additive_constant;
+    ctx->zero_end_planes = 0;
+    ctx->ones.alloc(nb);
+    mi::launch_fill(ctx->ones.p, nb, 1.F, st);
+    ctx->rebuild_mult(st);
+    const int nvg = ctx->plan->num_viewgrams() * ctx->ntof;
+    ctx->vg_small.alloc(nvg);
+    mi::launch_viewgram_max(ctx->y.p, ctx->vg_off.p, ctx->vg_len.p, nvg, ctx->vg_small.p, st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    ctx->lm = true;
+    ctx->have_data = true;
     ctx->have_sens = false;
   });
 }

@@ -63,7 +63,12 @@ enum FpMode : int {
   FP_GRADIENT_LOGLIK = 4,  // FP_GRADIENT and FP_LOGLIK in one pass
   FP_RATIO_LOGLIK = 5,     // FP_RATIO and FP_LOGLIK in one pass
   FP_DRATIO = 6,           // out = d divide_and_truncate(y, ybar) / d ybar at ybar = A x + b
-  FP_SCALE = 7             // out = (A x) * scale
+  FP_SCALE = 7,            // out = (A x) * scale
+  // list mode with the events histogrammed into y (LM_gradient_and_value,
+  // ...ListModeDataWithProjMatrixByBin.cxx:748-787): out = y / (A x + b),
+  // 0 where y = 0 or 1 > 1e4 (A x + b)
+  FP_LM_RATIO = 8,
+  FP_LM_VALUE = 9          // partial sums of -y log(A x + b) (-y log(1e-4) where capped)
 };
 
 struct FpData {
@@ -236,6 +241,7 @@ void launch_osl_update(float* d_x, const float* d_upd, const float* d_pgrad, con
 // dst += src; dst /= d (element-wise, n elements)
 void launch_accumulate(float* d_dst, const float* d_src, int64_t n, hipStream_t stream);
 void launch_divide_scalar(float* d_dst, float d, int64_t n, hipStream_t stream);
+void launch_subtract(float* d_dst, const float* d_src, int64_t n, hipStream_t stream);  // dst -= src
 // *d_out = sum of n doubles in a fixed order (one workgroup); d_out may be null-checked by the caller.
 void launch_sum_partials(const double* d_partial, int n, double scale, double* d_out, hipStream_t stream);
 // v = g * x / sens (0 where sens == 0), device image layout, n elements; v may alias g or x

@@ -156,7 +156,9 @@ __device__ inline float osem_factor(float u, float sv, int threshold, float lo, 
 constexpr bool fp_writes_ratio(int m) {
   return m == FP_RATIO || m == FP_GRADIENT || m == FP_GRADIENT_LOGLIK || m == FP_RATIO_LOGLIK;
 }
-constexpr bool fp_accumulates_loglik(int m) { return m == FP_LOGLIK || m == FP_GRADIENT_LOGLIK || m == FP_RATIO_LOGLIK; }
+constexpr bool fp_accumulates_loglik(int m) {
+  return m == FP_LOGLIK || m == FP_GRADIENT_LOGLIK || m == FP_RATIO_LOGLIK || m == FP_LM_VALUE;
+}
 
 // Forward-projection epilogue of one bin: a = (A x)_i, vg = its viewgram.
 template <int MODE>
@@ -165,6 +167,18 @@ __device__ inline void fp_epilogue(float a, int64_t i, int vg, const FpData& dat
     as_global(out)[i] = a;
   } else if constexpr (MODE == FP_SCALE) {
     as_global(out)[i] = a * as_const(data.scale)[i];
+  } else if constexpr (MODE == FP_LM_RATIO || MODE == FP_LM_VALUE) {
+    // per event (bin value 1): fwd = row . x + additive; skipped when
+    // 1 > 1e4 fwd; y events of one bin contribute y / fwd
+    const float fwd = a + (data.additive ? as_const(data.additive)[i] : data.additive_constant);
+    const float yv = as_const(data.measured)[i];
+    const bool capped = 1.F > 10000.F * fwd;
+    if constexpr (MODE == FP_LM_RATIO) {
+      as_global(out)[i] = (yv == 0.F || capped) ? 0.F : yv / fwd;
+    } else {
+      if (yv != 0.F)
+        ll -= static_cast<double>(yv) * (capped ? log(static_cast<double>(1.F / 10000.F)) : log(static_cast<double>(fwd)));
+    }
   } else {
     const float ybar = a + (data.additive ? as_const(data.additive)[i] : data.additive_constant);
     const float yv = as_const(data.measured)[i];

@@ -205,6 +205,12 @@ __global__ void accumulate_kernel(float* __restrict__ dst, const float* __restri
     dst[i] = dst[i] + src[i];
 }
 
+__global__ void subtract_kernel(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = dst[i] - src[i];
+}
+
 __global__ void divide_scalar_kernel(float* __restrict__ dst, float d, int64_t n) {
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -268,6 +274,11 @@ void launch_osl_update(float* d_x, const float* d_upd, const float* d_pgrad, con
 void launch_accumulate(float* d_dst, const float* d_src, int64_t n, hipStream_t stream) {
   if (n <= 0) return;
   hipLaunchKernelGGL(accumulate_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_dst, d_src, n);
+}
+
+void launch_subtract(float* d_dst, const float* d_src, int64_t n, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(subtract_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_dst, d_src, n);
 }
 
 void launch_divide_scalar(float* d_dst, float d, int64_t n, hipStream_t stream) {

@@ -438,6 +438,8 @@ void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, cons
     MI_FP_CASE(FP_RATIO_LOGLIK)
     MI_FP_CASE(FP_DRATIO)
     MI_FP_CASE(FP_SCALE)
+    MI_FP_CASE(FP_LM_RATIO)
+    MI_FP_CASE(FP_LM_VALUE)
 #undef MI_FP_CASE
   }
 }

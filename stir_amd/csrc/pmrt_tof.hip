@@ -366,6 +366,8 @@ void launch_forward_project_tof(FpMode mode, const FpTask* d_tasks, int ntasks, 
     MI_FPT_CASE(FP_RATIO_LOGLIK)
     MI_FPT_CASE(FP_DRATIO)
     MI_FPT_CASE(FP_SCALE)
+    MI_FPT_CASE(FP_LM_RATIO)
+    MI_FPT_CASE(FP_LM_VALUE)
   }
 #undef MI_FPT_CASE
 #undef MI_FPT

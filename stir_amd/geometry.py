@@ -296,6 +296,29 @@ class ProjDataInfoCylindrical:
         off = self.segment_offsets()[seg]
         return off + (ax * self.num_views + view) * self.num_tangential_poss + (tang - self.min_tangential_pos_num)
 
+    def bin_coordinates(self) -> np.ndarray:
+        """(segment, view, axial, tangential, timing) of every bin in
+        ProjDataInMemory order, TOF bins outermost: int32 array num_bins x 5."""
+        parts = []
+        offs = self.segment_offsets()
+        for k in range(self.num_tof_bins):
+            for seg in self.segment_sequence():
+                na = self.num_axial_poss_of(seg)
+                a, v, t = np.meshgrid(np.arange(na), np.arange(self.num_views),
+                                      np.arange(self.num_tangential_poss) + self.min_tangential_pos_num, indexing="ij")
+                c = np.stack([np.full(a.size, seg), v.reshape(-1), a.reshape(-1), t.reshape(-1),
+                              np.full(a.size, k + self.min_tof_pos_num)], axis=1)
+                parts.append(c)
+        assert offs[0] == 0
+        return np.concatenate(parts).astype(np.int32)
+
+    def events_from_counts(self, counts: np.ndarray, seed: int = 0) -> np.ndarray:
+        """List-mode events (int32 n x 5) with one event per count of each
+        bin, in a seeded random order."""
+        c = np.asarray(counts).reshape(-1).astype(np.int64)
+        ev = np.repeat(self.bin_coordinates(), c, axis=0)
+        return ev[np.random.default_rng(seed).permutation(ev.shape[0])]
+
     def segment_view(self, data: np.ndarray, seg: int) -> np.ndarray:
         """View of one segment of a flat STIR-ordered array as [axial, view, tang]."""
         off = self.segment_offsets()[seg]

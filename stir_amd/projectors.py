@@ -317,6 +317,37 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
         return float(v.value)
 
 
+class PoissonLogLikelihoodWithLinearModelForMeanAndListModeDataWithProjMatrixByBin(
+        PoissonLogLikelihoodWithLinearModelForMeanAndProjData):
+    """List-mode Poisson log-likelihood with the same matrix
+    (``PoissonLogLikelihoodWithLinearModelForMeanAndListModeDataWithProjMatrixByBin.cxx``):
+    prompt events (int32 n x 5: segment, view, axial, tangential, timing),
+    an optional additive projection-data term and multiplicative factors for
+    the sensitivities. Gradients and OSMAPOSL updates sum row^T / (row . x + b)
+    over the subset's events (events with 1 > 1e4 (row . x + b) add nothing);
+    ``compute_objective_function`` returns the reference's value, -sum log(row . x + b)."""
+
+    def set_up(self, events: np.ndarray, additive=0.0, multiplicative: Optional[np.ndarray] = None,
+               use_subset_sensitivities: bool = True, compute_sensitivities: bool = True,
+               use_tof_sensitivities: bool = False) -> None:
+        L = _native.load()
+        _native.check(L.mi_obj_set_use_tof_sensitivities(self._ctx.handle, int(use_tof_sensitivities)),
+                      "mi_obj_set_use_tof_sensitivities")
+        _native.check(L.mi_obj_set_use_subset_sensitivities(self._ctx.handle, int(use_subset_sensitivities)),
+                      "mi_obj_set_use_subset_sensitivities")
+        self.set_multiplicative(multiplicative)
+        ev = np.ascontiguousarray(events, dtype=np.int32).reshape(-1, 5)
+        if isinstance(additive, np.ndarray):
+            a = np.ascontiguousarray(additive, dtype=np.float32).reshape(-1)
+            _native.check(L.mi_lm_set_events(self._ctx.handle, ev, ev.shape[0], a.ctypes.data_as(ctypes.c_void_p),
+                                             ctypes.c_float(0.0)), "mi_lm_set_events")
+        else:
+            _native.check(L.mi_lm_set_events(self._ctx.handle, ev, ev.shape[0], None, ctypes.c_float(float(additive))),
+                          "mi_lm_set_events")
+        if compute_sensitivities:
+            self.compute_sensitivities()
+
+
 class QuadraticPrior:
     """``QuadraticPrior`` (``QuadraticPrior.cxx``) as a prior of the objective
     function: penalisation factor, optional 3x3x3 weights (default vox_x /

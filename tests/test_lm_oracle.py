@@ -1,0 +1,104 @@
+"""Oracle pins for the list-mode objective
+(PoissonLogLikelihoodWithLinearModelForMeanAndListModeDataWithProjMatrixByBin), CPU.
+
+The list-mode restatement works event by event, as the reference does. With
+one event per count of a histogram y and no quotient capping, its sums equal
+the projection-data objective's (pinned elsewhere) term by term: the
+gradient BP(y/ybar) of every subset (which pins the event-to-subset rule:
+basic view mod number of subsets), the OSEM images, and the value -sum y log
+ybar. The 1e4 cap of LM_gradient_and_value is pinned by a closed form (an
+event whose row misses the image, no additive term: no gradient, value
+-log(1e-4)), and events outside the projection data are dropped.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OraclePlan
+from stir_amd import phantoms
+from stir_amd.geometry import (ProjDataInfoCylindrical, RayTracingMatrixParameters, Scanner, VoxelsOnCartesianGrid,
+                               config0_tof)
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    sc = Scanner.cylindrical(4, 64, 80.0, 4.0)
+    pdi = ProjDataInfoCylindrical.construct(sc, span=1, max_delta=3, num_views=32, num_tangential_poss=32)
+    img = VoxelsOnCartesianGrid(7, 32, 32, 2.0, 2.0, 2.0)
+    O = OraclePlan(pdi, img, RayTracingMatrixParameters())
+    x_true, y, bkg = phantoms.config0_data(pdi, img, O.forward_project)
+    ev = pdi.events_from_counts(y, seed=1)
+    return pdi, img, O, x_true, y, bkg, ev
+
+
+def test_lm_gradient_equals_histogram_gradient(tiny):
+    pdi, img, O, x_true, y, bkg, ev = tiny
+    b = np.full(pdi.num_bins, bkg, np.float32)
+    x = (x_true + 0.2).astype(np.float32)
+    for s, n in ((0, 1), (0, 4), (3, 4)):
+        _, g = O.lm_gradient(ev, x, s, n, additive=b)
+        ref = O.subset_gradient(y, x, s if n > 1 else -1, n, bkg=b, add_sensitivity=True)
+        assert np.abs(g - ref).max() <= 1e-5 * np.abs(ref).max(), (s, n)
+    # subsets partition the events
+    total = sum(O.lm_gradient(ev, x, s, 4, additive=b)[1].astype(np.float64) for s in range(4))
+    full = O.lm_gradient(ev, x, 0, 1, additive=b)[1]
+    assert np.abs(total - full).max() <= 1e-5 * np.abs(full).max()
+
+
+def test_lm_value(tiny):
+    pdi, img, O, x_true, y, bkg, ev = tiny
+    b = np.full(pdi.num_bins, bkg, np.float32)
+    x = (x_true + 0.2).astype(np.float32)
+    v, _ = O.lm_gradient(ev, x, 0, 1, additive=b, gradient=False)
+    ybar = O.forward_project(x).astype(np.float64) + bkg
+    ref = -float((y.astype(np.float64) * np.log(ybar)).sum())
+    assert abs(v - ref) <= 1e-6 * abs(ref)
+
+
+def test_lm_cap_closed_form(tiny):
+    """An event at the outermost tangential position (its row misses the
+    cylindrical FOV, so A_e x = 0) without additive term: 1 > 1e4 * 0, no
+    gradient, value -log(float(1/1e4))."""
+    pdi, img, O, x_true, y, bkg, ev = tiny
+    e = np.array([[0, 3, 1, pdi.max_tangential_pos_num, 0]], np.int32)
+    x = np.ones(img.shape, np.float32)
+    assert O.forward_project(x)[pdi.bin_index(0, 3, 1, pdi.max_tangential_pos_num)] == 0
+    v, g = O.lm_gradient(e, x)
+    assert not g.any()
+    assert v == -math.log(float(np.float32(1) / np.float32(1e4)))
+
+
+def test_lm_events_outside_the_data_are_dropped(tiny):
+    pdi, img, O, x_true, y, bkg, ev = tiny
+    b = np.full(pdi.num_bins, bkg, np.float32)
+    bad = np.array([[pdi.max_segment_num + 1, 0, 0, 0, 0], [0, pdi.num_views, 0, 0, 0], [0, 0, 99, 0, 0],
+                    [0, 0, 0, pdi.max_tangential_pos_num + 1, 0], [0, 0, 0, 0, 1]], np.int32)
+    x = (x_true + 0.2).astype(np.float32)
+    v0, g0 = O.lm_gradient(ev, x, 0, 1, additive=b)
+    v1, g1 = O.lm_gradient(np.concatenate([bad, ev]), x, 0, 1, additive=b)
+    assert v0 == v1 and np.array_equal(g0, g1)
+
+
+def test_lm_osem_equals_histogram_osem(tiny):
+    pdi, img, O, x_true, y, bkg, ev = tiny
+    b = np.full(pdi.num_bins, bkg, np.float32)
+    x0 = np.ones(img.shape, np.float32)
+    got = O.lm_osem(ev, x0, 4, 8, additive=b)
+    ref = O.osem(y, x0, 4, 8, bkg=b)
+    m = ref > 1e-3 * ref.max()
+    assert float((np.abs(got - ref)[m] / ref[m]).max()) <= 1e-4
+
+
+def test_lm_tof_gradient_equals_histogram_gradient():
+    pdi, img = config0_tof(3)
+    O = OraclePlan(pdi, img, RayTracingMatrixParameters())
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.5, 1.5, img.shape).astype(np.float32)
+    ybar = O.forward_project(x)
+    y = rng.poisson(np.maximum(ybar, 0) * 0.02).astype(np.float32)
+    ev = pdi.events_from_counts(y, seed=2)
+    b = np.full(pdi.num_bins, 0.01, np.float32)
+    _, g = O.lm_gradient(ev, x, 1, 4, additive=b)
+    ref = O.subset_gradient(y, x, 1, 4, bkg=b, add_sensitivity=True)
+    assert np.abs(g - ref).max() <= 1e-5 * np.abs(ref).max()
