@@ -298,6 +298,17 @@ int mi_relative_difference_prior_dev(mi_ctx* ctx, const float* d_image, const fl
                                      float beta, float gamma, float epsilon, int only_2D, float* d_gradient,
                                      double* d_value, void* stream);
 
+/* Prior Hessian products on device images (STIR layout), accumulated into
+ * d_output: accumulate_Hessian_times_input (QuadraticPrior.cxx:604-690,
+ * RelativeDifferencePrior.cxx:535-614) of d_input at the estimate d_image,
+ * or with approximate = 1 the quadratic prior's
+ * add_multiplication_with_approximate_Hessian (:537-602; d_image unused).
+ * type MI_PRIOR_QUADRATIC or MI_PRIOR_RELATIVE_DIFFERENCE; other arguments
+ * as mi_relative_difference_prior_dev. */
+int mi_prior_hessian_times_input_dev(mi_ctx* ctx, int type, int approximate, const float* d_image,
+                                     const float* d_input, const float* d_kappa, const float* weights, float beta,
+                                     float gamma, float epsilon, int only_2D, float* d_output, void* stream);
+
 /* One OSMAPOSL subiteration on device images, out of place:
  * d_x_out = d_x_in * u, u = clamp(BP_s(y/(FP_s(x)+b)) / sens_s)
  * (OSMAPOSLReconstruction::update_estimate, OSMAPOSLReconstruction.cxx:373-516,

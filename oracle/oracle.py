@@ -81,6 +81,9 @@ def lib():
         L.or_lm_gradient.restype = ctypes.c_double
         L.or_lm_gradient.argtypes = [vp, _i32p, ctypes.c_int64, vp, _f32p, ctypes.c_int, ctypes.c_int, vp]
         L.or_lm_osem.argtypes = [vp, _i32p, ctypes.c_int64, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_prior_hessian_times_input.argtypes = [ctypes.c_int, _f32p, _f32p, vp, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                                   ctypes.c_float, _f32p]
         L.or_set_multiplicative.argtypes = [vp, vp]
         L.or_set_use_subset_sensitivities.argtypes = [vp, ctypes.c_int]
         L.or_set_osem_options.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float]
@@ -121,6 +124,23 @@ def relative_difference_prior(image, vox_zyx, beta, gamma=2.0, epsilon=0.0, kapp
                                            float(beta), float(gamma), float(epsilon),
                                            g.ctypes.data_as(ctypes.c_void_p) if g is not None else None)
     return float(v), g
+
+
+def prior_hessian_times_input(kind, image, inp, vox_zyx, beta, gamma=2.0, epsilon=0.0, kappa=None, only_2D=False,
+                              weights=None, out=None):
+    """kind "quadratic" / "relative difference": accumulate_Hessian_times_input;
+    "approximate quadratic": the quadratic prior's add_multiplication_with_approximate_Hessian.
+    Returns out (zeros if not given) + the product."""
+    k = {"quadratic": 1, "relative difference": 2, "approximate quadratic": 3}[kind]
+    x = np.ascontiguousarray(image, dtype=np.float32)
+    v = np.ascontiguousarray(inp, dtype=np.float32).reshape(x.shape)
+    nz, ny, nx = x.shape
+    w = (quadratic_prior_weights(vox_zyx) if weights is None else np.asarray(weights, np.float32)).reshape(-1).copy()
+    kp = _opt_ptr(None if kappa is None else np.ascontiguousarray(kappa, np.float32).reshape(x.shape))
+    o = np.zeros(x.shape, np.float32) if out is None else np.array(out, np.float32, copy=True)
+    lib().or_prior_hessian_times_input(k, x.reshape(-1), v.reshape(-1), kp[0] if kp else None, nz, ny, nx, w,
+                                       int(only_2D), float(beta), float(gamma), float(epsilon), o.reshape(-1))
+    return o
 
 
 def ray_trace(start_zyx, stop_zyx, vox_zyx, nc):

@@ -1744,3 +1744,66 @@ void or_lm_osem(const or_plan* p, const int32_t* ev, int64_t n, const float* add
   free(sens);
 }
 
+/* ------------------------------------------------------------------ */
+/* Prior Hessian products                                               */
+/* ------------------------------------------------------------------ */
+/* derivative_20 / derivative_11 of the relative difference prior
+ * (RelativeDifferencePrior.cxx:617-641): pow of float arguments is the double
+ * pow; a*a of a float is exact in double and (a*a)*a is a^3 correctly rounded. */
+static float rdp_d20(float xj, float xk, float gamma, float eps)
+{
+  if (!(xj > 0.F || xk > 0.F || eps > 0.F)) return INFINITY;
+  const double a = (double)(2 * xk + eps);
+  const double d = (double)(xj + xk + gamma * fabsf(xj - xk) + eps);
+  return (float)(2 * (a * a) / ((d * d) * d));
+}
+static float rdp_d11(float xj, float xk, float gamma, float eps)
+{
+  if (!(xj > 0.F || xk > 0.F || eps > 0.F)) return INFINITY;
+  const float num = -2 * (2 * xj + eps) * (2 * xk + eps);
+  const double d = (double)(xj + xk + gamma * fabsf(xj - xk) + eps);
+  return (float)((double)num / ((d * d) * d));
+}
+
+/* accumulate_Hessian_times_input (QuadraticPrior.cxx:604-690,
+ * RelativeDifferencePrior.cxx:535-614): out[j] += beta * sum over
+ * neighbours k with w != 0 of w (f20(x_j, x_k) in_j + f11(x_j, x_k) in_k)
+ * kappa_j kappa_k, float arithmetic, (dz, dy, dx) order. kind 1: quadratic
+ * (f20 = 1, f11 = -1); kind 2: relative difference. kind 3: the quadratic
+ * prior's add_multiplication_with_approximate_Hessian (:537-602): out[j] +=
+ * beta * sum w in_k kappa_j kappa_k (the centre included, weight 0). */
+void or_prior_hessian_times_input(int kind, const float* x, const float* in, const float* kappa, int nz, int ny, int nx,
+                                  const float* w27, int only_2D, float beta, float gamma, float eps, float* out)
+{
+  const int wz0 = only_2D ? 0 : -1, wz1 = only_2D ? 0 : 1;
+  for (int z = 0; z < nz; ++z) {
+    const int min_dz = wz0 > -z ? wz0 : -z, max_dz = wz1 < nz - 1 - z ? wz1 : nz - 1 - z;
+    for (int y = 0; y < ny; ++y) {
+      const int min_dy = -1 > -y ? -1 : -y, max_dy = 1 < ny - 1 - y ? 1 : ny - 1 - y;
+      for (int xx = 0; xx < nx; ++xx) {
+        const int min_dx = -1 > -xx ? -1 : -xx, max_dx = 1 < nx - 1 - xx ? 1 : nx - 1 - xx;
+        const int64_t v = ((int64_t)z * ny + y) * nx + xx;
+        float result = 0;
+        for (int dz = min_dz; dz <= max_dz; ++dz)
+          for (int dy = min_dy; dy <= max_dy; ++dy)
+            for (int dx = min_dx; dx <= max_dx; ++dx) {
+              const int64_t n = ((int64_t)(z + dz) * ny + (y + dy)) * nx + (xx + dx);
+              float current = w27[((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)];
+              if (kind == 3) {
+                current = current * in[n];
+              } else {
+                if (current == 0.F) continue;
+                const float f20 = kind == 1 ? 1.F : rdp_d20(x[v], x[n], gamma, eps);
+                const float f11 = kind == 1 ? -1.F : rdp_d11(x[v], x[n], gamma, eps);
+                if (dz == 0 && dy == 0 && dx == 0) current *= f20 * in[v];
+                else current *= (f20 * in[v] + f11 * in[n]);
+              }
+              if (kappa) current *= kappa[v] * kappa[n];
+              result += current;
+            }
+        out[v] += result * beta;
+      }
+    }
+  }
+}
+

@@ -195,6 +195,11 @@ double or_quadratic_prior(const float* img, const float* kappa, int nz, int ny, 
 double or_relative_difference_prior(const float* img, const float* kappa, int nz, int ny, int nx, const float* w27,
                                     int only_2D, float beta, float gamma, float eps, float* grad);
 
+/* Prior Hessian products accumulated into out: kind 1 quadratic and 2
+ * relative difference accumulate_Hessian_times_input, kind 3 the quadratic
+ * prior's add_multiplication_with_approximate_Hessian. */
+void or_prior_hessian_times_input(int kind, const float* x, const float* in, const float* kappa, int nz, int ny, int nx,
+                                  const float* w27, int only_2D, float beta, float gamma, float eps, float* out);
 /* List mode: events n x (segment, view, axial, tangential, timing position).
  * or_lm_gradient accumulates sum over the subset's events of row^T (1/fwd)
  * into grad (if not NULL) and returns the reference's accumulated value

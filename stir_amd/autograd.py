@@ -191,6 +191,8 @@ class QuadraticPriorLoss:
             kappa = _as_batch(kappa, pair.img.num_voxels, "kappa")[0]
         self.kappa = kappa
         self._w = None if weights is None else np.ascontiguousarray(np.asarray(weights, np.float32).reshape(27))
+        self._type = _native.MI_PRIOR_QUADRATIC
+        self._gamma_eps = (0.0, 0.0)
 
     def _weights_arg(self):
         return None if self._w is None else self._w.ctypes.data_as(ctypes.c_void_p)
@@ -205,6 +207,21 @@ class QuadraticPriorLoss:
         want_grad = torch.is_grad_enabled() and x.requires_grad
         return _NeighbourhoodPrior.apply(x, self, want_grad)
 
+    def hessian_times_input(self, x: torch.Tensor, v: torch.Tensor, out: Optional[torch.Tensor] = None,
+                            approximate: bool = False) -> torch.Tensor:
+        """out + H(x) v (``accumulate_Hessian_times_input``), or with approximate
+        the quadratic prior's ``add_multiplication_with_approximate_Hessian``."""
+        nvox = self.pair.img.num_voxels
+        x = _as_batch(x, nvox, "image")[0]
+        v = _as_batch(v, nvox, "input")[0]
+        out = torch.zeros_like(v) if out is None else _as_batch(out, nvox, "output")[0]
+        g, e = self._gamma_eps
+        _native.check(_native.load().mi_prior_hessian_times_input_dev(
+            self.pair.context.handle, self._type, int(approximate), _ptr(x), _ptr(v), _ptr(self.kappa),
+            self._weights_arg(), ctypes.c_float(self.beta), ctypes.c_float(g), ctypes.c_float(e), int(self.only_2D),
+            _ptr(out), _stream(v)), "mi_prior_hessian_times_input_dev")
+        return out.reshape(self.pair.img.shape)
+
 
 class RelativeDifferencePriorLoss(QuadraticPriorLoss):
     """``RelativeDifferencePrior``: beta * sum_v sum_n w_vn k_v k_n 0.5 (x_v - x_n)^2 /
@@ -218,6 +235,8 @@ class RelativeDifferencePriorLoss(QuadraticPriorLoss):
         super().__init__(pair, beta, kappa, only_2D, weights)
         self.gamma = float(gamma)
         self.epsilon = float(epsilon)
+        self._type = _native.MI_PRIOR_RELATIVE_DIFFERENCE
+        self._gamma_eps = (self.gamma, self.epsilon)
 
     def _native(self, x, grad, value, st):
         _native.check(_native.load().mi_relative_difference_prior_dev(

@@ -1402,6 +1402,30 @@ int mi_relative_difference_prior_dev(mi_ctx* ctx, const float* d_image, const fl
   });
 }
 
+int mi_prior_hessian_times_input_dev(mi_ctx* ctx, int type, int approximate, const float* d_image,
+                                     const float* d_input, const float* d_kappa, const float* weights, float beta,
+                                     float gamma, float epsilon, int only_2D, float* d_output, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_input || !d_output || (!d_image && !approximate)) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (type != MI_PRIOR_QUADRATIC && type != MI_PRIOR_RELATIVE_DIFFERENCE)
+      throw ApiError(MI_ERR_ARGUMENT, "unknown prior type");
+    if (approximate && type != MI_PRIOR_QUADRATIC)
+      // RelativeDifferencePrior.cxx:527-532
+      throw ApiError(MI_ERR_ARGUMENT, "the approximate Hessian is implemented for the quadratic prior only");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    if (beta == 0.F) return;  // accumulate_Hessian_times_input returns at once
+    hipStream_t st = ctx->pick(stream);
+    mi::PriorWeights W;
+    if (weights) std::memcpy(W.w, weights, sizeof(W.w));
+    else default_prior_weights(ctx->plan->geometry(), W.w);
+    const int kind = approximate ? 3 : (type == MI_PRIOR_QUADRATIC ? 1 : 2);
+    mi::launch_prior_hessian(kind, approximate ? d_input : d_image, d_input, d_kappa, ctx->stir_prior_image(), W,
+                             only_2D, beta, gamma, epsilon, d_output, st);
+    HIPCHECK(hipGetLastError());
+  });
+}
+
 int mi_set_tuning(mi_ctx* ctx, int param, int value) {
   return guarded([&] {
     if (!ctx) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");

@@ -231,6 +231,12 @@ int prior_num_partials(const PriorImage& im);
 void launch_prior(int kind, const float* d_img, const float* d_kappa, const PriorImage& im, const PriorWeights& w,
                   int only_2D, float beta, float gamma, float eps, float* d_grad, double* d_partial,
                   hipStream_t stream);
+// d_out += Hessian products of a [z][y][x] prior (kind 1 quadratic, 2 relative
+// difference: accumulate_Hessian_times_input; 3: the quadratic prior's
+// add_multiplication_with_approximate_Hessian)
+void launch_prior_hessian(int kind, const float* d_x, const float* d_in, const float* d_kappa, const PriorImage& im,
+                          const PriorWeights& w, int only_2D, float beta, float gamma, float eps, float* d_out,
+                          hipStream_t stream);
 // *d_small = max(max(v) * small_num, 0) over n floats (d_part: max_num_partials() floats)
 int max_num_partials();
 void launch_small_value(const float* d_v, int64_t n, float small_num, float* d_part, float* d_small, hipStream_t stream);

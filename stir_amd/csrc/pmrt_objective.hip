@@ -98,6 +98,69 @@ __global__ void __launch_bounds__(kPriorThreads) prior_kernel(const float* __res
   }
 }
 
+// derivative_20 / derivative_11 of the relative difference prior
+// (RelativeDifferencePrior.cxx:617-641): the reference's pow of float
+// arguments is the double pow; a * a of a float is exact in double and
+// (a * a) * a is a^3 correctly rounded.
+__device__ inline float rdp_d20(float xj, float xk, float gamma, float eps) {
+  if (!(xj > 0.F || xk > 0.F || eps > 0.F)) return INFINITY;
+  const double a = static_cast<double>(2 * xk + eps);
+  const double d = static_cast<double>(xj + xk + gamma * fabsf(xj - xk) + eps);
+  return static_cast<float>(2 * (a * a) / ((d * d) * d));
+}
+__device__ inline float rdp_d11(float xj, float xk, float gamma, float eps) {
+  if (!(xj > 0.F || xk > 0.F || eps > 0.F)) return INFINITY;
+  const float num = -2 * (2 * xj + eps) * (2 * xk + eps);
+  const double d = static_cast<double>(xj + xk + gamma * fabsf(xj - xk) + eps);
+  return static_cast<float>(static_cast<double>(num) / ((d * d) * d));
+}
+
+// out[j] += beta * sum_k w (f20(x_j, x_k) in_j + f11(x_j, x_k) in_k) kappa_j kappa_k
+// (accumulate_Hessian_times_input, QuadraticPrior.cxx:604-690,
+// RelativeDifferencePrior.cxx:535-614; KIND 1 quadratic f20 = 1, f11 = -1; 2
+// relative difference), or KIND 3: out[j] += beta * sum_k w in_k kappa_j
+// kappa_k (the quadratic prior's add_multiplication_with_approximate_Hessian,
+// :537-602). Float arithmetic in (dz, dy, dx) order, [z][y][x] images.
+template <int KIND, bool KAPPA>
+__global__ void __launch_bounds__(kPriorThreads) prior_hessian_kernel(const float* __restrict__ x,
+                                                                      const float* __restrict__ in,
+                                                                      const float* __restrict__ kappa, PriorImage im,
+                                                                      PriorWeights W, int wz0, int wz1, float beta,
+                                                                      float gamma, float eps, float* __restrict__ out) {
+  const int xx = blockIdx.x * kPriorTileX + static_cast<int>(threadIdx.x % kPriorTileX);
+  const int y = blockIdx.y * kPriorTileY + static_cast<int>(threadIdx.x / kPriorTileX);
+  const int z = blockIdx.z;
+  const int nz = im.nz, ny = im.ny, nx = im.nx;
+  if (xx >= nx || y >= ny) return;
+  const int64_t plane = static_cast<int64_t>(ny) * nx;
+  const int64_t v = z * plane + static_cast<int64_t>(y) * nx + xx;
+  const float xv = x[v], iv = in[v];
+  const float kv = KAPPA ? kappa[v] : 1.F;
+  const int min_dz = max(wz0, -z), max_dz = min(wz1, nz - 1 - z);
+  const int min_dy = max(-1, -y), max_dy = min(1, ny - 1 - y);
+  const int min_dx = max(-1, -xx), max_dx = min(1, nx - 1 - xx);
+  float result = 0.F;
+  for (int dz = min_dz; dz <= max_dz; ++dz)
+    for (int dy = min_dy; dy <= max_dy; ++dy)
+      for (int dx = min_dx; dx <= max_dx; ++dx) {
+        const int64_t n = v + dz * plane + static_cast<int64_t>(dy) * nx + dx;
+        float current = W.w[((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)];
+        if constexpr (KIND == 3) {
+          current = current * in[n];
+        } else {
+          if (current == 0.F) continue;
+          const float xn = x[n];
+          const float f20 = KIND == 1 ? 1.F : rdp_d20(xv, xn, gamma, eps);
+          const float f11 = KIND == 1 ? -1.F : rdp_d11(xv, xn, gamma, eps);
+          if (dz == 0 && dy == 0 && dx == 0) current *= f20 * iv;
+          else current *= (f20 * iv + f11 * in[n]);
+        }
+        if (KAPPA) current *= kv * kappa[n];
+        result += current;
+      }
+  out[v] = out[v] + result * beta;
+}
+
 // Maximum of n floats: per-workgroup maxima into part[blockIdx.x].
 __global__ void __launch_bounds__(256) max_partials_kernel(const float* __restrict__ v, int64_t n,
                                                            float* __restrict__ part) {
@@ -252,6 +315,24 @@ void launch_prior(int kind, const float* d_img, const float* d_kappa, const Prio
 #undef MI_PRIOR_K
 #undef MI_PRIOR_Z
 #undef MI_PRIOR
+}
+
+void launch_prior_hessian(int kind, const float* d_x, const float* d_in, const float* d_kappa, const PriorImage& im,
+                          const PriorWeights& w, int only_2D, float beta, float gamma, float eps, float* d_out,
+                          hipStream_t stream) {
+  const dim3 grid((im.nx + kPriorTileX - 1) / kPriorTileX, (im.ny + kPriorTileY - 1) / kPriorTileY, im.nz);
+  const int wz0 = only_2D ? 0 : -1, wz1 = only_2D ? 0 : 1;
+#define MI_PH(K, KA)                                                                                              \
+  hipLaunchKernelGGL((prior_hessian_kernel<K, KA>), grid, dim3(kPriorThreads), 0, stream, d_x, d_in, d_kappa, im, \
+                     w, wz0, wz1, beta, gamma, eps, d_out)
+#define MI_PH_K(K)                   \
+  if (d_kappa) { MI_PH(K, true); }   \
+  else { MI_PH(K, false); }
+  if (kind == 2) { MI_PH_K(2) }
+  else if (kind == 3) { MI_PH_K(3) }
+  else { MI_PH_K(1) }
+#undef MI_PH_K
+#undef MI_PH
 }
 
 int max_num_partials() { return 1024; }

@@ -146,3 +146,29 @@ def test_relative_difference_prior_kernel(c0, variant):
                                        only_2D=only_2D)
     assert abs(v.item() - vo) <= 1e-12 * abs(vo)
     assert np.array_equal(x.grad.cpu().numpy(), go)
+
+
+@pytest.mark.parametrize("kind,approximate,use_kappa", [("quadratic", False, False), ("quadratic", True, True),
+                                                        ("relative difference", False, False),
+                                                        ("relative difference", False, True)])
+def test_prior_hessian_times_input(c0, kind, approximate, use_kappa):
+    """accumulate_Hessian_times_input / add_multiplication_with_approximate_Hessian
+    on the GPU: the same float terms in the same order as the oracle,
+    accumulated into a non-zero output -- bit-identical."""
+    from oracle.oracle import prior_hessian_times_input
+    from stir_amd.autograd import QuadraticPriorLoss
+    pdi, img, P, O, x_true, y, bkg, mult = c0
+    rng = np.random.default_rng(25)
+    xn = np.exp(rng.normal(0, 0.5, img.shape)).astype(np.float32)
+    vn = rng.uniform(-1, 1, img.shape).astype(np.float32)
+    out0 = rng.uniform(-1, 1, img.shape).astype(np.float32)
+    kappa = rng.uniform(0.5, 1.5, img.shape).astype(np.float32) if use_kappa else None
+    dev = torch.device("cuda:0")
+    kt = None if kappa is None else torch.from_numpy(kappa.reshape(-1)).to(dev)
+    loss = (QuadraticPriorLoss(P, 0.03, kappa=kt) if kind == "quadratic"
+            else RelativeDifferencePriorLoss(P, 0.03, gamma=2.0, epsilon=0.01, kappa=kt))
+    got = loss.hessian_times_input(torch.from_numpy(xn).to(dev), torch.from_numpy(vn).to(dev),
+                                   out=torch.from_numpy(out0.copy()).to(dev), approximate=approximate)
+    ref = prior_hessian_times_input("approximate quadratic" if approximate else kind, xn, vn,
+                                    (img.vox_z, img.vox_y, img.vox_x), 0.03, 2.0, 0.01, kappa=kappa, out=out0)
+    assert np.array_equal(got.cpu().numpy(), ref)

@@ -196,3 +196,47 @@ def test_one_step_late_fixed_point(tiny, kind):
     print(f"{kind}: median |residual| {np.median(res) / scale:.2e}, flipped sign {np.median(flipped) / scale:.2e}")
     assert np.median(res) <= 0.2 * np.median(flipped)
     assert np.median(res) <= 0.02 * scale
+
+
+def test_rdp_hessian_two_voxels_closed_form():
+    """x = (1, 3), gamma 2, eps 0, input (1, 0): H in = (w f20(1,3), w f11(3,1))
+    = (2 (2*3)^2 / 8^3, -2 (2*3)(2*1) / 8^3) = (0.140625, -0.046875)."""
+    from oracle.oracle import prior_hessian_times_input
+    x = np.array([[[1.0, 3.0]]], np.float32)
+    out = prior_hessian_times_input("relative difference", x, np.array([[[1.0, 0.0]]], np.float32), VOX, 1.0)
+    assert out.reshape(-1).tolist() == [0.140625, -0.046875]
+
+
+@pytest.mark.parametrize("kind,use_kappa,only_2D", [("quadratic", False, False), ("quadratic", True, True),
+                                                    ("relative difference", False, False),
+                                                    ("relative difference", True, False)])
+def test_prior_hessian_is_the_derivative_of_the_gradient(kind, use_kappa, only_2D):
+    from oracle.oracle import prior_hessian_times_input
+    rng = np.random.default_rng(9)
+    x = rng.uniform(0.5, 2.0, (4, 5, 6)).astype(np.float32)
+    d = rng.uniform(-1, 1, x.shape).astype(np.float32)
+    kappa = rng.uniform(0.5, 1.5, x.shape).astype(np.float32) if use_kappa else None
+    beta, eps = 0.7, 0.01
+    hv = prior_hessian_times_input(kind, x, d, VOX, beta, 2.0, eps, kappa=kappa, only_2D=only_2D)
+    prior = quadratic_prior if kind == "quadratic" else relative_difference_prior
+    kw = {} if kind == "quadratic" else {"gamma": 2.0, "epsilon": eps}
+    t = 1e-3  # the relative difference prior's curvature changes fast (|x_j - x_k|)
+    _, gp = prior((x + t * d).astype(np.float32), VOX, beta, kappa=kappa, only_2D=only_2D, **kw)
+    _, gm = prior((x - t * d).astype(np.float32), VOX, beta, kappa=kappa, only_2D=only_2D, **kw)
+    fd = (gp.astype(np.float64) - gm) / (2 * t)
+    assert np.abs(hv - fd).max() <= 2e-3 * np.abs(fd).max()
+
+
+def test_quadratic_hessian_is_the_gradient_operator_and_approximate_hessian():
+    """Quadratic prior: H x = grad R(x) (R is quadratic); the approximate
+    Hessian of constant input 1 is beta * (sum of the neighbour weights) inside."""
+    from oracle.oracle import prior_hessian_times_input, quadratic_prior_weights
+    rng = np.random.default_rng(10)
+    x = rng.uniform(0.5, 2.0, (5, 6, 7)).astype(np.float32)
+    hx = prior_hessian_times_input("quadratic", np.zeros_like(x), x, VOX, 0.3)
+    _, g = quadratic_prior(x, VOX, 0.3)
+    assert np.abs(hx - g).max() <= 1e-5 * np.abs(g).max()
+    ones = np.ones_like(x)
+    a = prior_hessian_times_input("approximate quadratic", x, ones, VOX, 0.3)
+    w = quadratic_prior_weights(VOX)
+    assert abs(float(a[2, 3, 3]) - 0.3 * float(w.sum())) <= 1e-6 * float(w.sum())
