@@ -170,8 +170,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_group_kernel(const P
 }
 
 // Row groups: the families of one basic row (its symmetry-related
-// (segment, view, tangential) positions, all in the same subset) share every
-// row element; a wave runs up to 64 axial positions of several such families,
+// (segment, view, tangential) positions, all in the same subset; the host
+// groups them by row start and length) share every row element; a wave runs up to 64 axial positions of several such families,
 // lanes laid out family after family. Row elements are wave-uniform (scalar
 // loads) and each lane applies its own family's in-plane operation, so a
 // short family no longer leaves most of a wave idle. Each bin sums its row in

@@ -1618,15 +1618,17 @@ void batch_prepare(mi_batch* b) {
       std::vector<mi::FpTask> ht(c->task_off[s + 1] - c->task_off[s]);
       if (!ht.empty())
         HIPCHECK(hipMemcpy(ht.data(), c->tasks.p + c->task_off[s], ht.size() * sizeof(mi::FpTask), hipMemcpyDeviceToHost));
-      // families sharing a basic row, in order of first appearance
-      std::unordered_map<int, int> group_of_row;
+      // families sharing a basic row, in order of first appearance; keyed by
+      // start and length (an empty row starts where the next row does)
+      std::unordered_map<int64_t, int> group_of_row;
       std::vector<std::vector<int>> groups;
       for (size_t i = 0; i < ht.size(); ++i) {
         const mi::FpTask& t = ht[i];
         if (b->fp_rowgroup_max > 0 && c->h_fams[t.first].nax <= std::min(b->fp_rowgroup_max, 64)) {
           for (int f : {t.first, t.second}) {
             if (f < 0) continue;
-            auto it = group_of_row.emplace(c->h_fams[f].row_start, static_cast<int>(groups.size()));
+            const int64_t key = (static_cast<int64_t>(c->h_fams[f].row_start) << 32) | c->h_fams[f].row_len;
+            auto it = group_of_row.emplace(key, static_cast<int>(groups.size()));
             if (it.second) groups.emplace_back();
             groups[it.first->second].push_back(f);
           }
