@@ -221,11 +221,14 @@ def main():
     if not args.skip_map:
         result["map"] = map_bench(obj, img, nsub)
 
-    if not args.skip_listmode:
-        result["listmode"] = listmode_bench(pair, pdi, img, x, nsub)
-
     if not args.skip_autograd:
         result["autograd"] = autograd_bench(pair, obj, img, pdi, dev, args.autograd_steps, ms_per_step)
+
+    # the list-mode objective takes over the pair's context (its data become
+    # the events' histogram): it runs after everything that uses the
+    # projection-data objective
+    if not args.skip_listmode:
+        result["listmode"] = listmode_bench(pair, pdi, img, x, nsub)
 
     if not args.skip_tof:
         result["tof"] = tof_bench(dev, x, args.tof_steps, nsub, args.radius)
