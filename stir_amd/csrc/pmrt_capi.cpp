@@ -521,6 +521,12 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   have_data = have_sens = have_x = false;
   plan.reset();
   plan = std::make_unique<mi::Plan>(g, nsub);
+  // the back projection forms its gather offsets in 32 bits (bytes of one
+  // sinogram's worth of bins)
+  if (plan->num_bins() >= (int64_t{1} << 30)) {
+    plan.reset();
+    throw ApiError(MI_ERR_ARGUMENT, "more than 2^30 bins per TOF bin are not supported");
+  }
   dims.nz = g.nz; dims.ny = g.ny; dims.nx = g.nx;
   dims.min_z = g.min_z; dims.min_y = g.min_y; dims.min_x = g.min_x;
   dims.nzp = plan->nz_padded();

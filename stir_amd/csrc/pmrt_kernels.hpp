@@ -65,6 +65,19 @@ __device__ inline float load_u32off(cptr<float> base, unsigned idx) {
 }
 #endif
 
+// A table entry at a wave-uniform address, its fields made wave-uniform
+// (readfirstlane): a later per-lane choice between such entries stays a
+// select of registers instead of becoming a per-lane load of the chosen one.
+__device__ inline BpEntry load_entry_uniform(cptr<BpEntry> p) {
+  const BpEntry E = *p;
+  BpEntry r;
+  r.sino_base = __builtin_amdgcn_readfirstlane(E.sino_base);
+  r.k = __builtin_amdgcn_readfirstlane(E.k);
+  r.nax_ppa = __builtin_amdgcn_readfirstlane(E.nax_ppa);
+  r.w = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(E.w)));
+  return r;
+}
+
 template <int IP>
 __device__ inline void inplane_op(int x, int y, int& xo, int& yo) {
   if constexpr (IP == IP_ID) { xo = x; yo = y; }
@@ -126,6 +139,31 @@ __device__ inline unsigned entry_axial(int k, int nax_ppa, int zi, bool& ok) {
   }
   return static_cast<unsigned>(min(max(a, 0), nax - 1));
 }
+
+// Byte offset, within an entry's run, of the value that reaches plane zi
+// (zi4 = 4 * zi), clamped to the run, and whether the entry reaches zi at
+// all: the same test and the same value as entry_axial, with the
+// one-plane-per-axial-position case reduced to one subtraction, one compare
+// and one clamp.
+template <int PPA>
+__device__ inline unsigned entry_offset(int k, int nax_ppa, int zi, int zi4, bool& ok) {
+  const unsigned nax4 = static_cast<unsigned>(nax_ppa & 0xffff) << 2;
+  if (PPA == 1 || (PPA == 0 && (nax_ppa >> 16) == 1)) {
+    const unsigned off = static_cast<unsigned>(zi4 - 4 * k);  // negative differences wrap above nax4
+    ok = off < nax4;
+    return min(off, nax4 - 4U);
+  }
+  return entry_axial<PPA>(k, nax_ppa, zi, ok) << 2;
+}
+
+__device__ inline float load_byteoff(const float* base, unsigned off) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline float load_byteoff(cptr<float> base, unsigned off) {
+  return *(cptr<float>)((const MI_AS_CONST char*)base + off);
+}
+#endif
 
 // work index -> image column, visiting the image in tile.x x tile.y tiles
 // (columns of one tile run at the same time and share projection-data runs);

@@ -159,8 +159,13 @@ template <int MODE, int PPA, int NSLOT, bool SKIP>
 __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> ent, int sb, int se, cptr<float> proj,
                                  float* __restrict__ img, const ImageDims& d, const BpData& data, int lane) {
   float acc[NSLOT];
+  int zi4[NSLOT];  // 4 * plane of each slot
 #pragma unroll
-  for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
+  for (int q = 0; q < NSLOT; ++q) {
+    acc[q] = 0.F;
+    zi4[q] = 4 * (d.min_z + lane + kWave * q);
+    asm volatile("" : "+v"(zi4[q]));  // opaque: keeps zi4 - 4k one subtraction per entry
+  }
   for (int s = sb; s < se; ++s) {
     const int64_t beg = list_off[static_cast<int64_t>(s) * d.ncols + col];
     const int64_t end = list_off[static_cast<int64_t>(s) * d.ncols + col + 1];
@@ -168,10 +173,16 @@ __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> 
     for (; e + kUnroll <= end; e += kUnroll) {
       float p[kUnroll][NSLOT], wt[kUnroll];
       bool ok[kUnroll][NSLOT];
+      BpEntry Es[kUnroll];  // all kUnroll entries requested before any is decoded
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) Es[j] = ent[e + j];
 #pragma unroll
       for (int j = 0; j < kUnroll; ++j) {
-        const BpEntry E = ent[e + j];
-        cptr<float> run = proj + E.sino_base;  // wave-uniform
+        const BpEntry& E = Es[j];
+        // the run's byte offset joins the lane's offset (one 32-bit add), so
+        // the gather addresses from the kernel-wide base: the projection
+        // data hold fewer than 2^30 values (checked at set-up)
+        const unsigned run4 = static_cast<unsigned>(E.sino_base) << 2;
         wt[j] = E.w;
         // z range the entry's run can reach (wave-uniform): [k, k + ppa*(nax-1)]
         const int ppa_e = PPA > 0 ? PPA : (E.nax_ppa >> 16);
@@ -179,8 +190,8 @@ __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> 
 #pragma unroll
         for (int q = 0; q < NSLOT; ++q) {
           if (!SKIP || (zhi >= kWave * q && zlo < kWave * (q + 1))) {
-            const unsigned a = entry_axial<PPA>(E.k, E.nax_ppa, d.min_z + lane + kWave * q, ok[j][q]);
-            p[j][q] = load_u32off(run, a);
+            const unsigned off = entry_offset<PPA>(E.k, E.nax_ppa, d.min_z + lane + kWave * q, zi4[q], ok[j][q]);
+            p[j][q] = load_byteoff(proj, run4 + off);
           } else {
             ok[j][q] = false;
             p[j][q] = 0.F;
@@ -195,12 +206,12 @@ __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> 
     }
     for (; e < end; ++e) {
       const BpEntry E = ent[e];
-      cptr<float> run = proj + E.sino_base;
+      const unsigned run4 = static_cast<unsigned>(E.sino_base) << 2;
 #pragma unroll
       for (int q = 0; q < NSLOT; ++q) {
         bool ok;
-        const unsigned a = entry_axial<PPA>(E.k, E.nax_ppa, d.min_z + lane + kWave * q, ok);
-        if (ok) acc[q] = acc[q] + E.w * load_u32off(run, a);
+        const unsigned off = entry_offset<PPA>(E.k, E.nax_ppa, d.min_z + lane + kWave * q, zi4[q], ok);
+        if (ok) acc[q] = acc[q] + E.w * load_byteoff(proj, run4 + off);
       }
     }
   }
@@ -247,10 +258,10 @@ __device__ inline void bp_column_parts(int col, cptr<int64_t> list_off, cptr<BpE
       bool ok[U];
 #pragma unroll
       for (int j = 0; j < U; ++j) {
-        BpEntry E = ent[e + j * NPART];
+        BpEntry E = load_entry_uniform(ent + e + j * NPART);
 #pragma unroll
         for (int q = 1; q < NPART; ++q) {
-          const BpEntry Eq = ent[e + j * NPART + q];
+          const BpEntry Eq = load_entry_uniform(ent + e + j * NPART + q);
           if (part == q) E = Eq;
         }
         wt[j] = E.w;
@@ -263,11 +274,11 @@ __device__ inline void bp_column_parts(int col, cptr<int64_t> list_off, cptr<BpE
     }
     for (; e < end; e += NPART) {
       const int64_t n = end - e;  // entries left (wave-uniform)
-      BpEntry E = ent[e];
+      BpEntry E = load_entry_uniform(ent + e);
 #pragma unroll
       for (int q = 1; q < NPART; ++q) {
         if (q < n) {
-          const BpEntry Eq = ent[e + q];
+          const BpEntry Eq = load_entry_uniform(ent + e + q);
           if (part == q) E = Eq;
         }
       }
@@ -296,6 +307,88 @@ __device__ inline void bp_column_parts(int col, cptr<int64_t> list_off, cptr<BpE
   }
 }
 
+// Back projection of one image column of a table whose families all have 2
+// planes per axial position (span 1), nzp <= 64. An entry with z offset k
+// reaches the planes k + 2a only, all of the parity of k - min_z. The wave is
+// split into groups of GL lanes: group g takes the parity g & 1 and is part
+// g >> 1 of that parity's P = 32 / GL parts; its lane i holds the plane
+// 2i + parity. The column's list is read in chunks of 64 entries, one entry
+// per lane (one vector load), and written to the wave's 64 LDS slots
+// compacted by parity (the chunk's even entries in table order, then its odd
+// ones); step t gives part q of a parity that parity's entry t*P + q (one LDS
+// read per lane). With P == 1 every plane adds its entries in table order, as
+// bp_column does (identical sums); with P > 1 each part adds every P-th entry
+// of its parity and the parts' sums are added in part order.
+template <int MODE, int GL>
+__device__ inline void bp_column_parity(int col, cptr<int64_t> list_off, cptr<BpEntry> ent, int sb, int se,
+                                        cptr<float> proj, float* __restrict__ img, const ImageDims& d,
+                                        const BpData& data, int lane, int4* __restrict__ slots) {
+  constexpr int P = 32 / GL;
+  constexpr int U = 4;  // steps whose gathers are in flight together
+  const int g = lane / GL;
+  const int par = g & 1, part = g >> 1;
+  const int zr = 2 * (lane % GL) + par;  // plane relative to min_z
+  const int zi = d.min_z + zr;
+  float acc = 0.F;
+  for (int s = sb; s < se; ++s) {
+    const int64_t beg = list_off[static_cast<int64_t>(s) * d.ncols + col];
+    const int64_t end = list_off[static_cast<int64_t>(s) * d.ncols + col + 1];
+    for (int64_t c0 = beg; c0 < end; c0 += kWave) {
+      const int64_t left = end - c0;
+      const int n = left < kWave ? static_cast<int>(left) : kWave;  // wave-uniform, >= 1
+      int4 mine = make_int4(0, 0, 0, 0);
+      if (lane < n) mine = reinterpret_cast<cptr<int4>>(ent)[c0 + lane];
+      const bool odd = ((mine.y - d.min_z) & 1) != 0;
+      const uint64_t m1 = __ballot(lane < n && odd);
+      const uint64_t m0 = __ballot(lane < n && !odd);
+      const int ne = __popcll(m0), no = __popcll(m1);  // wave-uniform
+      const uint64_t mm = odd ? m1 : m0;
+      const int below = __popcll(mm & ((uint64_t{1} << lane) - 1));
+      if (lane < n) slots[odd ? ne + below : below] = mine;
+      // the wave's own LDS writes precede its reads (in-order LDS queue)
+      const int cnt = par ? no : ne;  // this lane's parity
+      const int first = par ? ne : 0;
+      const int nsteps = ((ne > no ? ne : no) + P - 1) / P;
+      for (int t = 0; t < nsteps; t += U) {
+        float pv[U], wv[U];
+        bool okv[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          const int r = (t + j) * P + part;  // entry of this parity for this lane's part
+          const bool valid = r < cnt;
+          const int4 E = slots[valid ? first + r : 0];  // slot 0 always holds an entry of this chunk
+          const int a = (zi - E.y) >> 1;  // zi - k is even for the entries of this parity
+          const int nax = E.z & 0xffff;
+          okv[j] = valid && static_cast<unsigned>(a) < static_cast<unsigned>(nax);
+          const int ac = min(max(a, 0), nax - 1);
+          pv[j] = load_u32off(proj + E.x, static_cast<unsigned>(ac));
+          wv[j] = __int_as_float(E.w);
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+          if (okv[j]) acc = acc + wv[j] * pv[j];
+      }
+    }
+  }
+  // parts' sums in part order, on the lanes of part 0
+  float total = acc;
+#pragma unroll
+  for (int q = 1; q < P; ++q) total = total + __shfl(acc, lane + 2 * q * GL, kWave);
+  const int z = zr;
+  if (part != 0 || z >= d.nzp) return;
+  gptr<float> out = as_global(img) + static_cast<int64_t>(col) * d.nzp;
+  if constexpr (MODE == BP_PLAIN) {
+    out[z] = z < d.nz ? total : 0.F;
+  } else {
+    if (z < d.nz) {
+      const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
+      const float u = osem_factor(total, as_const(data.sens)[idx], data.threshold, data.thr_lo, data.thr_hi);
+      as_global(data.estimate)[idx] = as_global(data.estimate)[idx] * u;
+      if (data.update) as_global(data.update)[idx] = u;
+    }
+  }
+}
+
 template <int MODE, int PPA, int NSLOT, bool SKIP>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t* __restrict__ list_off,
                                                                    const BpEntry* __restrict__ ent, int sb, int se,
@@ -308,7 +401,13 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t
   const int col = tiled_column(w, d, tile);
   if (col < 0) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
-  bp_column<MODE, PPA, NSLOT, SKIP>(col, as_const(list_off), as_const(ent), sb, se, as_const(proj), img, d, data, lane);
+  if constexpr (PPA == 2 && NSLOT == 1) {
+    __shared__ int4 slots[kWavesPerBlock * kWave];
+    bp_column_parity<MODE, 32>(col, as_const(list_off), as_const(ent), sb, se, as_const(proj), img, d, data, lane,
+                               slots + wave * kWave);
+  } else
+    bp_column<MODE, PPA, NSLOT, SKIP>(col, as_const(list_off), as_const(ent), sb, se, as_const(proj), img, d, data,
+                                      lane);
 }
 
 // ---- batches: one work item per wave, the problem's descriptor read
@@ -358,7 +457,11 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const P
   bd.estimate = P.estimate;
   bd.update = nullptr;
   bd.threshold = threshold;
-  if constexpr (NPART > 1)
+  if constexpr (PPA == 2 && NSLOT == 1) {  // NPART 1, 2, 4: groups of 32, 16, 8 lanes per parity part
+    __shared__ int4 slots[kWavesPerBlock * kWave];
+    bp_column_parity<MODE, 32 / NPART>(col, as_const(P.list_off), as_const(P.entries), subset, subset + 1,
+                                       as_const(P.bp_in), P.bp_out, P.d, bd, lane, slots + wave * kWave);
+  } else if constexpr (NPART > 1)
     bp_column_parts<MODE, PPA, NPART>(col, as_const(P.list_off), as_const(P.entries), subset, subset + 1,
                                       as_const(P.bp_in), P.bp_out, P.d, bd, lane);
   else
