@@ -85,10 +85,10 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_group_kernel(const P
       decode(el.x, x, y, z);
       const int xo = ca * x + cb * y, yo = cc * x + ce * y;
       const int col = has ? (yo - d.min_y) * d.nx + (xo - d.min_x) : 0;
-      const int zz = zs * z + zb;
+      const int zz = __mul24(zs, z) + zb;  // zs = +-1, |z| < 2^11
       ok[j] = valid && has && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
       w[j] = __int_as_float(el.y);
-      v[j] = img[static_cast<int64_t>(col) * d.nzp + min(max(zz, 0), d.nz - 1)];
+      v[j] = img[static_cast<int64_t>(col) * d.nzp + clamp_index(zz, d.nz)];
     }
 #pragma unroll
     for (int j = 0; j < kGroupUnroll; ++j)
@@ -192,7 +192,10 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_rowgroup_kernel(cons
   cptr<Family> fams = as_const(P.fams);
   cptr<int32_t> list = as_const(item_fams) + it.first;
   // this lane's family and axial position
-  int ca = 0, cb = 0, cc = 0, ce = 0, zs = 1, zb = 0, vg = 0;
+  // byte offset of the voxel column of element (x, y) for this lane's
+  // family: A4 x + B4 y + C4 (the in-plane operation folded into the
+  // column index); lanes without a bin keep 0 and a z offset no element reaches
+  int A4 = 0, B4 = 0, C4 = 0, zs = 1, zb = 1 << 20, vg = 0;
   int64_t out = 0;
   bool valid = false;
   int off = 0, row_start = 0, row_len = 0;
@@ -204,7 +207,12 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_rowgroup_kernel(cons
     }
     const int a = lane - off;
     if (a >= 0 && a < f.nax) {
+      int ca, cb, cc, ce;  // xo = ca x + cb y, yo = cc x + ce y
       inplane_coeffs(f.inplane, ca, cb, cc, ce);
+      const int nzp4 = 4 * d.nzp;
+      A4 = (cc * d.nx + ca) * nzp4;
+      B4 = (ce * d.nx + cb) * nzp4;
+      C4 = (-d.min_y * d.nx - d.min_x) * nzp4;
       zs = f.zsign;
       zb = f.zadd0 + f.ppa * a - d.min_z;
       out = static_cast<int64_t>(f.sino_base) + a;
@@ -225,12 +233,12 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_rowgroup_kernel(cons
       const int2 el = row[e + j];
       int x, y, z;
       decode(el.x, x, y, z);
-      const int xo = ca * x + cb * y, yo = cc * x + ce * y;
-      const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-      const int zz = zs * z + zb;
-      ok[j] = valid && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
+      // |A4|, |B4| <= 4 nx nzp < 2^23 and |x|, |y| < 2^9: 24-bit multiplies
+      const unsigned col4 = static_cast<unsigned>(__mul24(A4, x) + __mul24(B4, y) + C4);
+      const int zz = __mul24(zs, z) + zb;  // zs = +-1, |z| < 2^11
+      ok[j] = static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
       w[j] = __int_as_float(el.y);
-      v[j] = img[static_cast<int64_t>(valid ? col : 0) * d.nzp + min(max(zz, 0), d.nz - 1)];
+      v[j] = load_byteoff(img, col4 + (static_cast<unsigned>(clamp_index(zz, d.nz)) << 2));
     }
 #pragma unroll
     for (int j = 0; j < kGroupUnroll; ++j)
@@ -240,11 +248,10 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_rowgroup_kernel(cons
     const int2 el = row[e];
     int x, y, z;
     decode(el.x, x, y, z);
-    const int xo = ca * x + cb * y, yo = cc * x + ce * y;
-    const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-    const int zz = zs * z + zb;
-    if (valid && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz))
-      acc = acc + img[static_cast<int64_t>(col) * d.nzp + zz] * __int_as_float(el.y);
+    const unsigned col4 = static_cast<unsigned>(__mul24(A4, x) + __mul24(B4, y) + C4);
+    const int zz = __mul24(zs, z) + zb;  // zs = +-1, |z| < 2^11
+    if (static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz))
+      acc = acc + load_byteoff(img, col4 + (static_cast<unsigned>(zz) << 2)) * __int_as_float(el.y);
   }
   if (valid) {
     double ll = 0.0;

@@ -530,6 +530,11 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   dims.nz = g.nz; dims.ny = g.ny; dims.nx = g.nx;
   dims.min_z = g.min_z; dims.min_y = g.min_y; dims.min_x = g.min_x;
   dims.nzp = plan->nz_padded();
+  // the projections form image byte offsets in 32 bits (signed in the row groups)
+  if (static_cast<int64_t>(g.ny) * g.nx * dims.nzp >= (int64_t{1} << 29)) {
+    plan.reset();
+    throw ApiError(MI_ERR_ARGUMENT, "images of 2^29 or more (z-padded) voxels are not supported");
+  }
   dims.ncols = plan->num_columns();
   ntof = plan->num_tof_bins();
   block_bins = plan->num_bins();

@@ -165,6 +165,9 @@ __device__ inline float load_byteoff(cptr<float> base, unsigned off) {
 }
 #endif
 
+// i clamped to [0, n - 1]
+__device__ inline int clamp_index(int i, int n) { return min(max(i, 0), n - 1); }
+
 // work index -> image column, visiting the image in tile.x x tile.y tiles
 // (columns of one tile run at the same time and share projection-data runs);
 // returns -1 for the padding of partial tiles. A 1 x 1 tile is raster order.

@@ -51,14 +51,17 @@ __device__ inline void fp_row(cptr<int2> row, int n, const LaneBins<NSLOT>& L, c
       decode(el.x, x, y, z);
       inplane_op<IP>(x, y, xo, yo);
       const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-      cptr<float> colp = img + static_cast<int64_t>(col) * d.nzp;  // wave-uniform
+      // the column's byte offset (wave-uniform, 32 bits: images hold fewer
+      // than 2^29 padded voxels, checked at set-up) joins the lane's offset,
+      // so every gather addresses from the kernel-wide image base
+      const unsigned col4 = static_cast<unsigned>(col * d.nzp) << 2;
       w[j] = __int_as_float(el.y);
 #pragma unroll
       for (int q = 0; q < NSLOT; ++q) {
-        const int zz = L.zs[q] * z + L.zb[q];
+        const int zz = __mul24(L.zs[q], z) + L.zb[q];  // zs = +-1, |z| < 2^11
         ok[j][q] = L.valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
-        const int zc = min(max(zz, 0), d.nz - 1);
-        v[j][q] = load_u32off(colp, static_cast<unsigned>(zc));
+        const int zc = clamp_index(zz, d.nz);
+        v[j][q] = load_byteoff(img, col4 + (static_cast<unsigned>(zc) << 2));
       }
     }
 #pragma unroll
@@ -73,12 +76,12 @@ __device__ inline void fp_row(cptr<int2> row, int n, const LaneBins<NSLOT>& L, c
     decode(el.x, x, y, z);
     inplane_op<IP>(x, y, xo, yo);
     const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-    cptr<float> colp = img + static_cast<int64_t>(col) * d.nzp;
+    const unsigned col4 = static_cast<unsigned>(col * d.nzp) << 2;
 #pragma unroll
     for (int q = 0; q < NSLOT; ++q) {
-      const int zz = L.zs[q] * z + L.zb[q];
+      const int zz = __mul24(L.zs[q], z) + L.zb[q];  // zs = +-1, |z| < 2^11
       if (L.valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz))
-        acc[q] = acc[q] + load_u32off(colp, static_cast<unsigned>(zz)) * __int_as_float(el.y);
+        acc[q] = acc[q] + load_byteoff(img, col4 + (static_cast<unsigned>(zz) << 2)) * __int_as_float(el.y);
     }
   }
 }

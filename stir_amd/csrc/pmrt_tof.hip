@@ -152,9 +152,9 @@ __device__ void fp_tof_task(const Family& f, const FamilyLor& lor, const int2* _
         const float* colp = img + static_cast<int64_t>((y - d.min_y) * d.nx + (x - d.min_x)) * d.nzp;
 #pragma unroll
         for (int q = 0; q < NSLOT; ++q) {
-          const int zz = zs * z + zb[q];
+          const int zz = __mul24(zs, z) + zb[q];  // zs = +-1, |z| < 2^11
           ok[u][q] = valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
-          v[u][q] = load_u32off(colp, static_cast<unsigned>(min(max(zz, 0), d.nz - 1)));
+          v[u][q] = load_u32off(colp, static_cast<unsigned>(clamp_index(zz, d.nz)));
         }
       }
 #pragma unroll
@@ -183,9 +183,9 @@ __device__ void fp_tof_task(const Family& f, const FamilyLor& lor, const int2* _
       bool ok[NSLOT];
 #pragma unroll
       for (int q = 0; q < NSLOT; ++q) {
-        const int zz = zs * z + zb[q];
+        const int zz = __mul24(zs, z) + zb[q];  // zs = +-1, |z| < 2^11
         ok[q] = valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
-        v[q] = load_u32off(colp, static_cast<unsigned>(min(max(zz, 0), d.nz - 1)));
+        v[q] = load_u32off(colp, static_cast<unsigned>(clamp_index(zz, d.nz)));
       }
 #pragma unroll
       for (int k = 0; k < NT; ++k) {
