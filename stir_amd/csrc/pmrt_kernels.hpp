@@ -165,6 +165,21 @@ __device__ inline float load_byteoff(cptr<float> base, unsigned off) {
 }
 #endif
 
+// A run of n projection-data values as a raw buffer (stride 0; the third
+// descriptor word of the gfx9 family, as in composable_kernel's
+// CK_BUFFER_RESOURCE_3RD_DWORD) based at the data's start: a load at the run's
+// byte offset run4 (scalar offset, not range-checked) plus a lane offset
+// (range-checked against 4n) returns 0 for lane offsets outside [0, 4n),
+// negative ones included, instead of reading.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline __amdgpu_buffer_rsrc_t run_buffer(cptr<float> base, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, static_cast<short>(0), 4 * n, 0x00020000);
+}
+__device__ inline float load_run(__amdgpu_buffer_rsrc_t run, int lane_off, unsigned run4) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(run, lane_off, static_cast<int>(run4), 0));
+}
+#endif
+
 // i clamped to [0, n - 1]
 __device__ inline int clamp_index(int i, int n) { return min(max(i, 0), n - 1); }
 

@@ -190,6 +190,20 @@ __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> 
         // z range the entry's run can reach (wave-uniform): [k, k + ppa*(nax-1)]
         const int ppa_e = PPA > 0 ? PPA : (E.nax_ppa >> 16);
         const int zlo = E.k - d.min_z, zhi = zlo + ppa_e * ((E.nax_ppa & 0xffff) - 1);
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (PPA == 1 && !SKIP) {
+          // one plane per axial position: the run as a buffer of nax values,
+          // whose range check returns 0 for the planes the entry does not
+          // reach (x + w*0 == x: the same sums as skipping them)
+          const __amdgpu_buffer_rsrc_t run = run_buffer(proj, E.nax_ppa & 0xffff);
+#pragma unroll
+          for (int q = 0; q < NSLOT; ++q) {
+            ok[j][q] = true;
+            p[j][q] = load_run(run, zi4[q] - 4 * E.k, run4);
+          }
+          continue;
+        }
+#endif
 #pragma unroll
         for (int q = 0; q < NSLOT; ++q) {
           if (!SKIP || (zhi >= kWave * q && zlo < kWave * (q + 1))) {
