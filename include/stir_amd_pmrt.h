@@ -101,6 +101,7 @@ typedef struct {
   int32_t num_subsets;
   int32_t symmetry_flags[5];    /* effective 90,180,swap_segment,swap_s,shift_z */
   int32_t nz_padded;
+  int32_t planes_per_axial_position; /* shared by all segments (1 or 2), 0 when it varies */
 } mi_info;
 
 typedef struct mi_ctx mi_ctx;

@@ -69,6 +69,7 @@ class InfoStruct(ctypes.Structure):
         ("num_subsets", ctypes.c_int32),
         ("symmetry_flags", ctypes.c_int32 * 5),
         ("nz_padded", ctypes.c_int32),
+        ("planes_per_axial_position", ctypes.c_int32),
     ]
 
 

@@ -660,6 +660,7 @@ int mi_get_info(mi_ctx* ctx, mi_info* out) {
     out->num_subsets = ctx->plan->num_subsets();
     ctx->plan->symmetry_flags(out->symmetry_flags);
     out->nz_padded = ctx->dims.nzp;
+    out->planes_per_axial_position = ctx->uniform_ppa;
   });
 }
 

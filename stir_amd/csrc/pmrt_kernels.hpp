@@ -165,18 +165,18 @@ __device__ inline float load_byteoff(cptr<float> base, unsigned off) {
 }
 #endif
 
-// A run of n projection-data values as a raw buffer (stride 0; the third
-// descriptor word of the gfx9 family, as in composable_kernel's
-// CK_BUFFER_RESOURCE_3RD_DWORD) based at the data's start: a load at the run's
-// byte offset run4 (scalar offset, not range-checked) plus a lane offset
-// (range-checked against 4n) returns 0 for lane offsets outside [0, 4n),
-// negative ones included, instead of reading.
+// n floats at base + first as a raw buffer (stride 0; the third descriptor
+// word of the gfx9 family, as in composable_kernel's
+// CK_BUFFER_RESOURCE_3RD_DWORD): a load at a byte offset outside [0, 4n),
+// negative ones included, returns 0 instead of reading. The range check
+// covers the lane offset and the scalar offset together, so the buffer starts
+// at the run itself and loads pass no scalar offset.
 #if defined(__HIP_DEVICE_COMPILE__)
-__device__ inline __amdgpu_buffer_rsrc_t run_buffer(cptr<float> base, int n) {
-  return __builtin_amdgcn_make_buffer_rsrc((void*)base, static_cast<short>(0), 4 * n, 0x00020000);
+__device__ inline __amdgpu_buffer_rsrc_t run_buffer(cptr<float> base, unsigned first, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(base + first), static_cast<short>(0), 4 * n, 0x00020000);
 }
-__device__ inline float load_run(__amdgpu_buffer_rsrc_t run, int lane_off, unsigned run4) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(run, lane_off, static_cast<int>(run4), 0));
+__device__ inline float load_run(__amdgpu_buffer_rsrc_t run, int byte_off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(run, byte_off, 0, 0));
 }
 #endif
 

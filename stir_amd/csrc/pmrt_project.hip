@@ -32,15 +32,47 @@ constexpr int kUnroll = 8;      // matrix elements whose gathers are in flight t
 
 // Sum of img * w over one row in row order for every bin of the lane
 // (ProjMatrixElemsForOneBin.cxx:354-371). kUnroll elements have their
-// gathers issued before they are accumulated; lanes whose voxel is outside
-// the image read a clamped (valid) address and skip the add, so each sum is
-// the reference's, term by term.
+// gathers issued before they are accumulated; a voxel outside the image adds
+// nothing, so each sum is the reference's, term by term.
 template <int IP, int NSLOT>
 __device__ inline void fp_row(cptr<int2> row, int n, const LaneBins<NSLOT>& L, cptr<float> img, const ImageDims& d,
                               float (&acc)[NSLOT]) {
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
   int e = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Each element's image column as a raw buffer of nz values, read at the
+  // lane's 4 z: a voxel outside the image reads 0, and x + 0*w == x, so every
+  // bin still sums exactly the reference's terms. Lanes without a bin sum
+  // values that are never stored.
+  {
+    int zs4[NSLOT], zb4[NSLOT];
+#pragma unroll
+    for (int q = 0; q < NSLOT; ++q) {
+      zs4[q] = 4 * L.zs[q];
+      zb4[q] = 4 * L.zb[q];
+    }
+    for (; e + kUnroll <= n; e += kUnroll) {
+      float v[kUnroll][NSLOT], w[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) {
+        const int2 el = row[e + j];
+        int x, y, z, xo, yo;
+        decode(el.x, x, y, z);
+        inplane_op<IP>(x, y, xo, yo);
+        const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
+        const __amdgpu_buffer_rsrc_t column = run_buffer(img, static_cast<unsigned>(col * d.nzp), d.nz);
+        w[j] = __int_as_float(el.y);
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) v[j][q] = load_run(column, __mul24(zs4[q], z) + zb4[q]);
+      }
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j)
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) acc[q] = acc[q] + v[j][q] * w[j];
+    }
+  }
+#endif
   for (; e + kUnroll <= n; e += kUnroll) {
     float v[kUnroll][NSLOT], w[kUnroll];
     bool ok[kUnroll][NSLOT];
@@ -195,11 +227,11 @@ __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> 
           // one plane per axial position: the run as a buffer of nax values,
           // whose range check returns 0 for the planes the entry does not
           // reach (x + w*0 == x: the same sums as skipping them)
-          const __amdgpu_buffer_rsrc_t run = run_buffer(proj, E.nax_ppa & 0xffff);
+          const __amdgpu_buffer_rsrc_t run = run_buffer(proj, static_cast<unsigned>(E.sino_base), E.nax_ppa & 0xffff);
 #pragma unroll
           for (int q = 0; q < NSLOT; ++q) {
             ok[j][q] = true;
-            p[j][q] = load_run(run, zi4[q] - 4 * E.k, run4);
+            p[j][q] = load_run(run, zi4[q] - 4 * E.k);
           }
           continue;
         }
