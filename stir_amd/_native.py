@@ -48,7 +48,7 @@ class PriorStruct(ctypes.Structure):
         ("kappa", ctypes.c_void_p),
     ]
 
-MI_TUNE_BP_TILE_X, MI_TUNE_BP_TILE_Y, MI_TUNE_BP_XCD_REMAP, MI_TUNE_BP_SLOT_SKIP = 1, 2, 3, 4
+MI_TUNE_BP_TILE_X, MI_TUNE_BP_TILE_Y, MI_TUNE_BP_XCD_REMAP, MI_TUNE_BP_SLOT_SKIP, MI_TUNE_FP_XCD_CHUNK = 1, 2, 3, 4, 5
 
 
 class NativeError(RuntimeError):

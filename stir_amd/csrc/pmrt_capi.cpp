@@ -112,6 +112,9 @@ struct mi_ctx {
   bool tof_sensitivities = false;  // "use time-of-flight sensitivities" (...AndProjData.cxx:143)
   int64_t total_bins() const { return block_bins * ntof; }
   mi::BpTuning bp_tuning;
+  // the time-of-flight back projection: 32 x 8 tiles in hardware order (measured best for it)
+  mi::BpTuning tof_bp_tuning{32, 8, 0, 0};
+  int fp_xcd_chunk = 32;  // forward workgroups: 0 contiguous per XCD, k > 0 chunks of k dealt to the XCDs
   // kernel timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> fp_events, bp_events;
@@ -224,7 +227,7 @@ struct mi_ctx {
       mi::launch_forward_project_tof(mode, tasks.p + tb, te - tb, fams.p, lors.p, pool.p, d_img, d_out, dims, tofc, data,
                                      st);
     else
-      mi::launch_forward_project(mode, tasks.p + tb, te - tb, fams.p, pool.p, d_img, d_out, dims, data, st);
+      mi::launch_forward_project(mode, tasks.p + tb, te - tb, fams.p, pool.p, d_img, d_out, dims, data, st, fp_xcd_chunk);
     if (timing) { HIPCHECK(hipEventRecord(e1, st)); fp_events.emplace_back(e0, e1); }
   }
   // nontof: back-project one non-TOF block with the non-TOF values of the
@@ -235,7 +238,7 @@ struct mi_ctx {
     if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
     if (ntof > 1 && !nontof)
       mi::launch_back_project_tof(mode, uniform_ppa, list_off.p, entries.p, entry_d.p, subset_begin(subset),
-                                  subset_end(subset), d_proj, d_img, dims, tofc, data, bp_tuning, st);
+                                  subset_end(subset), d_proj, d_img, dims, tofc, data, tof_bp_tuning, st);
     else
       mi::launch_back_project(mode, uniform_ppa, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj,
                               d_img, dims, data, bp_tuning, st);
@@ -1450,8 +1453,15 @@ int mi_set_tuning(mi_ctx* ctx, int param, int value) {
         if (value < 1 || value > 1024) throw ApiError(MI_ERR_ARGUMENT, "tile y out of range");
         ctx->bp_tuning.tile_y = value;
         break;
-      case MI_TUNE_BP_XCD_REMAP: ctx->bp_tuning.xcd_remap = value != 0; break;
+      case MI_TUNE_BP_XCD_REMAP:
+        if (value < 0 || value > 4096) throw ApiError(MI_ERR_ARGUMENT, "XCD remap out of range");
+        ctx->bp_tuning.xcd_remap = value;
+        break;
       case MI_TUNE_BP_SLOT_SKIP: ctx->bp_tuning.slot_skip = value != 0; break;
+      case MI_TUNE_FP_XCD_CHUNK:
+        if (value < 0 || value > 4096) throw ApiError(MI_ERR_ARGUMENT, "forward XCD chunk out of range");
+        ctx->fp_xcd_chunk = value;
+        break;
       default: throw ApiError(MI_ERR_ARGUMENT, "unknown tuning parameter");
     }
   });
@@ -1848,7 +1858,7 @@ int mi_batch_tune(mi_batch* b, int* choice) {
     HIPCHECK(hipSetDevice(b->device));
     struct Cand { int rg, gmax, widen, pack, tx, ty; };
     const Cand cands[] = {{32, 32, 0, 2, 32, 8}, {32, 32, 0, 0, 32, 8}, {16, 32, 0, 2, 32, 8}, {0, 32, 0, 2, 32, 8},
-                          {0, 0, 0, 0, 32, 8},   {32, 32, 0, 2, 8, 8},  {32, 32, 0, 2, 64, 4}, {32, 32, 0, 1, 32, 8}};
+                          {0, 0, 0, 0, 32, 8},   {32, 32, 0, 2, 8, 8},  {32, 32, 0, 2, 16, 16}, {32, 32, 0, 1, 32, 8}};
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));

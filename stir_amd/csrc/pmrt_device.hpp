@@ -106,16 +106,16 @@ void launch_pack_rows(int first, int count, const uint64_t* scratch_k, const flo
 // ---- projections (pmrt_project.hip)
 void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams, const int2* d_pool,
                             const float* d_img, float* d_out, const ImageDims& dims, const FpData& data,
-                            hipStream_t stream);
+                            hipStream_t stream, int xcd_chunk = 0);
 // Launch shape of the back projection: image columns are visited in
 // tile_x x tile_y tiles (1 x 1 = raster order), optionally remapped so each XCD
 // gets a contiguous range of tiles, optionally skipping z slots an entry cannot reach.
 // Defaults: 32 x 8 tiles in launch order (the fastest shape measured at mMR
 // size: 13.5 ms against 18.1 ms for raster order with the XCD remap).
 struct BpTuning {
-  int tile_x = 32;
-  int tile_y = 8;
-  int xcd_remap = 0;
+  int tile_x = 16;
+  int tile_y = 16;
+  int xcd_remap = 64;  // 0: hardware order, 1: contiguous ranges per XCD, k > 1: chunks of k blocks dealt to the XCDs
   int slot_skip = 0;
 };
 

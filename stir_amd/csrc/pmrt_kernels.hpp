@@ -26,6 +26,18 @@ __device__ inline int xcd_remap(int bid, int nblocks) {
   return start + idx;
 }
 
+// blockIdx -> work index in chunks of `chunk` consecutive blocks dealt to
+// the XCDs in turn (chunk c runs on XCD c % 8): each XCD works through whole
+// chunks (neighbouring image columns that share projection-data runs in its
+// L2) while every XCD gets chunks from all over the work range (even load).
+// The grid is padded to a multiple of 8 * chunk; indices past the work are
+// dropped by the caller's bounds check.
+__device__ inline int xcd_chunk_remap(int bid, int chunk) {
+  const int xcd = bid % kNumXcd;
+  const int i = bid / kNumXcd;
+  return ((i / chunk) * kNumXcd + xcd) * chunk + i % chunk;
+}
+
 // Address spaces. Tables and inputs that a kernel only reads are accessed
 // through the constant address space, so that reads at wave-uniform
 // addresses become scalar loads; outputs through the global one. Kernel

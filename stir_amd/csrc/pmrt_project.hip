@@ -29,6 +29,9 @@ namespace {
 using namespace dev;
 
 constexpr int kUnroll = 8;      // matrix elements whose gathers are in flight together
+// batched back projection: chunks of 64 workgroups (256 columns of one
+// problem, in its tile order) dealt to the XCDs in turn
+constexpr int kBatchXcdChunk = 64;
 
 // Sum of img * w over one row in row order for every bin of the lane
 // (ProjMatrixElemsForOneBin.cxx:354-371). kUnroll elements have their
@@ -165,9 +168,10 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask*
                                                                    const Family* __restrict__ fams,
                                                                    const int2* __restrict__ pool,
                                                                    const float* __restrict__ img, float* __restrict__ out,
-                                                                   ImageDims d, FpData data) {
+                                                                   ImageDims d, FpData data, int xcd_chunk) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int t = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  const int b = xcd_chunk > 0 ? xcd_chunk_remap(blockIdx.x, xcd_chunk) : xcd_remap(blockIdx.x, gridDim.x);
+  const int t = b * kWavesPerBlock + wave;
   if (t >= ntasks) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   const FpTask task = tasks[t];
@@ -445,7 +449,10 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t
                                                                    float* __restrict__ img, ImageDims d, int nwork,
                                                                    int2 tile, int remap, BpData data) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int w = (remap ? xcd_remap(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kWavesPerBlock + wave;
+  const int b = remap == 1 ? xcd_remap(blockIdx.x, gridDim.x)
+                : remap > 1  ? xcd_chunk_remap(blockIdx.x, remap)
+                             : static_cast<int>(blockIdx.x);
+  const int w = b * kWavesPerBlock + wave;
   if (w >= nwork) return;
   const int col = tiled_column(w, d, tile);
   if (col < 0) return;
@@ -494,7 +501,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const P
                                                                          const WorkItem* __restrict__ work, int nwork,
                                                                          int subset, int threshold) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int wi = static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
+  const int wi = xcd_chunk_remap(blockIdx.x, kBatchXcdChunk) * kWavesPerBlock + wave;
   if (wi >= nwork) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   const int p = __builtin_amdgcn_readfirstlane(work[wi].problem);
@@ -573,14 +580,19 @@ __global__ void zero_end_planes_kernel(float* proj, const int64_t* off, int coun
 
 void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams, const int2* d_pool,
                             const float* d_img, float* d_out, const ImageDims& dims, const FpData& data,
-                            hipStream_t stream) {
+                            hipStream_t stream, int xcd_chunk) {
   if (ntasks <= 0) return;
-  const dim3 grid((ntasks + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  int nblocks = (ntasks + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (xcd_chunk > 0) {  // chunked remap: whole rounds of 8 chunks
+    const int round = 8 * xcd_chunk;
+    nblocks = (nblocks + round - 1) / round * round;
+  }
+  const dim3 grid(nblocks), block(kWave * kWavesPerBlock);
   switch (mode) {
 #define MI_FP_CASE(M)                                                                                            \
   case M:                                                                                                        \
     hipLaunchKernelGGL(fp_kernel<M>, grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, dims, \
-                       data);                                                                                    \
+                       data, xcd_chunk);                                                                         \
     break;
     MI_FP_CASE(FP_PLAIN)
     MI_FP_CASE(FP_RATIO)
@@ -602,7 +614,12 @@ void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const 
   const int nslot = (dims.nzp + kWave - 1) / kWave;
   const int2 tile = make_int2(tune.tile_x > 0 ? tune.tile_x : 1, tune.tile_y > 0 ? tune.tile_y : 1);
   const int nwork = ((dims.nx + tile.x - 1) / tile.x) * ((dims.ny + tile.y - 1) / tile.y) * tile.x * tile.y;
-  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  int nblocks = (nwork + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (tune.xcd_remap > 1) {  // chunked remap: whole rounds of 8 chunks
+    const int round = 8 * tune.xcd_remap;
+    nblocks = (nblocks + round - 1) / round * round;
+  }
+  const dim3 grid(nblocks), block(kWave * kWavesPerBlock);
 #define MI_BP_LAUNCH(M, P, S, K)                                                                                       \
   hipLaunchKernelGGL((bp_kernel<M, P, S, K>), grid, block, 0, stream, d_list_off, d_entries, subset_begin, subset_end, \
                      d_proj, d_img, dims, nwork, tile, tune.xcd_remap, data)
@@ -652,7 +669,9 @@ void launch_fp_batch(FpMode mode, int nslot, const ProblemDesc* d_descs, const F
 void launch_bp_batch(BpMode mode, int ppa, int nslot, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork,
                      int subset, int threshold, hipStream_t stream) {
   if (nwork <= 0) return;
-  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  const int round = 8 * kBatchXcdChunk;  // chunked XCD remap: whole rounds of 8 chunks
+  const int nblocks = ((nwork + kWavesPerBlock - 1) / kWavesPerBlock + round - 1) / round * round;
+  const dim3 grid(nblocks), block(kWave * kWavesPerBlock);
 #define MI_BPB(M, P, S, NP)                                                                                         \
   hipLaunchKernelGGL((bp_batch_kernel<M, P, S, NP>), grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold)
   // nslot < 0: one slot, -nslot parts
