@@ -277,6 +277,54 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_kernel(const int
           }
       }
       const int m = static_cast<int>(min(static_cast<int64_t>(kWave), end - c0));
+#if defined(__HIP_DEVICE_COMPILE__)
+      if constexpr (PPA == 1) {
+        // one plane per axial position: the chunk's entries come from one
+        // vector load (lane j holds entry c0 + j) and each (entry, TOF bin)
+        // run is read as a range-checked buffer of nax values, so a plane
+        // the entry does not reach reads 0 (x + w*0 == x); two entries per
+        // step keep their gathers in flight together
+        int4 mine = make_int4(0, 0, 0, 0);
+        if (lane < m) mine = reinterpret_cast<cptr<int4>>(as_const(ent))[c0 + lane];
+        int zi4[NSLOT];
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) zi4[q] = 4 * (d.min_z + lane + kWave * q);
+        for (int j = 0; j < m; j += 2) {
+          float pv[2][NT][NSLOT];
+          float wv[2][NT];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int jj = j + u < m ? j + u : j;  // a repeated entry adds nothing (its weights are zeroed)
+            const int kl = __builtin_amdgcn_readlane(klo, jj), kh = __builtin_amdgcn_readlane(khi, jj);
+            const int base = __builtin_amdgcn_readlane(mine.x, jj);
+            const int k4 = 4 * __builtin_amdgcn_readlane(mine.y, jj);
+            const int nax = __builtin_amdgcn_readlane(mine.z, jj) & 0xffff;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+              const float wk = (j + u < m && k >= kl && k <= kh) ? bcast(tw[k], jj) : 0.F;
+              wv[u][k] = wk;
+#pragma unroll
+              for (int q = 0; q < NSLOT; ++q) pv[u][k][q] = 0.F;
+              if (wk != 0.F) {  // wave-uniform: only the TOF bins of the reference's TOF row
+                const __amdgpu_buffer_rsrc_t run =
+                    run_buffer(as_const(proj), static_cast<unsigned>(static_cast<int64_t>(k) * T.block + base), nax);
+#pragma unroll
+                for (int q = 0; q < NSLOT; ++q) pv[u][k][q] = load_run(run, zi4[q] - k4);
+              }
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int k = 0; k < NT; ++k)
+              if (wv[u][k] != 0.F) {
+#pragma unroll
+                for (int q = 0; q < NSLOT; ++q) acc[q] = acc[q] + wv[u][k] * pv[u][k][q];
+              }
+        }
+        continue;
+      }
+#endif
       for (int j = 0; j < m; ++j) {
         const int kl = __builtin_amdgcn_readlane(klo, j), kh = __builtin_amdgcn_readlane(khi, j);
         if (kh < 0) continue;
