@@ -227,10 +227,12 @@ __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> 
         const int ppa_e = PPA > 0 ? PPA : (E.nax_ppa >> 16);
         const int zlo = E.k - d.min_z, zhi = zlo + ppa_e * ((E.nax_ppa & 0xffff) - 1);
 #if defined(__HIP_DEVICE_COMPILE__)
+        // one plane per axial position: the run as a buffer of nax values,
+        // whose range check returns 0 for the planes the entry does not
+        // reach (x + w*0 == x: the same sums as skipping them). (Taking this
+        // path for the one-plane entries of mixed tables, behind a per-entry
+        // test, made those tables' back projection slower.)
         if constexpr (PPA == 1 && !SKIP) {
-          // one plane per axial position: the run as a buffer of nax values,
-          // whose range check returns 0 for the planes the entry does not
-          // reach (x + w*0 == x: the same sums as skipping them)
           const __amdgpu_buffer_rsrc_t run = run_buffer(proj, static_cast<unsigned>(E.sino_base), E.nax_ppa & 0xffff);
 #pragma unroll
           for (int q = 0; q < NSLOT; ++q) {

@@ -1,10 +1,12 @@
 """Tables whose segments all have one image plane per axial position (span
-3, no single-ring-difference segment), GPU against the oracle: the kernels'
-range-checked run and column reads (a plane an entry does not reach reads 0).
+3, no single-ring-difference segment), and a mixed one, GPU against the
+oracle: the kernels' range-checked run and column reads (a plane an entry
+does not reach reads 0).
 
-Two scanners: 8 rings x 192 crystals (15-plane image, one z slot per lane)
-and 40 rings x 64 crystals (79-plane image, two z slots per lane); span 3,
-MRD 4, 2 mm voxels. Bars: forward projection within 1e-5 relative (bit-exact
+Scanners: 8 rings x 192 crystals (15-plane image, one z slot per lane) and
+40 rings x 64 crystals (79-plane image, two z slots per lane), span 3, MRD 4;
+the 8-ring scanner with MRD 5 (outer segments of one ring difference, 2
+planes per axial position: a mixed table); 2 mm voxels. Bars: forward projection within 1e-5 relative (bit-exact
 expected), back projection within 1e-5 of the maximum, OSEM within 1e-4.
 """
 import numpy as np
@@ -17,16 +19,23 @@ from stir_amd.projectors import (OSMAPOSLReconstruction, PoissonLogLikelihoodWit
 
 pytestmark = pytest.mark.gpu
 
+# name: scanner, views, tangential positions, image, maximum ring difference,
+# planes per axial position shared by all segments (0: mixed)
 CASES = {
-    "8rings": (Scanner.cylindrical(8, 192, 300.0, 4.0, name="cyl8x192"), 96, 128, VoxelsOnCartesianGrid(15, 128, 128, 2.0, 2.0, 2.0)),
-    "40rings": (Scanner.cylindrical(40, 64, 100.0, 4.0, name="cyl40x64"), 32, 48, VoxelsOnCartesianGrid(79, 48, 48, 2.0, 2.0, 2.0)),
+    "8rings": (Scanner.cylindrical(8, 192, 300.0, 4.0, name="cyl8x192"), 96, 128,
+               VoxelsOnCartesianGrid(15, 128, 128, 2.0, 2.0, 2.0), 4, 1),
+    "40rings": (Scanner.cylindrical(40, 64, 100.0, 4.0, name="cyl40x64"), 32, 48,
+                VoxelsOnCartesianGrid(79, 48, 48, 2.0, 2.0, 2.0), 4, 1),
+    # MRD 5: the outer segments hold ring difference 5 only (2 planes per axial position)
+    "8rings_mixed": (Scanner.cylindrical(8, 192, 300.0, 4.0, name="cyl8x192"), 96, 128,
+                     VoxelsOnCartesianGrid(15, 128, 128, 2.0, 2.0, 2.0), 5, 0),
 }
 
 
 @pytest.fixture(scope="module", params=sorted(CASES))
 def case(request):
-    sc, nv, nt, img = CASES[request.param]
-    pdi = ProjDataInfoCylindrical.construct(sc, span=3, max_delta=4, num_views=nv, num_tangential_poss=nt)
+    sc, nv, nt, img, mrd, ppa = CASES[request.param]
+    pdi = ProjDataInfoCylindrical.construct(sc, span=3, max_delta=mrd, num_views=nv, num_tangential_poss=nt)
     pm = RayTracingMatrixParameters()
     print(f"[span3] {request.param}: {pdi.num_bins} bins, image {img.shape}", flush=True)
     P = ProjectorByBinPairMI355X(ProjMatrixByBinUsingRayTracing(pm))
@@ -39,7 +48,7 @@ def case(request):
 def test_one_plane_per_axial_position(case):
     name, pdi, img, P, O = case
     info = P.info()
-    assert info.planes_per_axial_position == 1
+    assert info.planes_per_axial_position == CASES[name][5]
     assert info.nz_padded > 64 if name == "40rings" else info.nz_padded <= 64
 
 
