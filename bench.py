@@ -59,7 +59,8 @@ def main():
     ap.add_argument("--skip-listmode", action="store_true")
     ap.add_argument("--tof-steps", type=int, default=3)
     ap.add_argument("--autograd-steps", type=int, default=3)
-    ap.add_argument("--cpu-subsets", type=int, default=1, help="subsets of 21 in the CPU baseline sample")
+    ap.add_argument("--cpu-subsets", type=int, default=14,
+                    help="subsets of 21 in the CPU baseline sample (about 10-15 s on 16 cores)")
     ap.add_argument("--radius", type=float, default=328.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB read + 256 MiB write element-wise kernel first (known traffic for rocprof counters)")
