@@ -29,9 +29,6 @@ namespace {
 using namespace dev;
 
 constexpr int kUnroll = 8;      // matrix elements whose gathers are in flight together
-// batched back projection: chunks of 64 workgroups (256 columns of one
-// problem, in its tile order) dealt to the XCDs in turn
-constexpr int kBatchXcdChunk = 64;
 
 // Sum of img * w over one row in row order for every bin of the lane
 // (ProjMatrixElemsForOneBin.cxx:354-371). kUnroll elements have their
@@ -503,7 +500,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const P
                                                                          const WorkItem* __restrict__ work, int nwork,
                                                                          int subset, int threshold) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int wi = xcd_chunk_remap(blockIdx.x, kBatchXcdChunk) * kWavesPerBlock + wave;
+  const int wi = static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
   if (wi >= nwork) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   const int p = __builtin_amdgcn_readfirstlane(work[wi].problem);
@@ -671,9 +668,7 @@ void launch_fp_batch(FpMode mode, int nslot, const ProblemDesc* d_descs, const F
 void launch_bp_batch(BpMode mode, int ppa, int nslot, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork,
                      int subset, int threshold, hipStream_t stream) {
   if (nwork <= 0) return;
-  const int round = 8 * kBatchXcdChunk;  // chunked XCD remap: whole rounds of 8 chunks
-  const int nblocks = ((nwork + kWavesPerBlock - 1) / kWavesPerBlock + round - 1) / round * round;
-  const dim3 grid(nblocks), block(kWave * kWavesPerBlock);
+  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
 #define MI_BPB(M, P, S, NP)                                                                                         \
   hipLaunchKernelGGL((bp_batch_kernel<M, P, S, NP>), grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold)
   // nslot < 0: one slot, -nslot parts
