@@ -439,6 +439,19 @@ void mi_ctx::build_tasks() {
       if (partner >= 0) taken[partner] = 1;
       h.push_back({f, partner});
     }
+    // launch order within the subset: by (view, tangential position), then
+    // segment. The tasks of one (view, tang) follow the same path through the
+    // image columns (only their z differs), so running them together keeps
+    // those columns in L2; each task writes its own bins, so the order does
+    // not change any projected value.
+    std::stable_sort(h.begin() + task_off[s], h.end(), [&](const mi::FpTask& a, const mi::FpTask& b) {
+      const auto& ka = keys[a.first];
+      const auto& kb = keys[b.first];
+      if (ka.view != kb.view) return ka.view < kb.view;
+      if (ka.tang != kb.tang) return ka.tang < kb.tang;
+      if (ka.seg != kb.seg) return ka.seg < kb.seg;
+      return ka.a0 < kb.a0;
+    });
   }
   task_off[nsub] = static_cast<int>(h.size());
   tasks.alloc(h.size());
