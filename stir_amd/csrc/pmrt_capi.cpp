@@ -91,6 +91,7 @@ struct mi_ctx {
   mi::ImageDims dims{};
   // matrix
   DBuf<int2> pool;
+  DBuf<int2> pool_ops;  // the rows per in-plane operation: ((column << 12) | (z + 2048), weight)
   DBuf<mi::Family> fams;
   DBuf<mi::FpTask> tasks;
   std::vector<int> task_off;  // per subset, size nsub+1
@@ -227,7 +228,8 @@ struct mi_ctx {
       mi::launch_forward_project_tof(mode, tasks.p + tb, te - tb, fams.p, lors.p, pool.p, d_img, d_out, dims, tofc, data,
                                      st);
     else
-      mi::launch_forward_project(mode, tasks.p + tb, te - tb, fams.p, pool.p, d_img, d_out, dims, data, st, fp_xcd_chunk);
+      mi::launch_forward_project(mode, tasks.p + tb, te - tb, fams.p, pool.p, d_img, d_out, dims, data, st, fp_xcd_chunk,
+                                 pool_ops.p, num_basic_elements);
     if (timing) { HIPCHECK(hipEventRecord(e1, st)); fp_events.emplace_back(e0, e1); }
   }
   // nontof: back-project one non-TOF block with the non-TOF values of the
@@ -377,6 +379,13 @@ void mi_ctx::build_rows() {
       HIPCHECK(hipMemcpyAsync(pool.p + ch.global_start, ch.data->p, ch.size * sizeof(int2), hipMemcpyDeviceToDevice, stream));
   HIPCHECK(hipStreamSynchronize(stream));
   num_basic_elements = total;
+  // the rows once per in-plane operation for the (non-TOF) forward projection
+  pool_ops.release();
+  if (ntof == 1 && total > 0) {
+    pool_ops.alloc(8 * total);
+    mi::launch_expand_rows(pool.p, total, dims, pool_ops.p, stream);
+    HIPCHECK(hipGetLastError());
+  }
   // families
   h_fams = plan->families();
   const auto& keys = plan->family_keys();
@@ -529,7 +538,7 @@ void mi_ctx::build_table() {
 void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   HIPCHECK(hipSetDevice(device));
   // release everything of a previous set-up
-  pool.release(); fams.release(); tasks.release(); list_off.release(); entries.release();
+  pool.release(); pool_ops.release(); fams.release(); tasks.release(); list_off.release(); entries.release();
   img.release(); img2.release(); sino.release(); ratio.release(); stage_img.release(); stage_sino.release();
   y.release(); add.release(); mult.release(); ones.release(); vg_small.release(); x.release(); sens.clear();
   vg_off.release(); vg_len.release(); partial.release(); prior_partial.release();
@@ -546,10 +555,12 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   dims.nz = g.nz; dims.ny = g.ny; dims.nx = g.nx;
   dims.min_z = g.min_z; dims.min_y = g.min_y; dims.min_x = g.min_x;
   dims.nzp = plan->nz_padded();
-  // the projections form image byte offsets in 32 bits (signed in the row groups)
-  if (static_cast<int64_t>(g.ny) * g.nx * dims.nzp >= (int64_t{1} << 29)) {
+  // the projections form image byte offsets in 32 bits (signed in the row
+  // groups); the forward rows hold column indices in 20 bits
+  if (static_cast<int64_t>(g.ny) * g.nx * dims.nzp >= (int64_t{1} << 29) ||
+      static_cast<int64_t>(g.ny) * g.nx >= (int64_t{1} << 20)) {
     plan.reset();
-    throw ApiError(MI_ERR_ARGUMENT, "images of 2^29 or more (z-padded) voxels are not supported");
+    throw ApiError(MI_ERR_ARGUMENT, "images of 2^29 or more (z-padded) voxels or 2^20 or more columns are not supported");
   }
   dims.ncols = plan->num_columns();
   ntof = plan->num_tof_bins();

@@ -106,7 +106,11 @@ void launch_pack_rows(int first, int count, const uint64_t* scratch_k, const flo
 // ---- projections (pmrt_project.hip)
 void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams, const int2* d_pool,
                             const float* d_img, float* d_out, const ImageDims& dims, const FpData& data,
-                            hipStream_t stream, int xcd_chunk = 0);
+                            hipStream_t stream, int xcd_chunk = 0, const int2* d_ops = nullptr,
+                            int64_t nelem = 0);
+// copies of the basic rows per in-plane operation for launch_forward_project
+// (d_ops: 8 * n elements)
+void launch_expand_rows(const int2* d_pool, int64_t n, const ImageDims& dims, int2* d_ops, hipStream_t stream);
 // Launch shape of the back projection: image columns are visited in
 // tile_x x tile_y tiles (1 x 1 = raster order), optionally remapped so each XCD
 // gets a contiguous range of tiles, optionally skipping z slots an entry cannot reach.

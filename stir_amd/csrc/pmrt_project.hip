@@ -118,9 +118,61 @@ __device__ inline void fp_row(cptr<int2> row, int n, const LaneBins<NSLOT>& L, c
   }
 }
 
+// As fp_row, from the row's copy for the family's in-plane operation (see
+// expand_rows_kernel): element = ((column << 12) | (z + 2048), weight), so the
+// column needs no decoding or symmetry arithmetic, only its byte offset.
+template <int NSLOT>
+__device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& L, cptr<float> img, const ImageDims& d,
+                                  float (&acc)[NSLOT]) {
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
+  int e = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+  {
+    // image z of element z + 2048 is zs (z + 2048) + zb - 2048 zs
+    int zs4[NSLOT], zb4[NSLOT];
+#pragma unroll
+    for (int q = 0; q < NSLOT; ++q) {
+      zs4[q] = 4 * L.zs[q];
+      zb4[q] = 4 * L.zb[q] - 2048 * zs4[q];
+    }
+    for (; e + kUnroll <= n; e += kUnroll) {
+      float v[kUnroll][NSLOT], w[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) {
+        const int2 el = row[e + j];
+        const unsigned col = static_cast<unsigned>(el.x) >> 12;
+        const int zu = el.x & 0xfff;
+        const __amdgpu_buffer_rsrc_t column = run_buffer(img, col * static_cast<unsigned>(d.nzp), d.nz);
+        w[j] = __int_as_float(el.y);
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) v[j][q] = load_run(column, __mul24(zs4[q], zu) + zb4[q]);
+      }
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j)
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) acc[q] = acc[q] + v[j][q] * w[j];
+    }
+  }
+#endif
+  for (; e < n; ++e) {
+    const int2 el = row[e];
+    const int col = static_cast<int>(static_cast<unsigned>(el.x) >> 12);
+    const int z = (el.x & 0xfff) - 2048;
+    const unsigned col4 = static_cast<unsigned>(col * d.nzp) << 2;
+#pragma unroll
+    for (int q = 0; q < NSLOT; ++q) {
+      const int zz = __mul24(L.zs[q], z) + L.zb[q];  // zs = +-1, |z| < 2^11
+      if (L.valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz))
+        acc[q] = acc[q] + load_byteoff(img, col4 + (static_cast<unsigned>(zz) << 2)) * __int_as_float(el.y);
+    }
+  }
+}
+
 template <int MODE, int NSLOT>
 __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, cptr<int2> pool, cptr<float> img,
-                               float* __restrict__ out, const ImageDims& d, const FpData& data, int t, int lane) {
+                               float* __restrict__ out, const ImageDims& d, const FpData& data, int t, int lane,
+                               cptr<int2> ops = nullptr, int64_t nelem = 0) {
   LaneBins<NSLOT> L;
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) {
@@ -136,8 +188,11 @@ __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, cp
     L.vg[q] = in1 ? f1.viewgram : f2.viewgram;
     (void)f;
   }
-  cptr<int2> row = pool + f1.row_start;
   float acc[NSLOT];
+  if (ops) {
+    fp_row_ops<NSLOT>(ops + static_cast<int64_t>(f1.inplane) * nelem + f1.row_start, f1.row_len, L, img, d, acc);
+  } else {
+  cptr<int2> row = pool + f1.row_start;
   switch (f1.inplane) {
     case IP_ID: fp_row<IP_ID, NSLOT>(row, f1.row_len, L, img, d, acc); break;
     case IP_XMX: fp_row<IP_XMX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
@@ -147,6 +202,7 @@ __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, cp
     case IP_XMY_YX: fp_row<IP_XMY_YX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
     case IP_XY_YMX: fp_row<IP_XY_YMX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
     default: fp_row<IP_XMY_YMX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
+  }
   }
   double ll = 0.0;
 #pragma unroll
@@ -165,7 +221,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask*
                                                                    const Family* __restrict__ fams,
                                                                    const int2* __restrict__ pool,
                                                                    const float* __restrict__ img, float* __restrict__ out,
-                                                                   ImageDims d, FpData data, int xcd_chunk) {
+                                                                   ImageDims d, FpData data, int xcd_chunk,
+                                                                   const int2* __restrict__ ops, int64_t nelem) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
   const int b = xcd_chunk > 0 ? xcd_chunk_remap(blockIdx.x, xcd_chunk) : xcd_remap(blockIdx.x, gridDim.x);
   const int t = b * kWavesPerBlock + wave;
@@ -177,11 +234,12 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask*
   const Family f2 = fams[pair ? task.second : task.first];
   const int total = f1.nax + (pair ? f2.nax : 0);
   const int nslot = (total + kWave - 1) / kWave;
+  cptr<int2> rows = ops ? as_const(ops) : nullptr;
   switch (nslot) {
-    case 1: fp_task<MODE, 1>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane); break;
-    case 2: fp_task<MODE, 2>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane); break;
-    case 3: fp_task<MODE, 3>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane); break;
-    default: fp_task<MODE, 4>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane); break;
+    case 1: fp_task<MODE, 1>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
+    case 2: fp_task<MODE, 2>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
+    case 3: fp_task<MODE, 3>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
+    default: fp_task<MODE, 4>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
   }
 }
 
@@ -577,9 +635,47 @@ __global__ void zero_end_planes_kernel(float* proj, const int64_t* off, int coun
 
 }  // namespace
 
+// Per in-plane operation op, element e of the basic rows as
+// ops[op * n + e] = ((column << 12) | (z + 2048), weight), the column being
+// the image column of the element's (x, y) under op (0 where op leaves the
+// image: the symmetries then do not use op).
+__global__ void expand_rows_kernel(const int2* __restrict__ pool, int64_t n, ImageDims d, int2* __restrict__ ops) {
+  for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int2 el = pool[e];
+    int x, y, z;
+    decode(el.x, x, y, z);
+    for (int op = 0; op < 8; ++op) {
+      int xo, yo;
+      switch (op) {
+        case IP_ID: inplane_op<IP_ID>(x, y, xo, yo); break;
+        case IP_XMX: inplane_op<IP_XMX>(x, y, xo, yo); break;
+        case IP_YMY: inplane_op<IP_YMY>(x, y, xo, yo); break;
+        case IP_XMX_YMY: inplane_op<IP_XMX_YMY>(x, y, xo, yo); break;
+        case IP_XY_YX: inplane_op<IP_XY_YX>(x, y, xo, yo); break;
+        case IP_XMY_YX: inplane_op<IP_XMY_YX>(x, y, xo, yo); break;
+        case IP_XY_YMX: inplane_op<IP_XY_YMX>(x, y, xo, yo); break;
+        default: inplane_op<IP_XMY_YMX>(x, y, xo, yo); break;
+      }
+      const int cx = xo - d.min_x, cy = yo - d.min_y;
+      const int col = (cx >= 0 && cx < d.nx && cy >= 0 && cy < d.ny) ? cy * d.nx + cx : 0;
+      ops[op * n + e] = make_int2(static_cast<int>((static_cast<unsigned>(col) << 12) | static_cast<unsigned>(z + 2048)),
+                                  el.y);
+    }
+  }
+}
+
+void launch_expand_rows(const int2* d_pool, int64_t n, const ImageDims& dims, int2* d_ops, hipStream_t stream) {
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(expand_rows_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, d_pool, n, dims,
+                     d_ops);
+}
+
 void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams, const int2* d_pool,
                             const float* d_img, float* d_out, const ImageDims& dims, const FpData& data,
-                            hipStream_t stream, int xcd_chunk) {
+                            hipStream_t stream, int xcd_chunk, const int2* d_ops, int64_t nelem) {
   if (ntasks <= 0) return;
   int nblocks = (ntasks + kWavesPerBlock - 1) / kWavesPerBlock;
   if (xcd_chunk > 0) {  // chunked remap: whole rounds of 8 chunks
@@ -591,7 +687,7 @@ void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, cons
 #define MI_FP_CASE(M)                                                                                            \
   case M:                                                                                                        \
     hipLaunchKernelGGL(fp_kernel<M>, grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, dims, \
-                       data, xcd_chunk);                                                                         \
+                       data, xcd_chunk, d_ops, nelem);                                                           \
     break;
     MI_FP_CASE(FP_PLAIN)
     MI_FP_CASE(FP_RATIO)
