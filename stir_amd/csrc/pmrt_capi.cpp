@@ -678,7 +678,7 @@ int mi_get_info(mi_ctx* ctx, mi_info* out) {
     out->num_matrix_elements = ctx->num_matrix_elements;
     out->num_table_entries = static_cast<int64_t>(ctx->entries.n);
     out->num_nonzeros = ctx->num_nonzeros;
-    out->device_bytes = static_cast<int64_t>(ctx->pool.bytes() + ctx->fams.bytes() + ctx->tasks.bytes() +
+    out->device_bytes = static_cast<int64_t>(ctx->pool.bytes() + ctx->pool_ops.bytes() + ctx->fams.bytes() + ctx->tasks.bytes() +
                                              ctx->list_off.bytes() + ctx->entries.bytes() + ctx->img.bytes() +
                                              ctx->img2.bytes() + ctx->sino.bytes() + ctx->ratio.bytes() +
                                              ctx->stage_img.bytes() + ctx->stage_sino.bytes() + ctx->y.bytes() +
