@@ -67,7 +67,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
-                          "detail": "multi-GPU sharding is described in DESIGN.md but not launched"}))
+                          "detail": "the projector pair runs on one GPU; sharding across GPUs is designed, not launched"}))
         sys.exit(2)
 
     import numpy as np
