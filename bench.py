@@ -66,7 +66,7 @@ def main():
                     help="run one 256 MiB read + 256 MiB write element-wise kernel first (known traffic for rocprof counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "--gpus must be 1",
                           "detail": "the projector pair runs on one GPU; sharding across GPUs is designed, not launched"}))
         sys.exit(2)
 
