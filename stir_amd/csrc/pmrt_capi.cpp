@@ -1612,6 +1612,8 @@ void batch_prepare(mi_batch* b) {
     mi::ProblemDesc& d = h[p];
     d.fams = c->fams.p;
     d.pool = c->pool.p;
+    d.pool_ops = c->pool_ops.p;
+    d.nelem = c->num_basic_elements;
     d.fp_in = c->x.p;
     d.fp_out = c->ratio.p;
     d.fd = c->fp_data();

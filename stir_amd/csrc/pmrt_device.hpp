@@ -164,6 +164,8 @@ struct ProblemDesc {
   float* estimate;           // BP_OSEM_UPDATE: x, updated in place
   ImageDims d;
   int32_t ppa;               // planes per axial position shared by all families, 0 if they differ
+  const int2* pool_ops;      // the rows per in-plane operation (or null), nelem elements each
+  int64_t nelem;
 };
 // work item of a grouped launch: (problem, family) or (problem, image column);
 // index -1 is padding

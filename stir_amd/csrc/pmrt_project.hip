@@ -547,7 +547,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_batch_kernel(const P
   const Family f1 = fams[task.first];
   const bool pair = task.second >= 0;
   const Family f2 = fams[pair ? task.second : task.first];
-  fp_task<MODE, NSLOT>(f1, f2, pair, as_const(P.pool), as_const(P.fp_in), P.fp_out, P.d, P.fd, wi, lane);
+  fp_task<MODE, NSLOT>(f1, f2, pair, as_const(P.pool), as_const(P.fp_in), P.fp_out, P.d, P.fd, wi, lane,
+                       P.pool_ops ? as_const(P.pool_ops) : nullptr, P.nelem);
 }
 
 // back projection: work item = (problem, image column), subset `subset`;
