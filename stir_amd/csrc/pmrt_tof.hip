@@ -142,6 +142,37 @@ __device__ void fp_tof_task(const Family& f, const FamilyLor& lor, const int2* _
     }
     const int m = min(kWave, n - c0);
     int j = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // each element's image column as a range-checked buffer of nz values: a
+    // voxel outside the image reads 0 and adds nothing (0 * w == 0 for the
+    // finite, non-zero weights used), so no per-lane clamp, test or select;
+    // lanes without a bin sum values that are never stored
+    for (; j + kTofUnroll <= m; j += kTofUnroll) {
+      float v[kTofUnroll][NSLOT];
+#pragma unroll
+      for (int u = 0; u < kTofUnroll; ++u) {
+        int x, y, z;
+        decode(row[c0 + j + u].x, x, y, z);
+        const int col = (y - d.min_y) * d.nx + (x - d.min_x);
+        const __amdgpu_buffer_rsrc_t column =
+            run_buffer(as_const(img), static_cast<unsigned>(col * d.nzp), d.nz);
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) v[u][q] = load_run(column, 4 * (__mul24(zs, z) + zb[q]));
+      }
+#pragma unroll
+      for (int u = 0; u < kTofUnroll; ++u)
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+          if (k < T.ntof) {
+            const float wk = bcast(tw[k], j + u);
+            if (wk != 0.F) {
+#pragma unroll
+              for (int q = 0; q < NSLOT; ++q) acc[q][k] = acc[q][k] + v[u][q] * wk;
+            }
+          }
+        }
+    }
+#endif
     for (; j + kTofUnroll <= m; j += kTofUnroll) {
       float v[kTofUnroll][NSLOT];
       bool ok[kTofUnroll][NSLOT];
