@@ -202,9 +202,11 @@ int mi_obj_set_multiplicative(mi_ctx* ctx, const float* mult);
  * subset's events of row^T / fwd, minus the subset sensitivity unless
  * add_sensitivity) and mi_obj_log_likelihood (the value the reference's
  * actual_compute_objective_function_without_penalty returns: -sum over events
- * of log fwd, log(1e-4) for capped events, :748-787,874-911) work on the
- * events: fwd = row . x + additive, events with 1 > 1e4 fwd contribute no
- * gradient. mi_obj_set_data switches back to projection data. */
+ * of log fwd, log(1e-4) for events with 1 > 1e4 fwd, :756-787,876-911) work
+ * on the events, fwd = row . x + additive. Every event contributes row^T / fwd
+ * to the gradient: the reference's gradient instantiation
+ * LM_gradient_and_value<true, false> applies the 1e4 cap only to the value
+ * (:822-843). mi_obj_set_data switches back to projection data. */
 int mi_lm_set_events(mi_ctx* ctx, const int32_t* events, int64_t num_events, const float* additive,
                      float additive_constant);
 /* "use subset sensitivities" (default 1, ...ForMean.cxx:48). 0: every subset
@@ -315,17 +317,20 @@ int mi_prior_hessian_times_input_dev(mi_ctx* ctx, int type, int approximate, con
  * (OSMAPOSLReconstruction::update_estimate, OSMAPOSLReconstruction.cxx:373-516,
  * as mi_osem_subset_step); d_update (if not NULL) receives u. Needs
  * mi_obj_set_data and mi_obj_compute_sensitivities. d_x_out may equal d_x_in.
- * Prior-free with the default relative-change bounds (an error when a prior
- * is set). */
+ * Prior-free (an error when a prior is set), with the relative-change bounds
+ * of mi_osem_set_options. */
 int mi_osem_subset_step_dev(mi_ctx* ctx, const float* d_x_in, float* d_x_out, float* d_update, int subset,
                             int subiteration_num, void* stream);
 /* No counterpart in the reference (it has no derivatives of the OSEM map):
  * vector-Jacobian product of mi_osem_subset_step_dev with respect to x_in,
- *   d_grad_in = g * u + BP_s( r'(FP_s(x_in)+b) * FP_s(g * x_in / sens_s) ),
+ *   d_grad_in = g * u + BP_s( r'(FP_s(x_in)+b) * FP_s(c * g * x_in / sens_s) ),
  * r' the derivative of the truncated quotient y/ybar, g = d_grad_out, u the
- * update written by the forward step. Two forward and one back projection. */
+ * update written by the forward step, c the derivative of its clamp to the
+ * context's relative-change bounds (mi_osem_set_options): 0 where
+ * subiteration_num != 1 and u <= minimum or u >= maximum relative change,
+ * else 1. Two forward and one back projection. */
 int mi_osem_subset_step_vjp_dev(mi_ctx* ctx, const float* d_x_in, const float* d_update, const float* d_grad_out,
-                                float* d_grad_in, int subset, void* stream);
+                                float* d_grad_in, int subset, int subiteration_num, void* stream);
 
 /* Waits for all work of the context. */
 int mi_synchronize(mi_ctx* ctx);

@@ -1326,14 +1326,16 @@ static void add_background_vg(const or_plan* p, float* ybar, const float* bkg, i
     }
 }
 
-/* compute_sub_gradient_without_penalty[_plus_sensitivity]
- * -- ...AndProjData.cxx:694-735, RPC at :1423-1481 */
-void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, const float* x,
-                        int subset, int nsub, int zero_end_planes, int add_sensitivity, float* grad)
+/* The projection data RPC_process_related_viewgrams_gradient back-projects
+ * (...AndProjData.cxx:1423-1481): ybar = FP(x) + b, then
+ * divide_and_truncate(y, ybar) per viewgram, minus the multiplicative
+ * factors (or 1) unless add_sensitivity; bins outside the subset stay 0. */
+void or_subset_ratio(const or_plan* p, const float* y, const float* bkg, const float* x,
+                     int subset, int nsub, int zero_end_planes, int add_sensitivity, float* q)
 {
   const int64_t nb = p->num_bins * p->ntof;
   float* ybar = (float*)calloc((size_t)nb, sizeof(float));
-  float* q = (float*)calloc((size_t)nb, sizeof(float));
+  memset(q, 0, sizeof(float) * (size_t)nb);
   or_forward_project_omp(p, x, ybar, subset, nsub);
   vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
   for (int k = 0; k < n; ++k) {
@@ -1351,8 +1353,19 @@ void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, cons
     }
   }
   free(vs);
+  free(ybar);
+}
+
+/* compute_sub_gradient_without_penalty[_plus_sensitivity]
+ * -- ...AndProjData.cxx:694-735, RPC at :1423-1481 */
+void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, const float* x,
+                        int subset, int nsub, int zero_end_planes, int add_sensitivity, float* grad)
+{
+  const int64_t nb = p->num_bins * p->ntof;
+  float* q = (float*)malloc(sizeof(float) * (size_t)nb);
+  or_subset_ratio(p, y, bkg, x, subset, nsub, zero_end_planes, add_sensitivity, q);
   or_back_project(p, q, grad, subset, nsub);
-  free(ybar); free(q);
+  free(q);
 }
 
 /* accumulate_loglikelihood -- recon_array_functions.cxx:336-377 via
@@ -1677,10 +1690,13 @@ static int lm_event_in_range(const or_plan* p, const int32_t* e)
  * events n x (segment, view, axial, tangential, timing position); an event is
  * in subset s when its basic bin's view % num_subsets == s (num_subsets > 1);
  * fwd = row . x + additive (the additive term of the event's bin; for TOF
- * data the cache keeps that of the last timing position, .cxx:570-596);
- * 1 > 1e4 fwd: value -= log(1/1e4) and no gradient; otherwise grad += row^T
- * (1/fwd) and value -= log(fwd). add may be NULL (no additive term). grad
- * (if not NULL) is accumulated into. Returns the value. */
+ * data the cache keeps that of the last timing position, .cxx:570-596).
+ * The reference instantiates the function twice: the gradient with
+ * <do_gradient=true, do_value=false> (:822-843), whose early return for
+ * 1 > 1e4 fwd sits inside `if (do_value)`, so every event back-projects
+ * row^T (1/fwd); the value with <false, true> (:876-911): value -=
+ * log(1/1e4) when 1 > 1e4 fwd, else value -= log(fwd). add may be NULL (no
+ * additive term). grad (if not NULL) is accumulated into. Returns the value. */
 double or_lm_gradient(const or_plan* p, const int32_t* ev, int64_t n, const float* add, const float* x, int subset,
                       int nsub, float* grad)
 {
@@ -1699,12 +1715,9 @@ double or_lm_gradient(const or_plan* p, const int32_t* ev, int64_t n, const floa
     const float add_term = add ? add[tbin(p, seg, view, ax, tang, p->ntof - 1)] : 0.F;
     const float fwd = fp_bin(p, x, seg, view, ax, tang, p->ntof > 1 ? kk : -1) + add_term;
     const float num = 1.F;
-    if (num > max_quotient * fwd) {
-      value -= num * log((double)(num / max_quotient));
-      continue;
-    }
     if (grad) bp_bin(p, grad, num / fwd, seg, view, ax, tang, p->ntof > 1 ? kk : -1);
-    value -= num * log((double)fwd);
+    if (num > max_quotient * fwd) value -= num * log((double)(num / max_quotient));
+    else value -= num * log((double)fwd);
   }
   return value;
 }

@@ -155,6 +155,11 @@ void or_subset_sensitivity(const or_plan* p, int subset, int num_subsets, int ze
 /* "use time-of-flight sensitivities" (...AndProjData.cxx:143): off by default,
  * then TOF data get the non-TOF sensitivity (:533-536,648-670). */
 void or_set_use_tof_sensitivities(or_plan* p, int use);
+/* The ratio projection data of the gradient RPC: divide_and_truncate(y,
+ * FP(x) + b) per viewgram (minus mult or 1 unless add_sensitivity), the
+ * subset's bins only (others 0); q holds all bins. */
+void or_subset_ratio(const or_plan* p, const float* y, const float* bkg, const float* x,
+                     int subset, int num_subsets, int zero_end_planes, int add_sensitivity, float* q);
 /* compute_sub_gradient_without_penalty(_plus_sensitivity):
  * grad = BP_subset(y/ybar) (+sens) or BP_subset(y/ybar - 1). */
 void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, const float* x,

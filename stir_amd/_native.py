@@ -119,7 +119,7 @@ def load():
     L.mi_obj_value_and_gradient_dev.argtypes = [vp, vp, c_int, c_int, vp, vp, vp]
     L.mi_quadratic_prior_dev.argtypes = [vp, vp, vp, vp, ctypes.c_float, c_int, vp, vp, vp]
     L.mi_osem_subset_step_dev.argtypes = [vp, vp, vp, vp, c_int, c_int, vp]
-    L.mi_osem_subset_step_vjp_dev.argtypes = [vp, vp, vp, vp, vp, c_int, vp]
+    L.mi_osem_subset_step_vjp_dev.argtypes = [vp, vp, vp, vp, vp, c_int, c_int, vp]
     L.mi_set_kernel_timing.argtypes = [vp, c_int]
     L.mi_get_kernel_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]

@@ -247,17 +247,27 @@ class RelativeDifferencePriorLoss(QuadraticPriorLoss):
 
 # ---------------------------------------------------------------------------
 # unrolled OSEM
+FLT_MAX = 3.4028234663852886e38
+
+
+def _set_bounds(obj: PoissonLogLikelihoodWithLinearModelForMeanAndProjData, lo: float, hi: float) -> None:
+    # OSMAPOSL "minimum / maximum relative change" on the objective's context
+    _native.check(_native.load().mi_osem_set_options(obj.projector.context.handle, 0, ctypes.c_float(lo),
+                                                     ctypes.c_float(hi)), "mi_osem_set_options")
+
+
 class _OSEMSubiteration(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x: torch.Tensor, obj: PoissonLogLikelihoodWithLinearModelForMeanAndProjData, subset: int,
-                subiteration_num: int):
+                subiteration_num: int, bounds):
         pair = obj.projector
         x = _as_batch(x, pair.img.num_voxels, "image")[0].reshape(pair.img.shape)
         out = torch.empty_like(x)
         u = torch.empty_like(x)
+        _set_bounds(obj, *bounds)
         _native.check(_native.load().mi_osem_subset_step_dev(pair.context.handle, _ptr(x), _ptr(out), _ptr(u), subset,
                                                              subiteration_num, _stream(x)), "mi_osem_subset_step_dev")
-        ctx.obj, ctx.subset = obj, subset
+        ctx.obj, ctx.subset, ctx.subiteration_num, ctx.bounds = obj, subset, subiteration_num, bounds
         ctx.save_for_backward(x, u)
         return out
 
@@ -266,17 +276,23 @@ class _OSEMSubiteration(torch.autograd.Function):
         x, u = ctx.saved_tensors
         g = g.contiguous()
         gin = torch.empty_like(x)
+        _set_bounds(ctx.obj, *ctx.bounds)
         _native.check(_native.load().mi_osem_subset_step_vjp_dev(ctx.obj.projector.context.handle, _ptr(x), _ptr(u),
-                                                                 _ptr(g), _ptr(gin), ctx.subset, _stream(x)),
+                                                                 _ptr(g), _ptr(gin), ctx.subset, ctx.subiteration_num,
+                                                                 _stream(x)),
                       "mi_osem_subset_step_vjp_dev")
-        return gin, None, None, None
+        return gin, None, None, None, None
 
 
 def osem_subiteration(x: torch.Tensor, objective: PoissonLogLikelihoodWithLinearModelForMeanAndProjData, subset: int,
-                      subiteration_num: int) -> torch.Tensor:
+                      subiteration_num: int, minimum_relative_change: float = 0.0,
+                      maximum_relative_change: float = FLT_MAX) -> torch.Tensor:
     """One OSMAPOSL subiteration x * clamp(A_s^T(y/(A_s x + b)) / A_s^T 1)
-    (``OSMAPOSLReconstruction.cxx:373-516``), differentiable in x."""
-    return _OSEMSubiteration.apply(x, objective, subset, subiteration_num)
+    (``OSMAPOSLReconstruction.cxx:373-516``), differentiable in x. The clamp
+    to [minimum, maximum relative change] applies after subiteration 1
+    (``:491-507``); where it acts, the update does not depend on x."""
+    bounds = (float(minimum_relative_change), float(maximum_relative_change))
+    return _OSEMSubiteration.apply(x, objective, subset, subiteration_num, bounds)
 
 
 class UnrolledOSEM(torch.nn.Module):
@@ -287,15 +303,17 @@ class UnrolledOSEM(torch.nn.Module):
     forward projections. Needs ``objective.set_up`` with sensitivities."""
 
     def __init__(self, objective: PoissonLogLikelihoodWithLinearModelForMeanAndProjData, num_subiterations: int,
-                 start_subset: int = 0):
+                 start_subset: int = 0, minimum_relative_change: float = 0.0,
+                 maximum_relative_change: float = FLT_MAX):
         super().__init__()
         self.objective = objective
         self.num_subiterations = int(num_subiterations)
         self.start_subset = int(start_subset)
+        self.bounds = (float(minimum_relative_change), float(maximum_relative_change))
 
     def forward(self, x0: torch.Tensor) -> torch.Tensor:
         nsub = self.objective.projector.num_subsets
         x = x0
         for k in range(1, self.num_subiterations + 1):
-            x = osem_subiteration(x, self.objective, (k + self.start_subset - 1) % nsub, k)
+            x = osem_subiteration(x, self.objective, (k + self.start_subset - 1) % nsub, k, *self.bounds)
         return x

@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_group_kernel(const P
     as_global(P.bp_out)[idx] = zl < d.nz ? acc : 0.F;
   } else {
     if (zl < d.nz) {
-      const float u = osem_factor(acc, as_const(as_const(P.sens)[subset])[idx], threshold, 0.F, FLT_MAX);
+      const float u = osem_factor(acc, as_const(as_const(P.sens)[subset])[idx], threshold, P.thr_lo, P.thr_hi);
       as_global(P.estimate)[idx] = as_global(P.estimate)[idx] * u;
     }
   }

@@ -1225,6 +1225,8 @@ int mi_osem_subset_step_dev(mi_ctx* ctx, const float* d_x_in, float* d_x_out, fl
     bd.estimate = ctx->img.p;
     bd.update = d_update ? ctx->img2.p : nullptr;
     bd.threshold = subiteration_num != 1;
+    bd.thr_lo = ctx->min_rel;
+    bd.thr_hi = ctx->max_rel;
     ctx->bp(mi::BP_OSEM_UPDATE, ctx->ratio.p, nullptr, subset, bd, st);
     ctx->image_from_device(ctx->img.p, d_x_out, st);
     if (d_update) ctx->image_from_device(ctx->img2.p, d_update, st);
@@ -1233,7 +1235,7 @@ int mi_osem_subset_step_dev(mi_ctx* ctx, const float* d_x_in, float* d_x_out, fl
 }
 
 int mi_osem_subset_step_vjp_dev(mi_ctx* ctx, const float* d_x_in, const float* d_update, const float* d_grad_out,
-                                float* d_grad_in, int subset, void* stream) {
+                                float* d_grad_in, int subset, int subiteration_num, void* stream) {
   return guarded([&] {
     if (!ctx || !d_x_in || !d_update || !d_grad_out || !d_grad_in) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     if (d_grad_in == d_grad_out || d_grad_in == d_update)
@@ -1247,8 +1249,16 @@ int mi_osem_subset_step_vjp_dev(mi_ctx* ctx, const float* d_x_in, const float* d
     // r'(FP_s(x) + b) on the subset's bins
     ctx->image_to_device(d_x_in, ctx->img.p, st);
     ctx->fp(mi::FP_DRATIO, ctx->img.p, ctx->ratio.p, subset, ctx->fp_data(), st);
-    // v = g * x / sens_s, then FP_s(v) * r'
-    ctx->image_to_device(d_grad_out, ctx->img2.p, st);
+    // v = g * x / sens_s where the update was not clamped (the clamp of
+    // threshold_upper_lower has derivative 0 where it acts; it acts after
+    // subiteration 1 only), then FP_s(v) * r'
+    if (subiteration_num != 1) {
+      mi::launch_osem_vjp_mask(d_grad_out, d_update, ctx->stage_img.p, ctx->plan->num_voxels(), ctx->min_rel,
+                               ctx->max_rel, st);
+      ctx->image_to_device(ctx->stage_img.p, ctx->img2.p, st);
+    } else {
+      ctx->image_to_device(d_grad_out, ctx->img2.p, st);
+    }
     mi::launch_osem_vjp_scale(ctx->img2.p, ctx->img.p, ctx->sens[subset]->p, ctx->img.p, ndev, st);
     mi::FpData fd = ctx->fp_data();
     fd.scale = ctx->ratio.p;
@@ -1572,6 +1582,7 @@ struct mi_batch {
   hipGraphExec_t graph_exec = nullptr;
   int graph_key[3] = {0, 0, 0};  // start subiteration, count, start subset of the captured graph
   ~mi_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);  // queued runs still use the problems' buffers
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     for (mi_ctx* c : probs) delete c;
     if (stream) (void)hipStreamDestroy(stream);
@@ -1586,7 +1597,13 @@ int group_index_at_least(int n) {
   return -1;
 }
 
+// mi_batch_osem_run returns without waiting: anything that writes a
+// problem's buffers (on the problem's own stream), rebuilds the work lists or
+// destroys the graph first waits for the batch stream's queued work.
+void batch_wait(mi_batch* b) { HIPCHECK(hipStreamSynchronize(b->stream)); }
+
 void batch_invalidate(mi_batch* b) {
+  batch_wait(b);
   b->lists_ready = false;
   if (b->graph_exec) {
     (void)hipGraphExecDestroy(b->graph_exec);
@@ -1625,6 +1642,8 @@ void batch_prepare(mi_batch* b) {
     d.estimate = c->x.p;
     d.d = c->dims;
     d.ppa = c->uniform_ppa;
+    d.thr_lo = c->min_rel;
+    d.thr_hi = c->max_rel;
     b->sens_lists.push_back(std::move(sl));
   }
   b->descs.alloc(np);
@@ -1831,6 +1850,7 @@ int mi_batch_set_data(mi_batch* b, int problem, const float* measured, const flo
                       int zero_end_planes) {
   return guarded([&] {
     mi_ctx* c = batch_problem(b, problem);
+    HIPCHECK(hipSetDevice(b->device));
     batch_invalidate(b);
     if (mi_obj_set_data(c, measured, additive, additive_constant, zero_end_planes) != MI_OK ||
         mi_obj_compute_sensitivities(c) != MI_OK)
@@ -1841,6 +1861,8 @@ int mi_batch_set_data(mi_batch* b, int problem, const float* measured, const flo
 int mi_batch_set_estimate(mi_batch* b, int problem, const float* image, int enforce_positivity) {
   return guarded([&] {
     mi_ctx* c = batch_problem(b, problem);
+    HIPCHECK(hipSetDevice(b->device));
+    batch_wait(b);
     const bool had = c->have_x;
     if (mi_obj_set_estimate(c, image, enforce_positivity) != MI_OK) throw ApiError(MI_ERR_DEVICE, g_last_error);
     if (!had) batch_invalidate(b);  // the estimate buffer is allocated on first use
@@ -1882,6 +1904,7 @@ int mi_batch_tune(mi_batch* b, int* choice) {
   return guarded([&] {
     if (!b) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     HIPCHECK(hipSetDevice(b->device));
+    batch_wait(b);
     struct Cand { int rg, gmax, widen, pack, tx, ty; };
     const Cand cands[] = {{32, 32, 0, 2, 32, 8}, {32, 32, 0, 0, 32, 8}, {16, 32, 0, 2, 32, 8}, {0, 32, 0, 2, 32, 8},
                           {0, 0, 0, 0, 32, 8},   {32, 32, 0, 2, 8, 8},  {32, 32, 0, 2, 16, 16}, {32, 32, 0, 1, 32, 8}};
@@ -1942,6 +1965,7 @@ int mi_batch_osem_run(mi_batch* b, int start_subiteration_num, int num_subiterat
       const int key[3] = {start_subiteration_num, num_subiterations, start_subset};
       if (!b->graph_exec || std::memcmp(key, b->graph_key, sizeof(key)) != 0) {
         if (b->graph_exec) {
+          batch_wait(b);  // the previous graph may still be running
           HIPCHECK(hipGraphExecDestroy(b->graph_exec));
           b->graph_exec = nullptr;
         }

@@ -166,6 +166,7 @@ struct ProblemDesc {
   int32_t ppa;               // planes per axial position shared by all families, 0 if they differ
   const int2* pool_ops;      // the rows per in-plane operation (or null), nelem elements each
   int64_t nelem;
+  float thr_lo, thr_hi;      // OSMAPOSL "minimum / maximum relative change" of the problem
 };
 // work item of a grouped launch: (problem, family) or (problem, image column);
 // index -1 is padding
@@ -261,5 +262,9 @@ void launch_osem_vjp_scale(const float* d_g, const float* d_x, const float* d_se
                            hipStream_t stream);
 // out += g * u, n elements
 void launch_add_product(float* d_out, const float* d_g, const float* d_u, int64_t n, hipStream_t stream);
+// out = g where the update u was not clamped by threshold_upper_lower(lo, hi)
+// (lo < u < hi), 0 where it was (u <= lo or u >= hi): the clamp's derivative
+void launch_osem_vjp_mask(const float* d_g, const float* d_u, float* d_out, int64_t n, float lo, float hi,
+                          hipStream_t stream);
 
 }  // namespace mi

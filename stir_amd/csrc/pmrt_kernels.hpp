@@ -236,13 +236,17 @@ __device__ inline void fp_epilogue(float a, int64_t i, int vg, const FpData& dat
   } else if constexpr (MODE == FP_SCALE) {
     as_global(out)[i] = a * as_const(data.scale)[i];
   } else if constexpr (MODE == FP_LM_RATIO || MODE == FP_LM_VALUE) {
-    // per event (bin value 1): fwd = row . x + additive; skipped when
-    // 1 > 1e4 fwd; y events of one bin contribute y / fwd
+    // per event (bin value 1): fwd = row . x + additive; y events of one bin
+    // contribute y / fwd to the gradient -- every event, capped or not: the
+    // gradient instantiation LM_gradient_and_value<true, false> returns early
+    // only when it computes the value (...ListModeDataWithProjMatrixByBin.cxx:
+    // 756-787,822-843) -- and, to the value, log(1/1e4) in place of log(fwd)
+    // when 1 > 1e4 fwd (<false, true>, :876-911)
     const float fwd = a + (data.additive ? as_const(data.additive)[i] : data.additive_constant);
     const float yv = as_const(data.measured)[i];
     const bool capped = 1.F > 10000.F * fwd;
     if constexpr (MODE == FP_LM_RATIO) {
-      as_global(out)[i] = (yv == 0.F || capped) ? 0.F : yv / fwd;
+      as_global(out)[i] = yv == 0.F ? 0.F : yv / fwd;
     } else {
       if (yv != 0.F)
         ll -= static_cast<double>(yv) * (capped ? log(static_cast<double>(1.F / 10000.F)) : log(static_cast<double>(fwd)));

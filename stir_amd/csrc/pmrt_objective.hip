@@ -255,6 +255,15 @@ __global__ void osem_vjp_scale_kernel(const float* g, const float* x, const floa
   }
 }
 
+__global__ void osem_vjp_mask_kernel(const float* __restrict__ g, const float* __restrict__ u, float* __restrict__ out,
+                                     int64_t n, float lo, float hi) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const float ui = u[i];
+    out[i] = (ui > lo && ui < hi) ? g[i] : 0.F;
+  }
+}
+
 __global__ void add_product_kernel(float* __restrict__ out, const float* __restrict__ g, const float* __restrict__ u,
                                    int64_t n) {
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
@@ -375,6 +384,13 @@ void launch_osem_vjp_scale(const float* d_g, const float* d_x, const float* d_se
                            hipStream_t stream) {
   if (n <= 0) return;
   hipLaunchKernelGGL(osem_vjp_scale_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_g, d_x, d_sens, d_v, n);
+}
+
+void launch_osem_vjp_mask(const float* d_g, const float* d_u, float* d_out, int64_t n, float lo, float hi,
+                          hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(osem_vjp_mask_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_g, d_u, d_out, n, lo,
+                     hi);
 }
 
 void launch_add_product(float* d_out, const float* d_g, const float* d_u, int64_t n, hipStream_t stream) {

@@ -571,6 +571,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const P
   bd.estimate = P.estimate;
   bd.update = nullptr;
   bd.threshold = threshold;
+  bd.thr_lo = P.thr_lo;
+  bd.thr_hi = P.thr_hi;
   if constexpr (PPA == 2 && NSLOT == 1) {  // NPART 1, 2, 4: groups of 32, 16, 8 lanes per parity part
     __shared__ int4 slots[kWavesPerBlock * kWave];
     bp_column_parity<MODE, 32 / NPART>(col, as_const(P.list_off), as_const(P.entries), subset, subset + 1,

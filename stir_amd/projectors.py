@@ -324,7 +324,7 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndListModeDataWithProjMatrixByB
     prompt events (int32 n x 5: segment, view, axial, tangential, timing),
     an optional additive projection-data term and multiplicative factors for
     the sensitivities. Gradients and OSMAPOSL updates sum row^T / (row . x + b)
-    over the subset's events (events with 1 > 1e4 (row . x + b) add nothing);
+    over the subset's events (every event: the reference caps only the value);
     ``compute_objective_function`` returns the reference's value, -sum log(row . x + b)."""
 
     def set_up(self, events: np.ndarray, additive=0.0, multiplicative: Optional[np.ndarray] = None,

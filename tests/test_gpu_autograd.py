@@ -293,3 +293,48 @@ def test_osem_vjp_fp64_central_difference(c0):
     rhs = float(np.sum(gimg * (F(xd + h * v) - F(xd - h * v)) / (2 * h)))
     print(f"OSEM VJP vs FP64 central difference: {lhs:.8e} vs {rhs:.8e}")
     assert abs(lhs - rhs) <= 1e-4 * abs(rhs)
+
+
+def test_osem_step_and_vjp_with_relative_change_bounds(c0):
+    """Bounds 0.9 / 1.1 at subiteration 2 (they act after subiteration 1):
+    the out-of-place step equals mi_osem_subset_step bit for bit, clamps
+    some voxels at each bound, and its VJP (zero through the clamp where it
+    acts) matches FP64 central differences of the clamped subiteration."""
+    from stir_amd.projectors import OSMAPOSLReconstruction
+
+    pdi, img, P, O, obj, y, bkg = c0
+    obj.set_up(y, bkg)
+    lo, hi, subset = 0.9, 1.1, 1
+    rng = np.random.default_rng(23)
+    x0 = rng.uniform(0.5, 1.5, img.shape).astype(np.float32)
+    obj.set_estimate(x0)
+    OSMAPOSLReconstruction(obj, 1, minimum_relative_change=lo, maximum_relative_change=hi).update_estimate(subset, 2)
+    in_place = obj.get_estimate()
+    x = _t(x0, grad=True)
+    out = osem_subiteration(x, obj, subset, 2, lo, hi)
+    assert np.array_equal(out.detach().cpu().numpy(), in_place)
+    u = in_place / x0
+    print(f"clamped at lo {np.mean(u <= np.float32(lo) * 1.0000001):.3f}, at hi {np.mean(u >= np.float32(hi) * 0.9999999):.3f}")
+    assert np.mean(np.isclose(u, lo, rtol=1e-6)) > 0.01 and np.mean(np.isclose(u, hi, rtol=1e-6)) > 0.01
+    # subiteration 1: no clamp
+    out1 = osem_subiteration(_t(x0), obj, subset, 1, lo, hi).cpu().numpy()
+    assert np.mean(np.abs(out1 / x0 - 1) > 0.11) > 0.01
+    gimg = rng.uniform(-1, 1, img.shape)
+    v = rng.uniform(-1, 1, img.shape)
+    out.backward(_t(gimg.astype(np.float32)))
+    lhs = float(np.sum(x.grad.cpu().numpy().astype(np.float64) * v))
+    yd = y.astype(np.float64)
+    sens = O.back_project_f64(np.ones(pdi.num_bins), subset=subset, num_subsets=4)
+
+    def F(xx):
+        ybar = O.forward_project_f64(xx, subset=subset, num_subsets=4) + bkg
+        r = np.where(yd > 0, yd / ybar, 0.0)
+        f = O.back_project_f64(r, subset=subset, num_subsets=4)
+        f = np.where(sens > 0, f / np.where(sens > 0, sens, 1.0), 0.0)
+        return xx * np.clip(f, lo, hi)
+
+    h = 1e-5
+    xd = x0.astype(np.float64)
+    rhs = float(np.sum(gimg * (F(xd + h * v) - F(xd - h * v)) / (2 * h)))
+    print(f"clamped OSEM VJP vs FP64 central difference: {lhs:.8e} vs {rhs:.8e}")
+    assert abs(lhs - rhs) <= 1e-4 * abs(rhs)

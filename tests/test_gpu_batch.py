@@ -105,6 +105,24 @@ def test_graph_replays_equal_eager(batch):
             assert np.array_equal(a, b), f"replay {rep}"
 
 
+def test_set_estimate_right_after_run(batch):
+    """run() returns without waiting; set_estimate() (and set_data) must wait
+    for the queued subiterations before writing the estimate: a run, a
+    reset with no get_estimate in between, a second run equals a clean
+    reset and run."""
+    B, problems, data = batch
+    B.set_launch()
+    _reset(B, problems)
+    B.run(NSUB, use_graph=True)
+    ref = _images(B)
+    _reset(B, problems)
+    B.run(2 * NSUB, use_graph=True)
+    _reset(B, problems)  # no synchronisation with the run above
+    B.run(NSUB, use_graph=True)
+    for a, b in zip(ref, _images(B)):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("launch", [(32, True, 1, 32, 8, 0), (0, False, 0, 1, 1, 0), (16, False, 1, 8, 8, 16),
                                     (64, False, 1, 32, 8, 64), (32, False, 0, 32, 8, 32), (32, False, 2, 32, 8, 32),
                                     (0, False, 2, 8, 8, 0)])

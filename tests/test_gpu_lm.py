@@ -66,6 +66,25 @@ def test_lm_osem(lm):
     assert rel <= 1e-4
 
 
+def test_lm_capped_events(lm):
+    """Every event capped (x = 1e-9, no additive term: 1 > 1e4 fwd): the
+    gradient still back-projects y/fwd (the reference's gradient
+    instantiation caps nothing), the value takes log(1e-4) per event. Runs
+    last on the module's context: it replaces the events."""
+    pdi, img, P, O, x_true, ev, add, obj = lm
+    sub = np.ascontiguousarray(ev[:50000])
+    obj2 = PoissonLogLikelihoodWithLinearModelForMeanAndListModeDataWithProjMatrixByBin(P)
+    obj2.set_up(sub, 0.0)
+    x = np.full(img.shape, 1e-9, np.float32)
+    g = obj2.compute_sub_gradient_without_penalty(x, subset=2, plus_sensitivity=True)
+    vr, r = O.lm_gradient(sub, x, 2, NSUB)
+    err = np.abs(g - r).max() / np.abs(r).max()
+    print(f"capped LM gradient: max err {err:.2e} of max {np.abs(r).max():.3e}")
+    assert np.abs(r).max() > 0 and err <= 1e-5
+    v = obj2.compute_objective_function(x, subset=2)
+    assert abs(v - vr) <= 1e-9 * abs(vr)
+
+
 def test_lm_tof_gradient():
     pdi, img = config0_tof(3)
     pm = RayTracingMatrixParameters()

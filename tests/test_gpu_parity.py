@@ -189,3 +189,57 @@ def test_gradient_and_loglik(c0):
     L = obj.compute_objective_function(x)
     Lr = O.log_likelihood(y, x, bkg=bfull)
     assert abs(L - Lr) <= 1e-6 * abs(Lr)
+
+
+def test_truncation_branches_fire(c0):
+    """Data that make the HIP epilogues take every branch the oracle pins in
+    test_objective_oracle_pins.py: non-integer and negative counts; an
+    estimate that is zero outside r = 60 mm with an additive term of 1e-7
+    where A x = 0, so y > 1e4 ybar (quotient capped at 1e4 on non-empty rows)
+    and the log-likelihood floor max(ybar, y / 1e4) acts; one count of 1e8 in
+    some viewgrams (their small value becomes 100, so most of their bins are
+    thresholded); OSEM with relative-change bounds 0.8 / 1.25. HIP against
+    the oracle: gradients within 1e-5 of the maximum, the value within 1e-6
+    relative, OSEM images within 1e-4 relative."""
+    pdi, img, pm, P, O = c0
+    x_true, y, bkg = phantoms.config0_data(pdi, img, O.forward_project, seed=11)
+    rng = np.random.default_rng(12)
+    y = (y + rng.uniform(0, 1, y.shape)).astype(np.float32)
+    zz, yy, xx = np.meshgrid(np.arange(img.nz), (np.arange(img.ny) + img.min_y) * img.vox_y,
+                             (np.arange(img.nx) + img.min_x) * img.vox_x, indexing="ij")
+    x = np.where(yy ** 2 + xx ** 2 < 60.0 ** 2, x_true + 0.5, 0.0).astype(np.float32)
+    ax = O.forward_project(x)
+    b = np.full(pdi.num_bins, bkg, np.float32)
+    b[ax == 0] = 1e-7
+    nvg = 0
+    for view in range(0, pdi.num_views, 11):
+        y[pdi.bin_index(0, view, 3, 5)] = 1e8
+        nvg += 1
+    y[rng.choice(pdi.num_bins, 200, replace=False)] = -2.0
+    capped = (y > 1e4 * (ax + b)) & (ax == 0)
+    print(f"bins with A x = 0: {(ax == 0).sum()}, capped: {capped.sum()}, viewgrams with a 1e8 count: {nvg}")
+    assert capped.sum() > 1000
+    obj = PoissonLogLikelihoodWithLinearModelForMeanAndProjData(P)
+    obj.set_up(y, b)
+    for s, with_sens in ((1, True), (2, False)):
+        g = obj.compute_sub_gradient_without_penalty(x, subset=s, plus_sensitivity=with_sens)
+        gr = O.subset_gradient(y, x, s, 4, bkg=b, add_sensitivity=with_sens)
+        err = np.abs(g - gr).max() / np.abs(gr).max()
+        print(f"gradient subset {s} (plus sensitivity {with_sens}): max err {err:.2e} of max")
+        assert err <= 1e-5
+    L = obj.compute_objective_function(x)
+    Lr = O.log_likelihood(y, x, bkg=b)
+    print(f"log-likelihood {L:.10e} oracle {Lr:.10e}")
+    assert abs(L - Lr) <= 1e-6 * abs(Lr)
+    recon = OSMAPOSLReconstruction(obj, num_subiterations=8, minimum_relative_change=0.8,
+                                   maximum_relative_change=1.25)
+    got = recon.reconstruct(np.ones(img.shape, np.float32))
+    O.set_osem_options(minimum_relative_change=0.8, maximum_relative_change=1.25)
+    try:
+        ref = O.osem(y, np.ones(img.shape, np.float32), 4, 8, bkg=b)
+    finally:
+        O.set_osem_options()
+    mask = ref > 1e-3 * ref.max()
+    rel = (np.abs(got - ref)[mask] / ref[mask]).max()
+    print(f"bounded OSEM max rel err {rel:.3e}")
+    assert rel <= 1e-4

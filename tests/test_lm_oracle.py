@@ -6,9 +6,14 @@ one event per count of a histogram y and no quotient capping, its sums equal
 the projection-data objective's (pinned elsewhere) term by term: the
 gradient BP(y/ybar) of every subset (which pins the event-to-subset rule:
 basic view mod number of subsets), the OSEM images, and the value -sum y log
-ybar. The 1e4 cap of LM_gradient_and_value is pinned by a closed form (an
-event whose row misses the image, no additive term: no gradient, value
--log(1e-4)), and events outside the projection data are dropped.
+ybar. The 1e4 cap of LM_gradient_and_value is pinned by closed forms: an
+event whose row misses the image (no gradient, value -log(1e-4)), and an
+event on a non-empty row with a near-zero estimate, which the reference's
+gradient instantiation <do_gradient=true, do_value=false> still
+back-projects with 1/fwd (the early return is inside `if (do_value)`,
+...ListModeDataWithProjMatrixByBin.cxx:756-787,822-843) while its value
+instantiation takes log(1e-4). Events outside the projection data are
+dropped.
 """
 import math
 
@@ -67,6 +72,36 @@ def test_lm_cap_closed_form(tiny):
     v, g = O.lm_gradient(e, x)
     assert not g.any()
     assert v == -math.log(float(np.float32(1) / np.float32(1e4)))
+
+
+def test_lm_cap_on_a_nonempty_row(tiny):
+    """An event through the image centre with x = 1e-9 everywhere and no
+    additive term: fwd = sum of the row's weights * 1e-9 (float, row order),
+    so 1 > 1e4 fwd. Gradient: the row's weights times 1/fwd at its voxels
+    (no cap); value: -log(float(1/1e4)). Twice the event: twice both. Only
+    elements with z inside the image count (ProjMatrixElemsForOneBin.cxx:333-371)."""
+    pdi, img, O, x_true, y, bkg, ev = tiny
+    seg, view, ax, tang = 0, 5, 3, 0
+    z, yy, xx, w = O.row(seg, view, ax, tang)
+    inside = (z >= img.min_z) & (z < img.min_z + img.nz)
+    z, yy, xx, w = z[inside], yy[inside], xx[inside], w[inside]
+    assert len(w) > 10
+    x = np.full(img.shape, 1e-9, np.float32)
+    fwd = np.float32(0)
+    for wi in w:
+        fwd = np.float32(fwd + np.float32(wi * np.float32(1e-9)))
+    assert 1.0 > 1e4 * float(fwd)
+    e = np.array([[seg, view, ax, tang, 0]], np.int32)
+    v, g = O.lm_gradient(e, x)
+    expect = np.zeros(img.shape, np.float32)
+    q = np.float32(np.float32(1) / fwd)
+    for zi, yi, xi, wi in zip(z, yy, xx, w):
+        expect[zi - img.min_z, yi - img.min_y, xi - img.min_x] += np.float32(wi * q)
+    assert np.array_equal(g, expect)
+    assert v == -math.log(float(np.float32(1) / np.float32(1e4)))
+    v2, g2 = O.lm_gradient(np.concatenate([e, e]), x)
+    assert v2 == 2 * v
+    assert np.allclose(g2, 2 * expect, rtol=1e-6, atol=0)
 
 
 def test_lm_events_outside_the_data_are_dropped(tiny):
