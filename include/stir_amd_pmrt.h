@@ -164,6 +164,36 @@ int mi_back_project(mi_ctx* ctx, const float* proj_data, float* image, int subse
 int mi_forward_project_dev(mi_ctx* ctx, const float* d_image, float* d_proj_data, int subset, void* stream);
 int mi_back_project_dev(mi_ctx* ctx, const float* d_proj_data, float* d_image, int subset, void* stream);
 
+/* ---- projections of basic view-segments (the drop-in projector plugins) ---- */
+/* detail::find_basic_vs_nums_in_subset (find_basic_vs_nums_in_subset.cxx:31-69)
+ * for any subset / num_subsets with the context's symmetries: the basic
+ * (view, segment) pairs, segment-major, views subset, subset + num_subsets,
+ * ... *count is set to their number; at most cap are written. */
+int mi_basic_view_segments_in_subset(mi_ctx* ctx, int subset, int num_subsets, int32_t* views, int32_t* segments,
+                                     int cap, int* count);
+/* ForwardProjectorByBin::set_input (ForwardProjectorByBin.cxx:306-318): the
+ * image (host, STIR layout) that mi_forward_project_view_segments projects;
+ * kept on the device until the next call. */
+int mi_set_image(mi_ctx* ctx, const float* image);
+/* ForwardProjectorByBin::forward_project of the RelatedViewgrams of n
+ * view-segments (each replaced by its basic view-segment; repeats count
+ * once), as forward_project(ProjData&, subset_num, num_subsets) does for the
+ * basic view-segments of a subset (ForwardProjectorByBin.cxx:168-230): the
+ * image of mi_set_image is projected into exactly those viewgrams (all TOF
+ * bins) of proj_data (host, ProjDataInMemory order); no other bin of
+ * proj_data is written. The work is that of those view-segments only. */
+int mi_forward_project_view_segments(mi_ctx* ctx, int n, const int32_t* views, const int32_t* segments,
+                                     float* proj_data);
+/* BackProjectorByBin::back_project of the RelatedViewgrams of n view-segments
+ * followed by get_output (BackProjectorByBin.cxx:183-229,316-346): only
+ * those viewgrams of proj_data (host, ProjDataInMemory order) are read and
+ * back-projected; image (host, STIR layout) is overwritten. When the
+ * view-segments are exactly one subset of the context (mi_set_up's
+ * num_subsets) only that subset's part of the back-projection table runs;
+ * any other set runs the whole table over data zeroed outside the set. */
+int mi_back_project_view_segments(mi_ctx* ctx, int n, const int32_t* views, const int32_t* segments,
+                                  const float* proj_data, float* image);
+
 /* ---- objective function / OSEM, data resident on the device -------------- */
 /* Measured data and additive term of
  * PoissonLogLikelihoodWithLinearModelForMeanAndProjData (set_up_before_sensitivity,

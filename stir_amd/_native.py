@@ -30,7 +30,8 @@ EXPORTED_SYMBOLS = (
     "mi_batch_get_estimate", "mi_batch_set_launch", "mi_batch_tune", "mi_batch_osem_run", "mi_batch_synchronize",
     "mi_obj_set_multiplicative", "mi_obj_set_use_subset_sensitivities", "mi_osem_set_options", "mi_obj_set_prior",
     "mi_relative_difference_prior_dev", "mi_batch_set_row_groups", "mi_lm_set_events",
-    "mi_prior_hessian_times_input_dev",
+    "mi_prior_hessian_times_input_dev", "mi_basic_view_segments_in_subset", "mi_set_image",
+    "mi_forward_project_view_segments", "mi_back_project_view_segments",
 )
 
 MI_PRIOR_NONE, MI_PRIOR_QUADRATIC, MI_PRIOR_RELATIVE_DIFFERENCE = 0, 1, 2
@@ -138,6 +139,10 @@ def load():
     L.mi_lm_set_events.argtypes = [vp, _i32p, ctypes.c_int64, vp, ctypes.c_float]
     L.mi_prior_hessian_times_input_dev.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float,
                                                    ctypes.c_float, c_int, vp, vp]
+    L.mi_basic_view_segments_in_subset.argtypes = [vp, c_int, c_int, _i32p, _i32p, c_int, ctypes.POINTER(c_int)]
+    L.mi_set_image.argtypes = [vp, _f32p]
+    L.mi_forward_project_view_segments.argtypes = [vp, c_int, _i32p, _i32p, _f32p]
+    L.mi_back_project_view_segments.argtypes = [vp, c_int, _i32p, _i32p, _f32p, _f32p]
     L.mi_obj_set_multiplicative.argtypes = [vp, vp]
     L.mi_obj_set_use_subset_sensitivities.argtypes = [vp, c_int]
     L.mi_osem_set_options.argtypes = [vp, c_int, ctypes.c_float, ctypes.c_float]

@@ -95,6 +95,16 @@ struct mi_ctx {
   DBuf<mi::Family> fams;
   DBuf<mi::FpTask> tasks;
   std::vector<int> task_off;  // per subset, size nsub+1
+  // basic view-segment groups: the related viewgrams of one basic
+  // (view, segment), the unit find_basic_vs_nums_in_subset hands to the
+  // projectors (find_basic_vs_nums_in_subset.cxx:31-69)
+  std::unordered_map<int64_t, int> vs_group;            // basic (segment, view) -> group
+  std::vector<std::vector<std::pair<int, int>>> group_vs;  // group -> related (view, segment)
+  std::vector<int> task_group;                          // forward task -> group
+  std::vector<std::vector<int>> subset_groups;          // subset of the table -> its groups, ascending
+  std::vector<mi::FpTask> h_tasks;
+  DBuf<mi::FpTask> sel_tasks;  // forward tasks of a view-segment request
+  bool have_image = false;     // ctx->img holds the image of mi_set_image
   DBuf<int64_t> list_off;
   DBuf<mi::BpEntry> entries;
   std::vector<mi::Family> h_fams;
@@ -222,16 +232,58 @@ struct mi_ctx {
   // ---- projections on device-layout buffers
   void fp(mi::FpMode mode, const float* d_img, float* d_out, int subset, const mi::FpData& data, hipStream_t st) {
     const int tb = task_off[subset_begin(subset)], te = task_off[subset_end(subset)];
+    fp_tasks(mode, tasks.p + tb, te - tb, d_img, d_out, data, st);
+  }
+  void fp_tasks(mi::FpMode mode, const mi::FpTask* d_tasks, int ntasks, const float* d_img, float* d_out,
+                const mi::FpData& data, hipStream_t st) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
     if (ntof > 1)
-      mi::launch_forward_project_tof(mode, tasks.p + tb, te - tb, fams.p, lors.p, pool.p, d_img, d_out, dims, tofc, data,
-                                     st);
+      mi::launch_forward_project_tof(mode, d_tasks, ntasks, fams.p, lors.p, pool.p, d_img, d_out, dims, tofc, data, st);
     else
-      mi::launch_forward_project(mode, tasks.p + tb, te - tb, fams.p, pool.p, d_img, d_out, dims, data, st, fp_xcd_chunk,
+      mi::launch_forward_project(mode, d_tasks, ntasks, fams.p, pool.p, d_img, d_out, dims, data, st, fp_xcd_chunk,
                                  pool_ops.p, num_basic_elements);
     if (timing) { HIPCHECK(hipEventRecord(e1, st)); fp_events.emplace_back(e0, e1); }
   }
+  // groups of n (view, segment) pairs (each mapped to its basic
+  // view-segment), ascending, without repeats
+  std::vector<int> request_groups(int n, const int32_t* views, const int32_t* segments) const {
+    const mi_geometry& g = plan->geometry();
+    std::vector<int> out;
+    for (int i = 0; i < n; ++i) {
+      int v = views[i], sg = segments[i];
+      if (sg < g.min_segment_num || sg > g.max_segment_num || v < 0 || v >= g.num_views)
+        throw ApiError(MI_ERR_ARGUMENT, "view or segment out of range");
+      plan->basic_view_segment(v, sg);
+      auto it = vs_group.find(static_cast<int64_t>(sg + 512) * 65536 + v);
+      if (it == vs_group.end()) throw ApiError(MI_ERR_INTERNAL, "basic view-segment without forward tasks");
+      out.push_back(it->second);
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+  }
+  // one viewgram (segment, view, TOF bin k) between the device layout and
+  // STIR's layout on the host: nax rows of ntang values, row pitch
+  // views * ntang; staged contiguously at stage_sino + slot
+  void viewgram_to_host(const float* d_proj, int seg, int view, int k, float* host, int64_t slot, hipStream_t st) {
+    const mi_geometry& g = plan->geometry();
+    const int nt = g.num_tangential_poss, nax = plan->num_axial(seg);
+    const int64_t blk = k * block_bins + plan->segment_offset(seg);
+    mi::launch_transpose(d_proj + blk + static_cast<int64_t>(view) * nt * nax, stage_sino.p + slot, nt, nax, nax, nt, st);
+    HIPCHECK(hipMemcpy2DAsync(host + blk + static_cast<int64_t>(view) * nt, sizeof(float) * g.num_views * nt,
+                              stage_sino.p + slot, sizeof(float) * nt, sizeof(float) * nt, nax, hipMemcpyDeviceToHost,
+                              st));
+  }
+  void viewgram_from_host(const float* host, int seg, int view, int k, float* d_proj, int64_t slot, hipStream_t st) {
+    const mi_geometry& g = plan->geometry();
+    const int nt = g.num_tangential_poss, nax = plan->num_axial(seg);
+    const int64_t blk = k * block_bins + plan->segment_offset(seg);
+    HIPCHECK(hipMemcpy2DAsync(stage_sino.p + slot, sizeof(float) * nt, host + blk + static_cast<int64_t>(view) * nt,
+                              sizeof(float) * g.num_views * nt, sizeof(float) * nt, nax, hipMemcpyHostToDevice, st));
+    mi::launch_transpose(stage_sino.p + slot, d_proj + blk + static_cast<int64_t>(view) * nt * nax, nax, nt, nt, nax, st);
+  }
+  DBuf<float> input_img;  // the image of mi_set_image (device layout)
   // nontof: back-project one non-TOF block with the non-TOF values of the
   // table (the sensitivity of TOF data, unless TOF sensitivities are asked for)
   void bp(mi::BpMode mode, const float* d_proj, float* d_img, int subset, const mi::BpData& data, hipStream_t st,
@@ -463,6 +515,33 @@ void mi_ctx::build_tasks() {
     });
   }
   task_off[nsub] = static_cast<int>(h.size());
+  // the basic view-segment group of every task (a task's families are
+  // related view-segments of one basic view-segment)
+  vs_group.clear();
+  group_vs.clear();
+  task_group.assign(h.size(), -1);
+  subset_groups.assign(nsub, {});
+  auto group_of = [&](int fam) {
+    int v = keys[fam].view, sg = keys[fam].seg;
+    plan->basic_view_segment(v, sg);
+    const int64_t key = static_cast<int64_t>(sg + 512) * 65536 + v;
+    auto it = vs_group.emplace(key, static_cast<int>(group_vs.size()));
+    if (it.second) group_vs.push_back(plan->related_view_segments(v, sg));
+    return it.first->second;
+  };
+  for (int s = 0; s < nsub; ++s)
+    for (int t = task_off[s]; t < task_off[s + 1]; ++t) {
+      const int g = group_of(h[t].first);
+      if (h[t].second >= 0 && group_of(h[t].second) != g)
+        throw ApiError(MI_ERR_INTERNAL, "forward task across basic view-segments");
+      task_group[t] = g;
+      subset_groups[s].push_back(g);
+    }
+  for (auto& v : subset_groups) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  h_tasks = h;
   tasks.alloc(h.size());
   HIPCHECK(hipMemcpyAsync(tasks.p, h.data(), h.size() * sizeof(mi::FpTask), hipMemcpyHostToDevice, stream));
   partial.alloc(std::max<size_t>(h.size(), 1));
@@ -542,8 +621,8 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   img.release(); img2.release(); sino.release(); ratio.release(); stage_img.release(); stage_sino.release();
   y.release(); add.release(); mult.release(); ones.release(); vg_small.release(); x.release(); sens.clear();
   vg_off.release(); vg_len.release(); partial.release(); prior_partial.release();
-  lors.release(); entry_d.release(); erf_tab.release();
-  have_data = have_sens = have_x = false;
+  lors.release(); entry_d.release(); erf_tab.release(); sel_tasks.release();
+  have_data = have_sens = have_x = have_image = false;
   plan.reset();
   plan = std::make_unique<mi::Plan>(g, nsub);
   // the back projection forms its gather offsets in 32 bits (bytes of one
@@ -832,6 +911,106 @@ int mi_back_project(mi_ctx* ctx, const float* proj_data, float* image, int subse
       throw ApiError(MI_ERR_DEVICE, g_last_error);
     HIPCHECK(hipMemcpyAsync(image, ctx->stage_img.p, ctx->stage_img.bytes(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mi_basic_view_segments_in_subset(mi_ctx* ctx, int subset, int num_subsets, int32_t* views, int32_t* segments,
+                                     int cap, int* count) {
+  return guarded([&] {
+    if (!ctx || !count) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    if (num_subsets < 1 || subset < 0 || subset >= num_subsets) throw ApiError(MI_ERR_ARGUMENT, "subset out of range");
+    const mi_geometry& g = ctx->plan->geometry();
+    // find_basic_vs_nums_in_subset.cxx:31-69 (its TOF loop runs once)
+    int n = 0;
+    for (int seg = g.min_segment_num; seg <= g.max_segment_num; ++seg)
+      for (int view = subset; view < g.num_views; view += num_subsets) {
+        int v = view, sg = seg;
+        if (ctx->plan->basic_view_segment(v, sg)) continue;
+        if (n < cap) {
+          if (views) views[n] = view;
+          if (segments) segments[n] = seg;
+        }
+        ++n;
+      }
+    *count = n;
+  });
+}
+
+int mi_set_image(mi_ctx* ctx, const float* image) {
+  return guarded([&] {
+    if (!ctx || !image) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    if (ctx->input_img.n != ctx->img.n) ctx->input_img.alloc(ctx->img.n);
+    HIPCHECK(hipMemcpyAsync(ctx->stage_img.p, image, ctx->stage_img.bytes(), hipMemcpyHostToDevice, ctx->stream));
+    ctx->image_to_device(ctx->stage_img.p, ctx->input_img.p, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->have_image = true;
+  });
+}
+
+int mi_forward_project_view_segments(mi_ctx* ctx, int n, const int32_t* views, const int32_t* segments,
+                                     float* proj_data) {
+  return guarded([&] {
+    if (!ctx || !proj_data || (n > 0 && (!views || !segments))) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    if (!ctx->have_image) throw ApiError(MI_ERR_STATE, "mi_set_image has not been called");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const std::vector<int> groups = ctx->request_groups(n, views, segments);
+    if (groups.empty()) return;
+    hipStream_t st = ctx->stream;
+    // the request's forward tasks, in the order of the table's subsets
+    std::vector<char> want(ctx->group_vs.size(), 0);
+    for (int gi : groups) want[gi] = 1;
+    std::vector<mi::FpTask> sel;
+    for (size_t t = 0; t < ctx->h_tasks.size(); ++t)
+      if (want[ctx->task_group[t]]) sel.push_back(ctx->h_tasks[t]);
+    if (ctx->sel_tasks.n < sel.size()) ctx->sel_tasks.alloc(sel.size());
+    HIPCHECK(hipMemcpyAsync(ctx->sel_tasks.p, sel.data(), sel.size() * sizeof(mi::FpTask), hipMemcpyHostToDevice, st));
+    ctx->fp_tasks(mi::FP_PLAIN, ctx->sel_tasks.p, static_cast<int>(sel.size()), ctx->input_img.p, ctx->sino.p,
+                  mi::FpData{}, st);
+    for (int gi : groups)
+      for (const auto& vs : ctx->group_vs[gi])
+        for (int k = 0; k < ctx->ntof; ++k)
+          ctx->viewgram_to_host(ctx->sino.p, vs.second, vs.first, k, proj_data,
+                                k * ctx->block_bins + ctx->plan->device_bin_offset(vs.second, vs.first, 0,
+                                                                                   ctx->plan->geometry().min_tangential_pos_num),
+                                st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+  });
+}
+
+int mi_back_project_view_segments(mi_ctx* ctx, int n, const int32_t* views, const int32_t* segments,
+                                  const float* proj_data, float* image) {
+  return guarded([&] {
+    if (!ctx || !proj_data || !image || (n > 0 && (!views || !segments)))
+      throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    HIPCHECK(hipSetDevice(ctx->device));
+    const std::vector<int> groups = ctx->request_groups(n, views, segments);
+    hipStream_t st = ctx->stream;
+    // one subset of the table: its entries read only its own bins; otherwise
+    // every other bin is zeroed and the whole table runs
+    int subset = -1;
+    for (int s = 0; s < ctx->plan->num_subsets(); ++s)
+      if (ctx->subset_groups[s] == groups) subset = s;
+    if (subset < 0) HIPCHECK(hipMemsetAsync(ctx->sino.p, 0, ctx->sino.bytes(), st));
+    for (int gi : groups)
+      for (const auto& vs : ctx->group_vs[gi])
+        for (int k = 0; k < ctx->ntof; ++k)
+          ctx->viewgram_from_host(proj_data, vs.second, vs.first, k, ctx->sino.p,
+                                  k * ctx->block_bins + ctx->plan->device_bin_offset(vs.second, vs.first, 0,
+                                                                                     ctx->plan->geometry().min_tangential_pos_num),
+                                  st);
+    if (groups.empty()) mi::launch_fill(ctx->img.p, static_cast<int64_t>(ctx->img.n), 0.F, st);
+    else ctx->bp(mi::BP_PLAIN, ctx->sino.p, ctx->img.p, subset, mi::BpData{}, st);
+    ctx->image_from_device(ctx->img.p, ctx->stage_img.p, st);
+    HIPCHECK(hipMemcpyAsync(image, ctx->stage_img.p, ctx->stage_img.bytes(), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
   });
 }
 

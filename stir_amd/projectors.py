@@ -98,6 +98,48 @@ class ProjectorByBinPairMI355X:
                       "mi_subset_view_segments")
         return list(zip(s[:n.value].tolist(), v[:n.value].tolist()))
 
+    def basic_view_segments_in_subset(self, subset: int, num_subsets: int) -> List[Tuple[int, int]]:
+        """(view, segment) pairs of ``detail::find_basic_vs_nums_in_subset``
+        (``find_basic_vs_nums_in_subset.cxx:31-69``) with this matrix's symmetries."""
+        cap = self.pdi.num_views * len(self.pdi.num_axial_poss) + 16
+        v = np.zeros(cap, np.int32)
+        s = np.zeros(cap, np.int32)
+        n = ctypes.c_int()
+        _native.check(_native.load().mi_basic_view_segments_in_subset(self._ctx.handle, subset, num_subsets, v, s, cap,
+                                                                      ctypes.byref(n)), "mi_basic_view_segments_in_subset")
+        return list(zip(v[:n.value].tolist(), s[:n.value].tolist()))
+
+    def related_viewgram_mask(self, basic_view_segments: List[Tuple[int, int]]) -> np.ndarray:
+        """Boolean mask over the projection data (ProjDataInMemory order) of the
+        RelatedViewgrams of the given basic view-segments, all TOF bins."""
+        pdi = self.pdi
+        mask = np.zeros(pdi.num_bins, bool)
+        ntof = pdi.num_tof_bins
+        block = pdi.num_bins // ntof
+        offs = pdi.segment_offsets()
+        nt, nv = pdi.num_tangential_poss, pdi.num_views
+        for view, seg in basic_view_segments:
+            for rv, rs in self._related(view, seg):
+                nax = pdi.num_axial_poss_of(rs)
+                for k in range(ntof):
+                    blk = mask[k * block + offs[rs]: k * block + offs[rs] + nax * nv * nt].reshape(nax, nv, nt)
+                    blk[:, rv, :] = True
+        return mask
+
+    def _related(self, view: int, seg: int) -> List[Tuple[int, int]]:
+        # the (view, segment) pairs a basic view-segment's table subset holds:
+        # every pair whose basic view-segment is (view, seg)
+        key = (view, seg)
+        if not hasattr(self, "_related_cache"):
+            self._related_cache = {}
+            pairs = [(v, s) for s in range(self.pdi.min_segment_num, self.pdi.max_segment_num + 1)
+                     for v in range(self.pdi.num_views)]
+            bins = np.array([[s, v, 0, 0] for v, s in pairs], np.int32)
+            basic, _ = _native.symmetry_operations(self._geometry, bins)
+            for (v, s), b in zip(pairs, basic):
+                self._related_cache.setdefault((int(b[1]), int(b[0])), []).append((v, s))
+        return self._related_cache.get(key, [])
+
     def get_row(self, segment: int, view: int, axial_pos: int, tang_pos: int):
         """Row of one bin as the device uses it: (z, y, x, w) arrays."""
         cap = 1 << 14
@@ -178,43 +220,92 @@ class ProjectorByBinPairMI355X:
 
 
 class ForwardProjectorByBinMI355X:
-    """``ForwardProjectorByBin`` view of a :class:`ProjectorByBinPairMI355X`."""
+    """``ForwardProjectorByBin`` plugin view of a :class:`ProjectorByBinPairMI355X`,
+    the flow the C++ plugin of the integration notes follows.
+
+    ``set_input`` (``ForwardProjectorByBin.cxx:306-318``) keeps the image on the
+    device; ``forward_project(proj_data, subset_num, num_subsets, zero)``
+    (``:168-230``) projects the basic view-segments of the subset
+    (``find_basic_vs_nums_in_subset``) and nothing else;
+    ``forward_project_related_viewgrams`` is one call of
+    ``actual_forward_project``'s unit, the RelatedViewgrams of one basic
+    view-segment."""
 
     registered_name = "MI355X Ray Tracing"
 
     def __init__(self, pair: ProjectorByBinPairMI355X):
         self.pair = pair
-        self._input = None
+        self._has_input = False
 
-    def set_input(self, image):
-        self._input = image
+    def set_input(self, image) -> None:
+        x = np.ascontiguousarray(image, dtype=np.float32).reshape(-1)
+        if x.size != self.pair.img.num_voxels:
+            raise ValueError("image has the wrong number of voxels")
+        _native.check(_native.load().mi_set_image(self.pair.context.handle, x), "mi_set_image")
+        self._has_input = True
 
-    def forward_project(self, proj_data=None, image=None, subset: int = -1):
-        img = image if image is not None else self._input
-        return self.pair.forward_project(img, subset=subset, out=proj_data)
+    def _project(self, vs: List[Tuple[int, int]], proj_data: np.ndarray) -> None:
+        if not self._has_input:
+            raise RuntimeError("You need to call set_input() before forward_project()")
+        views = np.array([v for v, _ in vs], np.int32)
+        segs = np.array([s for _, s in vs], np.int32)
+        _native.check(_native.load().mi_forward_project_view_segments(self.pair.context.handle, len(vs), views, segs,
+                                                                      proj_data), "mi_forward_project_view_segments")
+
+    def forward_project(self, proj_data: Optional[np.ndarray] = None, subset_num: int = 0, num_subsets: int = 1,
+                        zero: bool = True) -> np.ndarray:
+        if subset_num < 0 or subset_num >= num_subsets:
+            raise ValueError(f"forward_project: wrong subset number {subset_num}")
+        if proj_data is None:
+            proj_data = np.zeros(self.pair.pdi.num_bins, np.float32)
+        elif zero and num_subsets > 1:
+            proj_data.fill(0.0)
+        if proj_data.dtype != np.float32 or not proj_data.flags.c_contiguous or proj_data.size != self.pair.pdi.num_bins:
+            raise ValueError("projection data must be a contiguous float32 array of the data's number of bins")
+        self._project(self.pair.basic_view_segments_in_subset(subset_num, num_subsets), proj_data.reshape(-1))
+        return proj_data
+
+    def forward_project_related_viewgrams(self, view: int, segment: int, proj_data: np.ndarray) -> None:
+        self._project([(view, segment)], proj_data.reshape(-1))
 
 
 class BackProjectorByBinMI355X:
-    """``BackProjectorByBin`` view of a :class:`ProjectorByBinPairMI355X`:
-    ``start_accumulating_in_new_target`` / ``back_project`` / ``get_output``."""
+    """``BackProjectorByBin`` plugin view of a :class:`ProjectorByBinPairMI355X`:
+    ``start_accumulating_in_new_target`` / ``back_project`` / ``get_output``
+    (``BackProjectorByBin.cxx:183-229,300-346``). ``back_project`` stages the
+    RelatedViewgrams of the subset's basic view-segments (viewgrams staged
+    twice are summed); ``get_output`` back-projects the staged viewgrams only."""
 
     registered_name = "MI355X Ray Tracing"
 
     def __init__(self, pair: ProjectorByBinPairMI355X):
         self.pair = pair
-        self._acc = None
+        self._staged: Optional[np.ndarray] = None
+        self._vs: List[Tuple[int, int]] = []
 
-    def start_accumulating_in_new_target(self):
-        self._acc = None
+    def start_accumulating_in_new_target(self) -> None:
+        self._staged = np.zeros(self.pair.pdi.num_bins, np.float32)
+        self._vs = []
+        self._mask = np.zeros(self.pair.pdi.num_bins, bool)
 
-    def back_project(self, proj_data, subset: int = -1):
-        img = self.pair.back_project(proj_data, subset=subset)
-        self._acc = img if self._acc is None else self._acc + img
+    def back_project(self, proj_data: np.ndarray, subset_num: int = 0, num_subsets: int = 1) -> None:
+        if self._staged is None:
+            raise RuntimeError("You need to call start_accumulating_in_new_target() before back_project()")
+        vs = self.pair.basic_view_segments_in_subset(subset_num, num_subsets)
+        m = self.pair.related_viewgram_mask(vs)
+        y = np.asarray(proj_data, np.float32).reshape(-1)
+        self._staged[m] += y[m]
+        self._vs += [p for p in vs if p not in self._vs]
 
-    def get_output(self):
-        if self._acc is None:
-            return np.zeros(self.pair.img.shape, np.float32)
-        return self._acc
+    def get_output(self) -> np.ndarray:
+        out = np.empty(self.pair.img.shape, np.float32)
+        views = np.array([v for v, _ in self._vs], np.int32)
+        segs = np.array([s for _, s in self._vs], np.int32)
+        staged = self._staged if self._staged is not None else np.zeros(self.pair.pdi.num_bins, np.float32)
+        _native.check(_native.load().mi_back_project_view_segments(self.pair.context.handle, len(self._vs), views, segs,
+                                                                   staged, out.reshape(-1)),
+                      "mi_back_project_view_segments")
+        return out
 
 
 class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
