@@ -102,6 +102,13 @@ typedef struct {
   int32_t symmetry_flags[5];    /* effective 90,180,swap_segment,swap_s,shift_z */
   int32_t nz_padded;
   int32_t planes_per_axial_position; /* shared by all segments (1 or 2), 0 when it varies */
+  /* Bytes one projection of all subsets must move through HBM under this
+   * design when every re-read is served by a cache: the matrix data it
+   * streams (forward: the rows per task, task and family records;
+   * back: the back-projection table and its offsets) plus one read of the
+   * input and one write of the output. */
+  int64_t fp_design_bytes;
+  int64_t bp_design_bytes;
 } mi_info;
 
 typedef struct mi_ctx mi_ctx;

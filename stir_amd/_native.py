@@ -71,6 +71,8 @@ class InfoStruct(ctypes.Structure):
         ("symmetry_flags", ctypes.c_int32 * 5),
         ("nz_padded", ctypes.c_int32),
         ("planes_per_axial_position", ctypes.c_int32),
+        ("fp_design_bytes", ctypes.c_int64),
+        ("bp_design_bytes", ctypes.c_int64),
     ]
 
 

@@ -299,6 +299,21 @@ struct mi_ctx {
     if (timing) { HIPCHECK(hipEventRecord(e1, st)); bp_events.emplace_back(e0, e1); }
   }
 
+  // HBM bytes of one projection over all subsets when every re-read hits a
+  // cache: the matrix data streamed once, the input read once, the output
+  // written once (mi_info::fp_design_bytes / bp_design_bytes)
+  void design_bytes(int64_t& fp_bytes, int64_t& bp_bytes) const {
+    const int64_t img_bytes = int64_t{4} * dims.ncols * dims.nzp;
+    const int64_t proj_bytes = int64_t{4} * total_bins();
+    fp_bytes = img_bytes + proj_bytes;
+    for (const mi::FpTask& t : h_tasks) {
+      const mi::Family& f = h_fams[t.first];
+      fp_bytes += int64_t{8} * f.row_len + int64_t(sizeof(mi::FpTask)) +
+                  int64_t(sizeof(mi::Family)) * (t.second >= 0 ? 2 : 1) +
+                  (ntof > 1 ? int64_t(sizeof(mi::FamilyLor)) : 0);
+    }
+    bp_bytes = img_bytes + proj_bytes + static_cast<int64_t>(entries.bytes() + list_off.bytes() + entry_d.bytes());
+  }
   void set_up(const mi_geometry& g, int nsub);
   void build_rows();
   void build_tasks();
@@ -767,6 +782,7 @@ int mi_get_info(mi_ctx* ctx, mi_info* out) {
     ctx->plan->symmetry_flags(out->symmetry_flags);
     out->nz_padded = ctx->dims.nzp;
     out->planes_per_axial_position = ctx->uniform_ppa;
+    ctx->design_bytes(out->fp_design_bytes, out->bp_design_bytes);
   });
 }
 

@@ -76,6 +76,17 @@ def test_rows_bit_exact_span11(mmr):
     assert nonempty > len(bins) // 2
 
 
+def test_nonzero_count(mmr):
+    """nnz at mMR size (3.0e10), counted by the device at set-up, equals the
+    oracle's bin-by-bin count of matrix elements inside the image."""
+    pdi, img, pm, P, O = mmr
+    t0 = time.time()
+    ref = O.nnz()
+    got = P.info().num_nonzeros
+    print(f"[mmr] nnz {got} (oracle {ref}, {time.time() - t0:.1f} s)", flush=True)
+    assert got == ref
+
+
 def test_subset_projections_match_oracle(mmr):
     pdi, img, pm, P, O = mmr
     rng = np.random.default_rng(21)

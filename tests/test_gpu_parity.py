@@ -56,6 +56,11 @@ def test_setup_info(c0):
     assert info.num_bins == pdi.num_bins == O.num_bins
     assert tuple(info.symmetry_flags) == O.symmetry_flags()
     assert info.num_table_entries > 0
+    # nnz (matrix elements whose voxel lies inside the image), the numerator
+    # of the gather-model bandwidth, counted by the oracle bin by bin
+    assert info.num_nonzeros == O.nnz()
+    assert info.bp_design_bytes > 4 * (info.num_bins + info.num_voxels)
+    assert info.fp_design_bytes > 4 * (info.num_bins + info.num_voxels)
 
 
 def test_rows_bit_exact(c0):
