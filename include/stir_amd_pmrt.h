@@ -109,6 +109,12 @@ typedef struct {
    * input and one write of the output. */
   int64_t fp_design_bytes;
   int64_t bp_design_bytes;
+  /* back projection: 1 through LDS slabs of image tiles, 0 by image column
+   * over the column table; the slab tile and its segment groups per view */
+  int32_t slab_back_projection;
+  int32_t slab_tile_x, slab_tile_y, slab_segment_groups;
+  int32_t slab_lds_bytes; /* LDS of one slab workgroup */
+  int32_t reserved0;
 } mi_info;
 
 typedef struct mi_ctx mi_ctx;
@@ -421,9 +427,21 @@ int mi_batch_synchronize(mi_batch* batch);
  * an entry cannot reach (default off). Results do not depend on these
  * settings. MI_TUNE_FP_XCD_CHUNK maps forward-projection workgroups (4 tasks
  * each) the same way: 0 one contiguous range per XCD, k > 0 chunks of k
- * (default 32). */
+ * (default 32).
+ * MI_TUNE_BP_SLAB 1 switches the back projection of tables with one plane
+ * per axial position (and images of at most 128 planes) to projection-data
+ * slabs staged in LDS, one workgroup per image tile (default 0: by image
+ * column over the column table, the faster of the two at mMR size);
+ * MI_TUNE_SLAB_TILE_X / _Y set its tile (default 8 x 8 columns),
+ * MI_TUNE_SLAB_COLUMNS_PER_WAVE (2, 4 or 8, default 4) the columns each wave
+ * of the tile's workgroup sums, and MI_TUNE_SLAB_LDS_KB (16-160, default 80)
+ * the LDS a slab may take (the segments are split into as few groups per
+ * view as keep it there). These take effect at the next mi_set_up (they
+ * decide the table's layout); the sums of one setting are deterministic,
+ * other settings agree to float rounding. */
 enum { MI_TUNE_BP_TILE_X = 1, MI_TUNE_BP_TILE_Y = 2, MI_TUNE_BP_XCD_REMAP = 3, MI_TUNE_BP_SLOT_SKIP = 4,
-       MI_TUNE_FP_XCD_CHUNK = 5 };
+       MI_TUNE_FP_XCD_CHUNK = 5, MI_TUNE_BP_SLAB = 6, MI_TUNE_SLAB_TILE_X = 7, MI_TUNE_SLAB_TILE_Y = 8,
+       MI_TUNE_SLAB_COLUMNS_PER_WAVE = 9, MI_TUNE_SLAB_LDS_KB = 10 };
 int mi_set_tuning(mi_ctx* ctx, int param, int value);
 
 /* ---- diagnostics ----------------------------------------------------------- */

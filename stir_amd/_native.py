@@ -50,6 +50,7 @@ class PriorStruct(ctypes.Structure):
     ]
 
 MI_TUNE_BP_TILE_X, MI_TUNE_BP_TILE_Y, MI_TUNE_BP_XCD_REMAP, MI_TUNE_BP_SLOT_SKIP, MI_TUNE_FP_XCD_CHUNK = 1, 2, 3, 4, 5
+MI_TUNE_BP_SLAB, MI_TUNE_SLAB_TILE_X, MI_TUNE_SLAB_TILE_Y, MI_TUNE_SLAB_COLUMNS_PER_WAVE, MI_TUNE_SLAB_LDS_KB = 6, 7, 8, 9, 10
 
 
 class NativeError(RuntimeError):
@@ -73,6 +74,12 @@ class InfoStruct(ctypes.Structure):
         ("planes_per_axial_position", ctypes.c_int32),
         ("fp_design_bytes", ctypes.c_int64),
         ("bp_design_bytes", ctypes.c_int64),
+        ("slab_back_projection", ctypes.c_int32),
+        ("slab_tile_x", ctypes.c_int32),
+        ("slab_tile_y", ctypes.c_int32),
+        ("slab_segment_groups", ctypes.c_int32),
+        ("slab_lds_bytes", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
     ]
 
 

@@ -22,6 +22,9 @@ hipError_t radix_sort_pairs(void* d_temp, size_t& temp_bytes, const uint32_t* ke
                             const uint32_t* vals_in, uint32_t* vals_out, int n, int end_bit, hipStream_t stream);
 hipError_t exclusive_scan_counts(void* d_temp, size_t& temp_bytes, const int32_t* in, int64_t* out, int n,
                                  hipStream_t stream);
+hipError_t exclusive_scan_u32(void* d_temp, size_t& temp_bytes, const uint32_t* in, uint32_t* out, int n,
+                              hipStream_t stream);
+void launch_fill_int2(int2* d, int64_t n, int2 v, hipStream_t stream);
 }  // namespace mi
 
 namespace {
@@ -107,6 +110,24 @@ struct mi_ctx {
   bool have_image = false;     // ctx->img holds the image of mi_set_image
   DBuf<int64_t> list_off;
   DBuf<mi::BpEntry> entries;
+  // back projection through LDS slabs (build_slab_table), or by image
+  // column over the column table (list_off) otherwise
+  bool slab_allowed = true;  // batches read the column table
+  int slab_enable = 0;       // MI_TUNE_BP_SLAB
+  int slab_tx = 8, slab_ty = 8, slab_cpw = 4, slab_lds_kb = 80;
+  bool use_slab = false;
+  DBuf<uint2> slab_hdr;
+  DBuf<mi::SlabEntry> slab_ent;
+  DBuf<int2> slab_rng, slab_seg_pos, slab_groups;
+  DBuf<int32_t> slab_view_of, slab_nviews, slab_tiles;
+  DBuf<int4> slab_segs;
+  mi::SlabArgs slab{};
+  void release_slab() {
+    use_slab = false;
+    slab_hdr.release(); slab_ent.release(); slab_rng.release(); slab_seg_pos.release(); slab_groups.release();
+    slab_view_of.release(); slab_nviews.release(); slab_tiles.release(); slab_segs.release();
+    slab = mi::SlabArgs{};
+  }
   std::vector<mi::Family> h_fams;
   std::unordered_map<int64_t, int> fam_lookup;  // (seg,view,tang,a0) -> family
   int64_t num_basic_elements = 0;
@@ -293,6 +314,9 @@ struct mi_ctx {
     if (ntof > 1 && !nontof)
       mi::launch_back_project_tof(mode, uniform_ppa, list_off.p, entries.p, entry_d.p, subset_begin(subset),
                                   subset_end(subset), d_proj, d_img, dims, tofc, data, tof_bp_tuning, st);
+    else if (use_slab)
+      mi::launch_back_project_slab(mode, slab, subset_begin(subset), subset_end(subset), d_proj, d_img, dims, data,
+                                   st);
     else
       mi::launch_back_project(mode, uniform_ppa, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj,
                               d_img, dims, data, bp_tuning, st);
@@ -312,12 +336,15 @@ struct mi_ctx {
                   int64_t(sizeof(mi::Family)) * (t.second >= 0 ? 2 : 1) +
                   (ntof > 1 ? int64_t(sizeof(mi::FamilyLor)) : 0);
     }
-    bp_bytes = img_bytes + proj_bytes + static_cast<int64_t>(entries.bytes() + list_off.bytes() + entry_d.bytes());
+    bp_bytes = img_bytes + proj_bytes +
+               static_cast<int64_t>(entries.bytes() + list_off.bytes() + entry_d.bytes() + slab_hdr.bytes() +
+                                    slab_ent.bytes());
   }
   void set_up(const mi_geometry& g, int nsub);
   void build_rows();
   void build_tasks();
   void build_table();
+  bool build_slab_table();
   mi::FpData fp_data() const {
     mi::FpData d{};
     d.measured = y.p;
@@ -334,7 +361,7 @@ namespace {
 // compute_weights (QuadraticPrior.cxx:202-226): vox_x / |offset|, the squared
 // length summed in float and the quotient formed in double (the reference's
 // unqualified sqrt resolves to the C library's double overload there).
-void default_prior_weights(const mi_geometry& g, float* w27) {
+void default_prior_weights(float vox_z, float vox_y, float vox_x, float* w27) {
   for (int z = -1; z <= 1; ++z)
     for (int y = -1; y <= 1; ++y)
       for (int x = -1; x <= 1; ++x) {
@@ -342,12 +369,14 @@ void default_prior_weights(const mi_geometry& g, float* w27) {
         if (z == 0 && y == 0 && x == 0) {
           w = 0;
         } else {
-          const float sx = static_cast<float>(x) * g.vox_x, sy = static_cast<float>(y) * g.vox_y;
-          const float sz = static_cast<float>(z) * g.vox_z;
-          w = static_cast<float>(static_cast<double>(g.vox_x) / std::sqrt(static_cast<double>(sx * sx + sy * sy + sz * sz)));
+          const float sx = static_cast<float>(x) * vox_x, sy = static_cast<float>(y) * vox_y;
+          const float sz = static_cast<float>(z) * vox_z;
+          w = static_cast<float>(static_cast<double>(vox_x) / std::sqrt(static_cast<double>(sx * sx + sy * sy + sz * sz)));
         }
       }
 }
+
+void default_prior_weights(const mi_geometry& g, float* w27) { default_prior_weights(g.vox_z, g.vox_y, g.vox_x, w27); }
 
 int64_t fam_key(int seg, int view, int tang, int a0) {
   return (((static_cast<int64_t>(seg + 512) * 4096 + view) * 8192) + (tang + 4096)) * 8192 + a0;
@@ -449,7 +478,9 @@ void mi_ctx::build_rows() {
   // the rows once per in-plane operation for the (non-TOF) forward projection
   pool_ops.release();
   if (ntof == 1 && total > 0) {
-    pool_ops.alloc(8 * total);
+    // 8 copies, then slack for the forward kernel's read-ahead of one element group
+    pool_ops.alloc(8 * total + 16);
+    HIPCHECK(hipMemsetAsync(pool_ops.p + 8 * total, 0, 16 * sizeof(int2), stream));
     mi::launch_expand_rows(pool.p, total, dims, pool_ops.p, stream);
     HIPCHECK(hipGetLastError());
   }
@@ -564,6 +595,12 @@ void mi_ctx::build_tasks() {
 }
 
 void mi_ctx::build_table() {
+  release_slab();
+  if (build_slab_table()) {
+    use_slab = true;
+    return;
+  }
+  release_slab();
   const int nsub = plan->num_subsets();
   const int ncols = dims.ncols;
   // entry offsets per family (family order == subset order)
@@ -629,10 +666,226 @@ void mi_ctx::build_table() {
   }
 }
 
+// The slab table (pmrt_slab.hip): for one plane per axial position, families
+// that cover whole runs, images of at most 128 planes and slabs that fit the
+// LDS of a workgroup; false when the geometry does not qualify (the column
+// table is built instead).
+bool mi_ctx::build_slab_table() {
+  const mi_geometry& g = plan->geometry();
+  if (!slab_allowed || !slab_enable || ntof > 1 || uniform_ppa != 1 || dims.nzp > 2 * 64 || h_fams.empty())
+    return false;
+  const auto& keys = plan->family_keys();
+  for (size_t i = 0; i < h_fams.size(); ++i)
+    if (keys[i].a0 != 0 || h_fams[i].nax != plan->num_axial(keys[i].seg) || h_fams[i].nax > 2 * 64) return false;
+  const int nsub = plan->num_subsets();
+  const int tx = slab_tx, ty = slab_ty, TC = tx * ty, cpw = slab_cpw;
+  if (tx < 1 || ty < 1 || TC % cpw != 0 || TC / cpw > 16) return false;
+  const int nwaves = TC / cpw;
+  const int ntx = (g.nx + tx - 1) / tx, nty = (g.ny + ty - 1) / ty, ntiles = ntx * nty;
+  // view slots: the distinct views of each subset's families, ascending
+  std::vector<std::vector<int>> views(nsub);
+  for (int s = 0; s < nsub; ++s) {
+    for (int f = plan->subset_family_begin(s); f < plan->subset_family_begin(s + 1); ++f) views[s].push_back(keys[f].view);
+    std::sort(views[s].begin(), views[s].end());
+    views[s].erase(std::unique(views[s].begin(), views[s].end()), views[s].end());
+  }
+  int nv_max = 1;
+  for (auto& v : views) nv_max = std::max<int>(nv_max, static_cast<int>(v.size()));
+  // segments in storage order
+  const std::vector<int>& seq = plan->segment_sequence();
+  const int nseg = static_cast<int>(seq.size());
+  std::vector<int4> hsegs(nseg);
+  std::unordered_map<int, int> seg_slot;
+  for (int j = 0; j < nseg; ++j) {
+    hsegs[j] = make_int4(static_cast<int>(plan->segment_offset(seq[j])), plan->num_axial(seq[j]), 0, 0);
+    seg_slot[seq[j]] = j;
+  }
+  // per family: view slot, tangential index, segment slot
+  std::vector<int4> hfs(h_fams.size());
+  for (int s = 0; s < nsub; ++s)
+    for (int f = plan->subset_family_begin(s); f < plan->subset_family_begin(s + 1); ++f) {
+      const int vs = static_cast<int>(std::lower_bound(views[s].begin(), views[s].end(), keys[f].view) - views[s].begin());
+      hfs[f] = make_int4(vs, keys[f].tang - g.min_tangential_pos_num, seg_slot[keys[f].seg], 0);
+    }
+  std::vector<int32_t> hvo(static_cast<size_t>(nsub) * nv_max, 0), hnv(nsub);
+  for (int s = 0; s < nsub; ++s) {
+    hnv[s] = static_cast<int>(views[s].size());
+    for (size_t v = 0; v < views[s].size(); ++v) hvo[static_cast<size_t>(s) * nv_max + v] = views[s][v];
+  }
+  DBuf<int4> d_fs;
+  d_fs.alloc(hfs.size());
+  HIPCHECK(hipMemcpyAsync(d_fs.p, hfs.data(), hfs.size() * sizeof(int4), hipMemcpyHostToDevice, stream));
+  slab_view_of.alloc(hvo.size());
+  HIPCHECK(hipMemcpyAsync(slab_view_of.p, hvo.data(), hvo.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  slab_nviews.alloc(nsub);
+  HIPCHECK(hipMemcpyAsync(slab_nviews.p, hnv.data(), nsub * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  mi::SlabArgs A{};
+  A.tx = tx; A.ty = ty; A.ntx = ntx; A.ntiles = ntiles; A.nv_max = nv_max; A.nseg = nseg;
+  A.ntang = g.num_tangential_poss;
+  A.cpw = cpw; A.nwaves = nwaves;
+  A.view_of = slab_view_of.p;
+  A.nviews = slab_nviews.p;
+  // pass 1: tangential ranges per (subset, tile, view slot)
+  const int64_t nrng = static_cast<int64_t>(nsub) * ntiles * nv_max;
+  slab_rng.alloc(nrng);
+  mi::launch_fill_int2(slab_rng.p, nrng, make_int2(0x7fffffff, -1), stream);
+  A.rng = slab_rng.p;
+  for (int s = 0; s < nsub; ++s)
+    mi::launch_slab_range(fams.p, d_fs.p, plan->subset_family_begin(s), plan->subset_family_begin(s + 1), pool.p, s,
+                          dims, A, slab_rng.p, stream);
+  HIPCHECK(hipGetLastError());
+  std::vector<int2> hr(nrng);
+  HIPCHECK(hipMemcpyAsync(hr.data(), slab_rng.p, nrng * sizeof(int2), hipMemcpyDeviceToHost, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  int tmax = 0;
+  for (const int2& r : hr) tmax = std::max(tmax, r.y - r.x + 1);
+  if (tmax <= 0) return false;
+  // segment groups: consecutive segments of the storage order, as few as
+  // keep every slab within the LDS budget at the widest tangential range
+  auto group_floats = [&](int nax_sum, int n) {
+    return int64_t{2} * mi::kSlabLead + int64_t{n} * mi::kSlabPad + int64_t{tmax} * (nax_sum + int64_t{n} * mi::kSlabPad);
+  };
+  int64_t budget = std::min<int64_t>(int64_t{slab_lds_kb} * 1024, mi::kSlabMaxBytes) / 4;
+  for (int j = 0; j < nseg; ++j)
+    if (group_floats(hsegs[j].y, 1) > budget) budget = mi::kSlabMaxBytes / 4;
+  std::vector<int2> hgroups, hpos(nseg);
+  int64_t slab_floats = 0;
+  for (int j = 0; j < nseg;) {
+    int e = j, nax_sum = 0;
+    while (e < nseg && group_floats(nax_sum + hsegs[e].y, e - j + 1) <= budget) nax_sum += hsegs[e++].y;
+    if (e == j) return false;  // one segment's slab does not fit the LDS
+    int cum = 0;
+    for (int k = j; k < e; ++k) {
+      hsegs[k].w = static_cast<int>(hgroups.size());
+      hpos[k] = make_int2(cum, k - j);
+      cum += hsegs[k].y;
+    }
+    slab_floats = std::max(slab_floats, group_floats(nax_sum, e - j));
+    hgroups.push_back(make_int2(j, e));
+    j = e;
+  }
+  const int ngroups = static_cast<int>(hgroups.size());
+  slab_segs.alloc(nseg);
+  HIPCHECK(hipMemcpyAsync(slab_segs.p, hsegs.data(), nseg * sizeof(int4), hipMemcpyHostToDevice, stream));
+  slab_seg_pos.alloc(nseg);
+  HIPCHECK(hipMemcpyAsync(slab_seg_pos.p, hpos.data(), nseg * sizeof(int2), hipMemcpyHostToDevice, stream));
+  slab_groups.alloc(ngroups);
+  HIPCHECK(hipMemcpyAsync(slab_groups.p, hgroups.data(), ngroups * sizeof(int2), hipMemcpyHostToDevice, stream));
+  A.segs = slab_segs.p;
+  A.seg_pos = slab_seg_pos.p;
+  A.groups = slab_groups.p;
+  A.ngroups = ngroups;
+  // pass 2: terms per (subset, tile, phase, column)
+  const int64_t nph_max = int64_t{nv_max} * ngroups;
+  const int64_t nkeys = int64_t{nsub} * ntiles * nph_max * TC;
+  if (nkeys >= 0x7fffffffLL) return false;
+  DBuf<uint32_t> counts;  // one more zero at the end: the scan then ends with the total
+  counts.alloc(nkeys + 1);
+  HIPCHECK(hipMemsetAsync(counts.p, 0, (nkeys + 1) * sizeof(uint32_t), stream));
+  for (int s = 0; s < nsub; ++s)
+    mi::launch_slab_count(fams.p, d_fs.p, plan->subset_family_begin(s), plan->subset_family_begin(s + 1), pool.p, s,
+                          dims, A, counts.p, stream);
+  HIPCHECK(hipGetLastError());
+  // block of every (subset, tile, phase, wave): steps = the most terms of its columns
+  const int64_t nblocks = nkeys / TC * nwaves;
+  std::vector<uint32_t> hsteps(nblocks);
+  {
+    DBuf<uint32_t> d_steps;
+    d_steps.alloc(nblocks);
+    mi::launch_slab_steps(counts.p, nblocks, cpw, TC, d_steps.p, stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(hsteps.data(), d_steps.p, nblocks * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  std::vector<uint2> hh(nblocks);
+  std::vector<int64_t> load(ntiles, 0);
+  int64_t total = 0;
+  for (int64_t b = 0; b < nblocks; ++b) {
+    hh[b] = make_uint2(static_cast<uint32_t>(total), hsteps[b]);
+    total += int64_t{hsteps[b]} * cpw;
+    if (total >= (int64_t(1) << 31)) throw ApiError(MI_ERR_ARGUMENT, "back-projection table above 2^31 entries");
+    load[(b / nwaves / nph_max) % ntiles] += int64_t{hsteps[b]} * cpw;
+  }
+  slab_hdr.alloc(nblocks);
+  HIPCHECK(hipMemcpyAsync(slab_hdr.p, hh.data(), nblocks * sizeof(uint2), hipMemcpyHostToDevice, stream));
+  // rank of every term within its column's list: the list's first position
+  // in the sorted order (exclusive scan of the counts)
+  DBuf<uint32_t> list_start;
+  list_start.alloc(nkeys + 1);
+  {
+    size_t tb = 0;
+    HIPCHECK(mi::exclusive_scan_u32(nullptr, tb, counts.p, list_start.p, static_cast<int>(nkeys + 1), stream));
+    DBuf<char> tmp;
+    tmp.alloc(std::max<size_t>(tb, 1));
+    HIPCHECK(mi::exclusive_scan_u32(tmp.p, tb, counts.p, list_start.p, static_cast<int>(nkeys + 1), stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  counts.release();
+  // pass 3: every element's key and entry in family order, element order;
+  // stable sort by key; each sorted element to its block slot
+  std::vector<int64_t> fam_off(h_fams.size() + 1, 0);
+  for (size_t i = 0; i < h_fams.size(); ++i) fam_off[i + 1] = fam_off[i] + h_fams[i].row_len;
+  const int64_t n = fam_off.back();
+  if (n >= (int64_t(1) << 31)) throw ApiError(MI_ERR_ARGUMENT, "more than 2^31 matrix elements");
+  slab_ent.alloc(std::max<int64_t>(total, 1));
+  mi::launch_slab_fill(slab_ent.p, static_cast<int64_t>(slab_ent.n), stream);
+  if (n > 0) {
+    DBuf<int64_t> d_famoff;
+    d_famoff.alloc(fam_off.size());
+    HIPCHECK(hipMemcpyAsync(d_famoff.p, fam_off.data(), fam_off.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    DBuf<uint32_t> keys_in, keys_out, vals_in, vals_out;
+    DBuf<mi::SlabEntry> raw;
+    keys_in.alloc(n);
+    raw.alloc(n);
+    A.hdr = slab_hdr.p;
+    for (int s = 0; s < nsub; ++s) {
+      const int fb = plan->subset_family_begin(s), fe = plan->subset_family_begin(s + 1);
+      mi::launch_slab_make(fams.p, d_fs.p, fb, fe, d_famoff.p + fb, pool.p, s, dims, A, keys_in.p, raw.p, stream);
+    }
+    HIPCHECK(hipGetLastError());
+    keys_out.alloc(n);
+    vals_in.alloc(n);
+    vals_out.alloc(n);
+    mi::launch_iota(vals_in.p, n, stream);
+    size_t tb = 0;
+    HIPCHECK(mi::radix_sort_pairs(nullptr, tb, keys_in.p, keys_out.p, vals_in.p, vals_out.p, static_cast<int>(n), 32,
+                                  stream));
+    DBuf<char> tmp;
+    tmp.alloc(std::max<size_t>(tb, 1));
+    HIPCHECK(mi::radix_sort_pairs(tmp.p, tb, keys_in.p, keys_out.p, vals_in.p, vals_out.p, static_cast<int>(n), 32,
+                                  stream));
+    // elements without an image voxel (key 0xffffffff) sort last: place the others
+    uint32_t nvalid = 0;
+    HIPCHECK(hipMemcpyAsync(&nvalid, list_start.p + nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (static_cast<int64_t>(nvalid) > n) throw ApiError(MI_ERR_INTERNAL, "slab table count");
+    mi::launch_slab_place(keys_out.p, vals_out.p, raw.p, list_start.p, slab_hdr.p, nvalid, cpw, TC, slab_ent.p, stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  // tiles with entries first, most entries first (the longest workgroups start first)
+  std::vector<int32_t> order(ntiles);
+  for (int t = 0; t < ntiles; ++t) order[t] = t;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return load[a] > load[b]; });
+  int nbusy = 0;
+  while (nbusy < ntiles && load[order[nbusy]] > 0) ++nbusy;
+  slab_tiles.alloc(ntiles);
+  HIPCHECK(hipMemcpyAsync(slab_tiles.p, order.data(), ntiles * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  A.hdr = slab_hdr.p;
+  A.ent = slab_ent.p;
+  A.tile_order = slab_tiles.p;
+  A.nbusy = nbusy;
+  A.slab_floats = static_cast<int32_t>(slab_floats);
+  slab = A;
+  return true;
+}
+
 void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   HIPCHECK(hipSetDevice(device));
   // release everything of a previous set-up
   pool.release(); pool_ops.release(); fams.release(); tasks.release(); list_off.release(); entries.release();
+  release_slab();
   img.release(); img2.release(); sino.release(); ratio.release(); stage_img.release(); stage_sino.release();
   y.release(); add.release(); mult.release(); ones.release(); vg_small.release(); x.release(); sens.clear();
   vg_off.release(); vg_len.release(); partial.release(); prior_partial.release();
@@ -770,19 +1023,28 @@ int mi_get_info(mi_ctx* ctx, mi_info* out) {
     out->num_basic_elements = ctx->num_basic_elements;
     out->num_families = static_cast<int64_t>(ctx->h_fams.size());
     out->num_matrix_elements = ctx->num_matrix_elements;
-    out->num_table_entries = static_cast<int64_t>(ctx->entries.n);
+    out->num_table_entries = static_cast<int64_t>(ctx->use_slab ? ctx->slab_ent.n : ctx->entries.n);
     out->num_nonzeros = ctx->num_nonzeros;
     out->device_bytes = static_cast<int64_t>(ctx->pool.bytes() + ctx->pool_ops.bytes() + ctx->fams.bytes() + ctx->tasks.bytes() +
                                              ctx->list_off.bytes() + ctx->entries.bytes() + ctx->img.bytes() +
                                              ctx->img2.bytes() + ctx->sino.bytes() + ctx->ratio.bytes() +
                                              ctx->stage_img.bytes() + ctx->stage_sino.bytes() + ctx->y.bytes() +
-                                             ctx->add.bytes() + ctx->mult.bytes() + ctx->ones.bytes() + ctx->x.bytes());
+                                             ctx->add.bytes() + ctx->mult.bytes() + ctx->ones.bytes() + ctx->x.bytes() +
+                                             ctx->slab_hdr.bytes() + ctx->slab_ent.bytes() + ctx->slab_rng.bytes() +
+                                             ctx->slab_tiles.bytes());
     for (auto& s : ctx->sens) out->device_bytes += static_cast<int64_t>(s->bytes());
     out->num_subsets = ctx->plan->num_subsets();
     ctx->plan->symmetry_flags(out->symmetry_flags);
     out->nz_padded = ctx->dims.nzp;
     out->planes_per_axial_position = ctx->uniform_ppa;
     ctx->design_bytes(out->fp_design_bytes, out->bp_design_bytes);
+    out->slab_back_projection = ctx->use_slab ? 1 : 0;
+    if (ctx->use_slab) {
+      out->slab_tile_x = ctx->slab.tx;
+      out->slab_tile_y = ctx->slab.ty;
+      out->slab_segment_groups = ctx->slab.ngroups;
+      out->slab_lds_bytes = ctx->slab.slab_floats * 4;
+    }
   });
 }
 
@@ -1691,6 +1953,20 @@ int mi_set_tuning(mi_ctx* ctx, int param, int value) {
         if (value < 0 || value > 4096) throw ApiError(MI_ERR_ARGUMENT, "forward XCD chunk out of range");
         ctx->fp_xcd_chunk = value;
         break;
+      case MI_TUNE_BP_SLAB: ctx->slab_enable = value != 0; break;
+      case MI_TUNE_SLAB_TILE_X:
+      case MI_TUNE_SLAB_TILE_Y:
+        if (value < 1 || value > 64) throw ApiError(MI_ERR_ARGUMENT, "slab tile side out of range");
+        (param == MI_TUNE_SLAB_TILE_X ? ctx->slab_tx : ctx->slab_ty) = value;
+        break;
+      case MI_TUNE_SLAB_COLUMNS_PER_WAVE:
+        if (value != 2 && value != 4 && value != 8) throw ApiError(MI_ERR_ARGUMENT, "columns per wave must be 2, 4 or 8");
+        ctx->slab_cpw = value;
+        break;
+      case MI_TUNE_SLAB_LDS_KB:
+        if (value < 16 || value > 160) throw ApiError(MI_ERR_ARGUMENT, "slab LDS budget out of range");
+        ctx->slab_lds_kb = value;
+        break;
       default: throw ApiError(MI_ERR_ARGUMENT, "unknown tuning parameter");
     }
   });
@@ -2025,6 +2301,7 @@ int mi_batch_create(int device, int n, const mi_geometry* geometries, int num_su
       mi_ctx* c = nullptr;
       if (mi_ctx_create(device, &c) != MI_OK) throw ApiError(MI_ERR_DEVICE, g_last_error);
       b->probs.push_back(c);
+      c->slab_allowed = false;  // the batch kernels read the column table
       if (mi_set_up(c, &geometries[i], num_subsets) != MI_OK)
         throw ApiError(MI_ERR_ARGUMENT, "problem " + std::to_string(i) + ": " + g_last_error);
     }

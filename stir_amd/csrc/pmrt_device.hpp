@@ -203,6 +203,55 @@ void launch_fp_batch(FpMode mode, int nslot, const ProblemDesc* d_descs, const F
 void launch_bp_batch(BpMode mode, int ppa, int nslot, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork,
                      int subset, int threshold, hipStream_t stream);
 
+// ---- back projection through LDS slabs (pmrt_slab.hip): tables of one
+// plane per axial position whose families cover whole runs (z-shift
+// symmetry), images of at most 128 planes
+constexpr int kSlabPad = 64;               // zeros opening each segment of a slab and following every run
+constexpr int kSlabLead = 128;             // zeros before the first segment and after the last one
+constexpr int kSlabMaxBytes = 160 * 1024;  // LDS of a workgroup on gfx950
+// table entry: byte offset in the slab of axial index -k of the entry's run
+// (k = plane of its axial position 0), and its weight for each 64-plane slot
+// (0 for a slot it does not reach)
+struct SlabEntry {
+  int32_t off4;
+  float w0, w1;
+};
+static_assert(sizeof(SlabEntry) == 12, "SlabEntry layout");
+// first slab index of a segment (after `before` segments of `cum` axial
+// positions in total in its group) in a slab of T tangential positions
+__host__ __device__ inline int slab_segment_start(int before, int cum, int T) {
+  return kSlabLead + before * kSlabPad + T * (cum + before * kSlabPad);
+}
+struct SlabArgs {
+  const uint2* hdr;        // per (subset, tile, phase, wave): first table entry, steps
+  const SlabEntry* ent;
+  const int2* rng;         // tangential index range per (subset, tile, view slot); y < x: no entries
+  const int32_t* view_of;  // view of (subset, view slot)
+  const int32_t* nviews;   // view slots per subset
+  const int4* segs;        // per segment in storage order: device offset, nax, 0, group
+  const int2* seg_pos;     // per segment: nax of the segments before it in its group, their number
+  const int2* groups;      // per segment group: first segment slot, end slot
+  const int32_t* tile_order;  // tiles with entries (most entries first), then the empty ones
+  int32_t tx, ty, ntx, ntiles, nv_max, ngroups, nseg, ntang, cpw, nwaves;
+  int32_t nbusy;        // tiles with entries
+  int32_t slab_floats;  // LDS floats of a workgroup (the largest slab)
+};
+void launch_slab_range(const Family* d_fams, const int4* d_fam_slab, int fam_begin, int fam_end, const int2* d_pool,
+                       int subset, const ImageDims& dims, const SlabArgs& A, int2* d_rng, hipStream_t stream);
+void launch_slab_count(const Family* d_fams, const int4* d_fam_slab, int fam_begin, int fam_end, const int2* d_pool,
+                       int subset, const ImageDims& dims, const SlabArgs& A, uint32_t* d_counts, hipStream_t stream);
+void launch_slab_make(const Family* d_fams, const int4* d_fam_slab, int fam_begin, int fam_end,
+                      const int64_t* d_fam_entry_off, const int2* d_pool, int subset, const ImageDims& dims,
+                      const SlabArgs& A, uint32_t* d_keys, SlabEntry* d_entries, hipStream_t stream);
+void launch_slab_steps(const uint32_t* d_counts, int64_t nblocks, int cpw, int tc, uint32_t* d_steps,
+                       hipStream_t stream);
+void launch_slab_place(const uint32_t* d_keys_sorted, const uint32_t* d_idx_sorted, const SlabEntry* d_raw,
+                       const uint32_t* d_list_start, const uint2* d_hdr, int64_t n, int cpw, int tc,
+                       SlabEntry* d_table, hipStream_t stream);
+void launch_slab_fill(SlabEntry* d_table, int64_t n, hipStream_t stream);
+void launch_back_project_slab(BpMode mode, const SlabArgs& A, int subset_begin, int subset_end, const float* d_proj,
+                              float* d_img, const ImageDims& dims, const BpData& data, hipStream_t stream);
+
 // ---- BP table build (pmrt_table.hip)
 void launch_count_entries(const Family* d_fams, int fam_begin, int fam_end, const int2* d_pool, int subset,
                           const ImageDims& dims, int32_t* d_counts, hipStream_t stream);

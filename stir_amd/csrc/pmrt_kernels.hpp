@@ -102,6 +102,20 @@ __device__ inline void inplane_op(int x, int y, int& xo, int& yo) {
   else { xo = -y; yo = -x; }
 }
 
+// the in-plane operation ip (an IP_* value) at run time
+__device__ inline void inplane_any_op(int ip, int x, int y, int& xo, int& yo) {
+  switch (ip) {
+    case IP_ID: inplane_op<IP_ID>(x, y, xo, yo); break;
+    case IP_XMX: inplane_op<IP_XMX>(x, y, xo, yo); break;
+    case IP_YMY: inplane_op<IP_YMY>(x, y, xo, yo); break;
+    case IP_XMX_YMY: inplane_op<IP_XMX_YMY>(x, y, xo, yo); break;
+    case IP_XY_YX: inplane_op<IP_XY_YX>(x, y, xo, yo); break;
+    case IP_XMY_YX: inplane_op<IP_XMY_YX>(x, y, xo, yo); break;
+    case IP_XY_YMX: inplane_op<IP_XY_YMX>(x, y, xo, yo); break;
+    default: inplane_op<IP_XMY_YMX>(x, y, xo, yo); break;
+  }
+}
+
 __device__ inline void decode(int c, int& x, int& y, int& z) {
   x = (c & 1023) - 512;
   y = ((c >> 10) & 1023) - 512;

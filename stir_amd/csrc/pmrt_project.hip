@@ -120,7 +120,11 @@ __device__ inline void fp_row(cptr<int2> row, int n, const LaneBins<NSLOT>& L, c
 
 // As fp_row, from the row's copy for the family's in-plane operation (see
 // expand_rows_kernel): element = ((column << 12) | (z + 2048), weight), so the
-// column needs no decoding or symmetry arithmetic, only its byte offset.
+// column needs no decoding or symmetry arithmetic, only its byte offset. The
+// elements of the next kUnroll group are requested before the current
+// group's gathers are issued (the copies end in kUnroll elements of slack, so
+// reading a group past the row's end stays inside the allocation; those
+// elements are never used).
 template <int NSLOT>
 __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& L, cptr<float> img, const ImageDims& d,
                                   float (&acc)[NSLOT]) {
@@ -128,7 +132,7 @@ __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& 
   for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
   int e = 0;
 #if defined(__HIP_DEVICE_COMPILE__)
-  {
+  if (n >= kUnroll) {
     // image z of element z + 2048 is zs (z + 2048) + zb - 2048 zs
     int zs4[NSLOT], zb4[NSLOT];
 #pragma unroll
@@ -136,15 +140,20 @@ __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& 
       zs4[q] = 4 * L.zs[q];
       zb4[q] = 4 * L.zb[q] - 2048 * zs4[q];
     }
+    int2 cur[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) cur[j] = row[j];
     for (; e + kUnroll <= n; e += kUnroll) {
+      int2 nxt[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) nxt[j] = row[e + kUnroll + j];
       float v[kUnroll][NSLOT], w[kUnroll];
 #pragma unroll
       for (int j = 0; j < kUnroll; ++j) {
-        const int2 el = row[e + j];
-        const unsigned col = static_cast<unsigned>(el.x) >> 12;
-        const int zu = el.x & 0xfff;
+        const unsigned col = static_cast<unsigned>(cur[j].x) >> 12;
+        const int zu = cur[j].x & 0xfff;
         const __amdgpu_buffer_rsrc_t column = run_buffer(img, col * static_cast<unsigned>(d.nzp), d.nz);
-        w[j] = __int_as_float(el.y);
+        w[j] = __int_as_float(cur[j].y);
 #pragma unroll
         for (int q = 0; q < NSLOT; ++q) v[j][q] = load_run(column, __mul24(zs4[q], zu) + zb4[q]);
       }
@@ -152,6 +161,8 @@ __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& 
       for (int j = 0; j < kUnroll; ++j)
 #pragma unroll
         for (int q = 0; q < NSLOT; ++q) acc[q] = acc[q] + v[j][q] * w[j];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) cur[j] = nxt[j];
     }
   }
 #endif

@@ -161,4 +161,22 @@ hipError_t exclusive_scan_counts(void* d_temp, size_t& temp_bytes, const int32_t
   return hipcub::DeviceScan::ExclusiveSum(d_temp, temp_bytes, in, out, n, stream);
 }
 
+hipError_t exclusive_scan_u32(void* d_temp, size_t& temp_bytes, const uint32_t* in, uint32_t* out, int n,
+                              hipStream_t stream) {
+  return hipcub::DeviceScan::ExclusiveSum(d_temp, temp_bytes, in, out, n, stream);
+}
+
+namespace {
+__global__ void fill_int2_kernel(int2* d, int64_t n, int2 v) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    d[i] = v;
+}
+}  // namespace
+
+void launch_fill_int2(int2* d, int64_t n, int2 v, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_int2_kernel, dim3(grid_for(n)), dim3(256), 0, stream, d, n, v);
+}
+
 }  // namespace mi
