@@ -337,6 +337,22 @@ int mi_obj_value_and_gradient_dev(mi_ctx* ctx, const float* d_image, int subset,
 int mi_quadratic_prior_dev(mi_ctx* ctx, const float* d_image, const float* d_kappa, const float* weights, float beta,
                            int only_2D, float* d_gradient, double* d_value, void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mi_quadratic_prior_dev of a device image of nz x ny x nx
+ * voxels (STIR layout, voxel sizes in mm for the default weights) with no
+ * context and no allocation. The prior's FP64 partial sums, one per
+ * workgroup of 64 x 4 voxels of one plane, live in a fixed array of
+ * 65536 doubles (512 KB) that every GPU the library runs on holds until the
+ * process ends: images of more than 65536 such workgroups (e.g. above
+ * 256 x 256 x 1024) are rejected. Calls that overlap on one GPU (other
+ * streams, other threads) race on that array; calls on one stream do not.
+ * device becomes the current device. Results equal mi_quadratic_prior_dev's
+ * on the same image. */
+int mi_quadratic_prior_noalloc_dev(int device, int nz, int ny, int nx, float vox_z, float vox_y, float vox_x,
+                                   const float* d_image, const float* d_kappa, const float* weights, float beta,
+                                   int only_2D, float* d_gradient, double* d_value, void* stream);
+
 /* RelativeDifferencePrior::compute_value and compute_gradient
  * (RelativeDifferencePrior.cxx:250-433) of a device image, arguments as
  * mi_quadratic_prior_dev plus gamma and epsilon. */

@@ -31,7 +31,7 @@ EXPORTED_SYMBOLS = (
     "mi_obj_set_multiplicative", "mi_obj_set_use_subset_sensitivities", "mi_osem_set_options", "mi_obj_set_prior",
     "mi_relative_difference_prior_dev", "mi_batch_set_row_groups", "mi_lm_set_events",
     "mi_prior_hessian_times_input_dev", "mi_basic_view_segments_in_subset", "mi_set_image",
-    "mi_forward_project_view_segments", "mi_back_project_view_segments",
+    "mi_forward_project_view_segments", "mi_back_project_view_segments", "mi_quadratic_prior_noalloc_dev",
 )
 
 MI_PRIOR_NONE, MI_PRIOR_QUADRATIC, MI_PRIOR_RELATIVE_DIFFERENCE = 0, 1, 2
@@ -128,6 +128,8 @@ def load():
     L.mi_set_tuning.argtypes = [vp, c_int, c_int]
     L.mi_obj_value_and_gradient_dev.argtypes = [vp, vp, c_int, c_int, vp, vp, vp]
     L.mi_quadratic_prior_dev.argtypes = [vp, vp, vp, vp, ctypes.c_float, c_int, vp, vp, vp]
+    L.mi_quadratic_prior_noalloc_dev.argtypes = [c_int, c_int, c_int, c_int, ctypes.c_float, ctypes.c_float,
+                                                ctypes.c_float, vp, vp, vp, ctypes.c_float, c_int, vp, vp, vp]
     L.mi_osem_subset_step_dev.argtypes = [vp, vp, vp, vp, c_int, c_int, vp]
     L.mi_osem_subset_step_vjp_dev.argtypes = [vp, vp, vp, vp, vp, c_int, c_int, vp]
     L.mi_set_kernel_timing.argtypes = [vp, c_int]

@@ -1664,6 +1664,37 @@ int mi_quadratic_prior_dev(mi_ctx* ctx, const float* d_image, const float* d_kap
   });
 }
 
+int mi_quadratic_prior_noalloc_dev(int device, int nz, int ny, int nx, float vox_z, float vox_y, float vox_x,
+                                   const float* d_image, const float* d_kappa, const float* weights, float beta,
+                                   int only_2D, float* d_gradient, double* d_value, void* stream) {
+  return guarded([&] {
+    if (!d_image) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    if (!d_gradient && !d_value) throw ApiError(MI_ERR_ARGUMENT, "neither a gradient nor a value was asked for");
+    if (nz < 1 || ny < 1 || nx < 1) throw ApiError(MI_ERR_ARGUMENT, "empty image");
+    const mi::PriorImage im{nz, ny, nx, nz, 0};
+    const int nparts = mi::prior_num_partials(im);
+    if (nparts > mi::kNoAllocPartials)
+      throw ApiError(MI_ERR_ARGUMENT, "image too large for the fixed partial sums (" + std::to_string(nparts) +
+                                          " workgroups, at most " + std::to_string(mi::kNoAllocPartials) + ")");
+    if (device < 0 || device >= mi::kMaxDevices) throw ApiError(MI_ERR_ARGUMENT, "device out of range");
+    HIPCHECK(hipSetDevice(device));
+    double* partials = mi::noalloc_partials(device);
+    if (!partials) throw ApiError(MI_ERR_DEVICE, "hipGetSymbolAddress of the partial sums failed");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (beta == 0.F) {  // QuadraticPrior.cxx:233-237,305-309
+      if (d_gradient) mi::launch_fill(d_gradient, static_cast<int64_t>(nz) * ny * nx, 0.F, st);
+      if (d_value) mi::launch_sum_partials(partials, 0, 0.0, d_value, st);
+    } else {
+      mi::PriorWeights W;
+      if (weights) std::memcpy(W.w, weights, sizeof(W.w));
+      else default_prior_weights(vox_z, vox_y, vox_x, W.w);
+      mi::launch_prior(1, d_image, d_kappa, im, W, only_2D, beta, 0.F, 0.F, d_gradient, partials, st);
+      if (d_value) mi::launch_sum_partials(partials, nparts, static_cast<double>(beta), d_value, st);
+    }
+    HIPCHECK(hipGetLastError());
+  });
+}
+
 int mi_osem_subset_step_dev(mi_ctx* ctx, const float* d_x_in, float* d_x_out, float* d_update, int subset,
                             int subiteration_num, void* stream) {
   return guarded([&] {

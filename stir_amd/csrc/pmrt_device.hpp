@@ -293,6 +293,12 @@ void launch_prior(int kind, const float* d_img, const float* d_kappa, const Prio
 void launch_prior_hessian(int kind, const float* d_x, const float* d_in, const float* d_kappa, const PriorImage& im,
                           const PriorWeights& w, int only_2D, float beta, float gamma, float eps, float* d_out,
                           hipStream_t stream);
+// Allocation-free prior: FP64 partials in a fixed per-device array of
+// kNoAllocPartials doubles (512 KB), one per prior workgroup; its device
+// address on `device`, which must be current (null if it cannot be had).
+constexpr int kNoAllocPartials = 65536;
+constexpr int kMaxDevices = 64;
+double* noalloc_partials(int device);
 // *d_small = max(max(v) * small_num, 0) over n floats (d_part: max_num_partials() floats)
 int max_num_partials();
 void launch_small_value(const float* d_v, int64_t n, float small_num, float* d_part, float* d_small, hipStream_t stream);

@@ -5,10 +5,18 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <mutex>
 
 #include "pmrt_device.hpp"
 
 namespace mi {
+
+// The FP64 partial sums of the allocation-free prior entry point: a fixed
+// array in every GPU's global memory, held until the process ends. Host code
+// reaches it only through the per-device address from hipGetSymbolAddress
+// (noalloc_partials); the array's name is not a device address on the host.
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
 namespace {
 
 constexpr int kPriorTileX = 64;
@@ -396,6 +404,21 @@ void launch_osem_vjp_mask(const float* d_g, const float* d_u, float* d_out, int6
 void launch_add_product(float* d_out, const float* d_g, const float* d_u, int64_t n, hipStream_t stream) {
   if (n <= 0) return;
   hipLaunchKernelGGL(add_product_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_out, d_g, d_u, n);
+}
+
+// the device address of `scratch` on `device` (the current device), looked
+// up on the first call for that device and kept
+double* noalloc_partials(int device) {
+  static std::mutex m;
+  static double* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(m);
+  if (!addr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    addr[device] = static_cast<double*>(p);
+  }
+  return addr[device];
 }
 
 }  // namespace mi
