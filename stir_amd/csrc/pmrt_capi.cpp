@@ -29,6 +29,8 @@ void launch_fill_int2(int2* d, int64_t n, int2 v, hipStream_t stream);
 
 namespace {
 
+constexpr int kRowSlack = 16;  // elements after the row copies: the forward kernel reads a group ahead
+
 thread_local std::string g_last_error;
 
 struct ApiError : std::runtime_error {
@@ -479,8 +481,8 @@ void mi_ctx::build_rows() {
   pool_ops.release();
   if (ntof == 1 && total > 0) {
     // 8 copies, then slack for the forward kernel's read-ahead of one element group
-    pool_ops.alloc(8 * total + 16);
-    HIPCHECK(hipMemsetAsync(pool_ops.p + 8 * total, 0, 16 * sizeof(int2), stream));
+    pool_ops.alloc(8 * total + kRowSlack);
+    HIPCHECK(hipMemsetAsync(pool_ops.p + 8 * total, 0, kRowSlack * sizeof(int2), stream));
     mi::launch_expand_rows(pool.p, total, dims, pool_ops.p, stream);
     HIPCHECK(hipGetLastError());
   }

@@ -29,98 +29,15 @@ namespace {
 using namespace dev;
 
 constexpr int kUnroll = 8;      // matrix elements whose gathers are in flight together
+constexpr int kRowSlack = 16;   // elements after the last row copy (read-ahead of the forward row loop)
 
 // Sum of img * w over one row in row order for every bin of the lane
-// (ProjMatrixElemsForOneBin.cxx:354-371). kUnroll elements have their
-// gathers issued before they are accumulated; a voxel outside the image adds
-// nothing, so each sum is the reference's, term by term.
-template <int IP, int NSLOT>
-__device__ inline void fp_row(cptr<int2> row, int n, const LaneBins<NSLOT>& L, cptr<float> img, const ImageDims& d,
-                              float (&acc)[NSLOT]) {
-#pragma unroll
-  for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
-  int e = 0;
-#if defined(__HIP_DEVICE_COMPILE__)
-  // Each element's image column as a raw buffer of nz values, read at the
-  // lane's 4 z: a voxel outside the image reads 0, and x + 0*w == x, so every
-  // bin still sums exactly the reference's terms. Lanes without a bin sum
-  // values that are never stored.
-  {
-    int zs4[NSLOT], zb4[NSLOT];
-#pragma unroll
-    for (int q = 0; q < NSLOT; ++q) {
-      zs4[q] = 4 * L.zs[q];
-      zb4[q] = 4 * L.zb[q];
-    }
-    for (; e + kUnroll <= n; e += kUnroll) {
-      float v[kUnroll][NSLOT], w[kUnroll];
-#pragma unroll
-      for (int j = 0; j < kUnroll; ++j) {
-        const int2 el = row[e + j];
-        int x, y, z, xo, yo;
-        decode(el.x, x, y, z);
-        inplane_op<IP>(x, y, xo, yo);
-        const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-        const __amdgpu_buffer_rsrc_t column = run_buffer(img, static_cast<unsigned>(col * d.nzp), d.nz);
-        w[j] = __int_as_float(el.y);
-#pragma unroll
-        for (int q = 0; q < NSLOT; ++q) v[j][q] = load_run(column, __mul24(zs4[q], z) + zb4[q]);
-      }
-#pragma unroll
-      for (int j = 0; j < kUnroll; ++j)
-#pragma unroll
-        for (int q = 0; q < NSLOT; ++q) acc[q] = acc[q] + v[j][q] * w[j];
-    }
-  }
-#endif
-  for (; e + kUnroll <= n; e += kUnroll) {
-    float v[kUnroll][NSLOT], w[kUnroll];
-    bool ok[kUnroll][NSLOT];
-#pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-      const int2 el = row[e + j];
-      int x, y, z, xo, yo;
-      decode(el.x, x, y, z);
-      inplane_op<IP>(x, y, xo, yo);
-      const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-      // the column's byte offset (wave-uniform, 32 bits: images hold fewer
-      // than 2^29 padded voxels, checked at set-up) joins the lane's offset,
-      // so every gather addresses from the kernel-wide image base
-      const unsigned col4 = static_cast<unsigned>(col * d.nzp) << 2;
-      w[j] = __int_as_float(el.y);
-#pragma unroll
-      for (int q = 0; q < NSLOT; ++q) {
-        const int zz = __mul24(L.zs[q], z) + L.zb[q];  // zs = +-1, |z| < 2^11
-        ok[j][q] = L.valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz);
-        const int zc = clamp_index(zz, d.nz);
-        v[j][q] = load_byteoff(img, col4 + (static_cast<unsigned>(zc) << 2));
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kUnroll; ++j)
-#pragma unroll
-      for (int q = 0; q < NSLOT; ++q)
-        if (ok[j][q]) acc[q] = acc[q] + v[j][q] * w[j];
-  }
-  for (; e < n; ++e) {
-    const int2 el = row[e];
-    int x, y, z, xo, yo;
-    decode(el.x, x, y, z);
-    inplane_op<IP>(x, y, xo, yo);
-    const int col = (yo - d.min_y) * d.nx + (xo - d.min_x);
-    const unsigned col4 = static_cast<unsigned>(col * d.nzp) << 2;
-#pragma unroll
-    for (int q = 0; q < NSLOT; ++q) {
-      const int zz = __mul24(L.zs[q], z) + L.zb[q];  // zs = +-1, |z| < 2^11
-      if (L.valid[q] && static_cast<unsigned>(zz) < static_cast<unsigned>(d.nz))
-        acc[q] = acc[q] + load_byteoff(img, col4 + (static_cast<unsigned>(zz) << 2)) * __int_as_float(el.y);
-    }
-  }
-}
-
-// As fp_row, from the row's copy for the family's in-plane operation (see
-// expand_rows_kernel): element = ((column << 12) | (z + 2048), weight), so the
-// column needs no decoding or symmetry arithmetic, only its byte offset. The
+// (ProjMatrixElemsForOneBin.cxx:354-371), from the row's copy for the family's
+// in-plane operation (see expand_rows_kernel): element = ((column << 12) |
+// (z + 2048), weight), so the column needs no decoding or symmetry arithmetic,
+// only its byte offset. Each element's image column is a range-checked raw
+// buffer of nz values read at the lane's z: a voxel outside the image reads 0,
+// and x + 0*w == x, so every bin still sums exactly the reference's terms. The
 // elements of the next kUnroll group are requested before the current
 // group's gathers are issued (the copies end in kUnroll elements of slack, so
 // reading a group past the row's end stays inside the allocation; those
@@ -128,11 +45,15 @@ __device__ inline void fp_row(cptr<int2> row, int n, const LaneBins<NSLOT>& L, c
 template <int NSLOT>
 __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& L, cptr<float> img, const ImageDims& d,
                                   float (&acc)[NSLOT]) {
+  // elements per group: 16 gathers in flight whatever the slot count, so
+  // that no task shape holds more registers than another needs
+  constexpr int U = NSLOT >= 4 ? 4 : (NSLOT == 3 ? 5 : (NSLOT == 2 ? 8 : 16));
+  static_assert(U <= kRowSlack, "read-ahead beyond the row copies' slack");
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
   int e = 0;
 #if defined(__HIP_DEVICE_COMPILE__)
-  if (n >= kUnroll) {
+  if (n >= U) {
     // image z of element z + 2048 is zs (z + 2048) + zb - 2048 zs
     int zs4[NSLOT], zb4[NSLOT];
 #pragma unroll
@@ -140,16 +61,16 @@ __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& 
       zs4[q] = 4 * L.zs[q];
       zb4[q] = 4 * L.zb[q] - 2048 * zs4[q];
     }
-    int2 cur[kUnroll];
+    int2 cur[U];
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) cur[j] = row[j];
-    for (; e + kUnroll <= n; e += kUnroll) {
-      int2 nxt[kUnroll];
+    for (int j = 0; j < U; ++j) cur[j] = row[j];
+    for (; e + U <= n; e += U) {
+      int2 nxt[U];
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) nxt[j] = row[e + kUnroll + j];
-      float v[kUnroll][NSLOT], w[kUnroll];
+      for (int j = 0; j < U; ++j) nxt[j] = row[e + U + j];
+      float v[U][NSLOT], w[U];
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) {
+      for (int j = 0; j < U; ++j) {
         const unsigned col = static_cast<unsigned>(cur[j].x) >> 12;
         const int zu = cur[j].x & 0xfff;
         const __amdgpu_buffer_rsrc_t column = run_buffer(img, col * static_cast<unsigned>(d.nzp), d.nz);
@@ -158,11 +79,11 @@ __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& 
         for (int q = 0; q < NSLOT; ++q) v[j][q] = load_run(column, __mul24(zs4[q], zu) + zb4[q]);
       }
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j)
+      for (int j = 0; j < U; ++j)
 #pragma unroll
         for (int q = 0; q < NSLOT; ++q) acc[q] = acc[q] + v[j][q] * w[j];
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) cur[j] = nxt[j];
+      for (int j = 0; j < U; ++j) cur[j] = nxt[j];
     }
   }
 #endif
@@ -183,7 +104,7 @@ __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& 
 template <int MODE, int NSLOT>
 __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, cptr<int2> pool, cptr<float> img,
                                float* __restrict__ out, const ImageDims& d, const FpData& data, int t, int lane,
-                               cptr<int2> ops = nullptr, int64_t nelem = 0) {
+                               cptr<int2> ops, int64_t nelem) {
   LaneBins<NSLOT> L;
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) {
@@ -200,21 +121,7 @@ __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, cp
     (void)f;
   }
   float acc[NSLOT];
-  if (ops) {
-    fp_row_ops<NSLOT>(ops + static_cast<int64_t>(f1.inplane) * nelem + f1.row_start, f1.row_len, L, img, d, acc);
-  } else {
-  cptr<int2> row = pool + f1.row_start;
-  switch (f1.inplane) {
-    case IP_ID: fp_row<IP_ID, NSLOT>(row, f1.row_len, L, img, d, acc); break;
-    case IP_XMX: fp_row<IP_XMX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
-    case IP_YMY: fp_row<IP_YMY, NSLOT>(row, f1.row_len, L, img, d, acc); break;
-    case IP_XMX_YMY: fp_row<IP_XMX_YMY, NSLOT>(row, f1.row_len, L, img, d, acc); break;
-    case IP_XY_YX: fp_row<IP_XY_YX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
-    case IP_XMY_YX: fp_row<IP_XMY_YX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
-    case IP_XY_YMX: fp_row<IP_XY_YMX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
-    default: fp_row<IP_XMY_YMX, NSLOT>(row, f1.row_len, L, img, d, acc); break;
-  }
-  }
+  fp_row_ops<NSLOT>(ops + static_cast<int64_t>(f1.inplane) * nelem + f1.row_start, f1.row_len, L, img, d, acc);
   double ll = 0.0;
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q)
