@@ -360,7 +360,10 @@ def tof_bench(dev, x, steps, nsub, radius):
     ctx.set_kernel_timing(False)
     out = {"pairs_per_s": round(1.0 / dt, 4), "ms_per_pair": round(dt * 1000, 3),
            "fp_kernel_ms": round(fp_ms / max(nfp, 1), 3), "bp_kernel_ms": round(bp_ms / max(nbp, 1), 3),
-           "num_bins": int(pdi.num_bins), "tof_bins": int(pdi.num_tof_bins), "setup_s": round(setup_s, 3)}
+           "num_bins": int(pdi.num_bins), "tof_bins": int(pdi.num_tof_bins), "setup_s": round(setup_s, 3),
+           "weights": ("stored windows of %d (forward) and %d (back) TOF bins" % (info.tof_fp_window, info.tof_bp_window)
+                       if info.tof_fp_window else "computed on the fly"),
+           "live_span": int(info.tof_live_span)}
     log(f"TOF pair {dt * 1000:.2f} ms (fp {out['fp_kernel_ms']} ms, bp {out['bp_kernel_ms']} ms)")
     y, _ = phantoms.poisson_data(sino.cpu().numpy(), 0.3, 0.05, seed=303)
     del sino, back

@@ -114,7 +114,12 @@ typedef struct {
   int32_t slab_back_projection;
   int32_t slab_tile_x, slab_tile_y, slab_segment_groups;
   int32_t slab_lds_bytes; /* LDS of one slab workgroup */
-  int32_t reserved0;
+  /* TOF: window widths of the stored forward and back records (TOF bins per
+   * element; 0 when the weights are computed on the fly) and the widest
+   * range of TOF bins with a non-zero weight of one element */
+  int32_t tof_fp_window;
+  int32_t tof_bp_window;
+  int32_t tof_live_span;
 } mi_info;
 
 typedef struct mi_ctx mi_ctx;
@@ -452,12 +457,20 @@ int mi_batch_synchronize(mi_batch* batch);
  * MI_TUNE_SLAB_COLUMNS_PER_WAVE (2, 4 or 8, default 4) the columns each wave
  * of the tile's workgroup sums, and MI_TUNE_SLAB_LDS_KB (16-160, default 80)
  * the LDS a slab may take (the segments are split into as few groups per
- * view as keep it there). These take effect at the next mi_set_up (they
- * decide the table's layout); the sums of one setting are deterministic,
- * other settings agree to float rounding. */
+ * view as keep it there).
+ * MI_TUNE_TOF_ROWS selects how TOF projections get each element's TOF
+ * weights: 1 (default) from records built at set-up that hold the weights of
+ * a window of consecutive TOF bins per row element and per table entry (the
+ * narrowest window that holds every element's live bins), 2 the same with
+ * wide windows (14 forward, 13 back), 0 computed on the fly in every
+ * projection (also used when no window fits). Forward projections are the
+ * same bit for bit in all three.
+ * These take effect at the next mi_set_up (they decide the table's layout);
+ * the sums of one setting are deterministic, other settings agree to float
+ * rounding. */
 enum { MI_TUNE_BP_TILE_X = 1, MI_TUNE_BP_TILE_Y = 2, MI_TUNE_BP_XCD_REMAP = 3, MI_TUNE_BP_SLOT_SKIP = 4,
        MI_TUNE_FP_XCD_CHUNK = 5, MI_TUNE_BP_SLAB = 6, MI_TUNE_SLAB_TILE_X = 7, MI_TUNE_SLAB_TILE_Y = 8,
-       MI_TUNE_SLAB_COLUMNS_PER_WAVE = 9, MI_TUNE_SLAB_LDS_KB = 10 };
+       MI_TUNE_SLAB_COLUMNS_PER_WAVE = 9, MI_TUNE_SLAB_LDS_KB = 10, MI_TUNE_TOF_ROWS = 11 };
 int mi_set_tuning(mi_ctx* ctx, int param, int value);
 
 /* ---- diagnostics ----------------------------------------------------------- */

@@ -51,6 +51,7 @@ class PriorStruct(ctypes.Structure):
 
 MI_TUNE_BP_TILE_X, MI_TUNE_BP_TILE_Y, MI_TUNE_BP_XCD_REMAP, MI_TUNE_BP_SLOT_SKIP, MI_TUNE_FP_XCD_CHUNK = 1, 2, 3, 4, 5
 MI_TUNE_BP_SLAB, MI_TUNE_SLAB_TILE_X, MI_TUNE_SLAB_TILE_Y, MI_TUNE_SLAB_COLUMNS_PER_WAVE, MI_TUNE_SLAB_LDS_KB = 6, 7, 8, 9, 10
+MI_TUNE_TOF_ROWS = 11
 
 
 class NativeError(RuntimeError):
@@ -79,7 +80,9 @@ class InfoStruct(ctypes.Structure):
         ("slab_tile_y", ctypes.c_int32),
         ("slab_segment_groups", ctypes.c_int32),
         ("slab_lds_bytes", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("tof_fp_window", ctypes.c_int32),
+        ("tof_bp_window", ctypes.c_int32),
+        ("tof_live_span", ctypes.c_int32),
     ]
 
 

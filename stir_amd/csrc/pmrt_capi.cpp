@@ -141,6 +141,13 @@ struct mi_ctx {
   int64_t block_bins = 0;
   DBuf<mi::FamilyLor> lors;
   DBuf<float> entry_d;   // TOF distance of every back-projection table entry
+  // stored TOF rows (records with each element's TOF weights, pmrt_tof.hip):
+  // mode 0 computes the weights on the fly, 1 stores the narrowest windows
+  // that hold every element's live TOF bins, 2 the wide ones
+  int tof_rows_mode = 1;
+  DBuf<int32_t> tof_fp_rec, tof_bp_rec;
+  int tof_fp_nt = 0, tof_fp_w = 0, tof_bp_w = 0;  // window widths, 0 when not stored
+  int tof_live_span = 0;  // widest range of TOF bins with a non-zero weight of one element
   DBuf<double> erf_tab;  // FastErf samples
   mi::TofConsts tofc{};
   bool tof_sensitivities = false;  // "use time-of-flight sensitivities" (...AndProjData.cxx:143)
@@ -261,7 +268,10 @@ struct mi_ctx {
                 const mi::FpData& data, hipStream_t st) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
-    if (ntof > 1)
+    if (ntof > 1 && tof_fp_w > 0)
+      mi::launch_forward_project_tof_rows(mode, d_tasks, ntasks, fams.p, tof_fp_rec.p, tof_fp_nt, tof_fp_w, d_img, d_out,
+                                          dims, tofc, data, st);
+    else if (ntof > 1)
       mi::launch_forward_project_tof(mode, d_tasks, ntasks, fams.p, lors.p, pool.p, d_img, d_out, dims, tofc, data, st);
     else
       mi::launch_forward_project(mode, d_tasks, ntasks, fams.p, pool.p, d_img, d_out, dims, data, st, fp_xcd_chunk,
@@ -313,7 +323,10 @@ struct mi_ctx {
           bool nontof = false) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
-    if (ntof > 1 && !nontof)
+    if (ntof > 1 && !nontof && tof_bp_w > 0)
+      mi::launch_back_project_tof_rows(mode, uniform_ppa, list_off.p, tof_bp_rec.p, tof_bp_w, subset_begin(subset),
+                                       subset_end(subset), d_proj, d_img, dims, tofc, data, tof_bp_tuning, st);
+    else if (ntof > 1 && !nontof)
       mi::launch_back_project_tof(mode, uniform_ppa, list_off.p, entries.p, entry_d.p, subset_begin(subset),
                                   subset_end(subset), d_proj, d_img, dims, tofc, data, tof_bp_tuning, st);
     else if (use_slab)
@@ -332,21 +345,26 @@ struct mi_ctx {
     const int64_t img_bytes = int64_t{4} * dims.ncols * dims.nzp;
     const int64_t proj_bytes = int64_t{4} * total_bins();
     fp_bytes = img_bytes + proj_bytes;
+    const bool fp_rec = ntof > 1 && tof_fp_w > 0;
     for (const mi::FpTask& t : h_tasks) {
       const mi::Family& f = h_fams[t.first];
-      fp_bytes += int64_t{8} * f.row_len + int64_t(sizeof(mi::FpTask)) +
-                  int64_t(sizeof(mi::Family)) * (t.second >= 0 ? 2 : 1) +
-                  (ntof > 1 ? int64_t(sizeof(mi::FamilyLor)) : 0);
+      fp_bytes += (fp_rec ? int64_t{4} * mi::tof_fp_words(tof_fp_w) : int64_t{8}) * f.row_len +
+                  int64_t(sizeof(mi::FpTask)) + int64_t(sizeof(mi::Family)) * (t.second >= 0 ? 2 : 1) +
+                  (ntof > 1 && !fp_rec ? int64_t(sizeof(mi::FamilyLor)) : 0);
     }
-    bp_bytes = img_bytes + proj_bytes +
-               static_cast<int64_t>(entries.bytes() + list_off.bytes() + entry_d.bytes() + slab_hdr.bytes() +
-                                    slab_ent.bytes());
+    if (ntof > 1 && tof_bp_w > 0)
+      bp_bytes = img_bytes + proj_bytes + static_cast<int64_t>(tof_bp_rec.bytes() + list_off.bytes());
+    else
+      bp_bytes = img_bytes + proj_bytes +
+                 static_cast<int64_t>(entries.bytes() + list_off.bytes() + entry_d.bytes() + slab_hdr.bytes() +
+                                      slab_ent.bytes());
   }
   void set_up(const mi_geometry& g, int nsub);
   void build_rows();
   void build_tasks();
   void build_table();
   bool build_slab_table();
+  void build_tof_rows(int64_t n);
   mi::FpData fp_data() const {
     mi::FpData d{};
     d.measured = y.p;
@@ -666,6 +684,60 @@ void mi_ctx::build_table() {
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(stream));
   }
+  build_tof_rows(n);
+}
+
+// Stored TOF rows when every family has a row of its own, the planes per
+// axial position fit the record's 4 bits and the widest live range fits a
+// window; otherwise the on-the-fly kernels stay in use.
+void mi_ctx::build_tof_rows(int64_t n) {
+  tof_fp_rec.release();
+  tof_bp_rec.release();
+  tof_fp_nt = tof_fp_w = tof_bp_w = tof_live_span = 0;
+  if (ntof <= 1 || tof_rows_mode == 0 || h_fams.empty() || n <= 0) return;
+  std::vector<std::pair<int64_t, int64_t>> rows;
+  rows.reserve(h_fams.size());
+  int64_t pool_end = 0;
+  for (const mi::Family& f : h_fams) {
+    if (f.ppa < 1 || f.ppa > 15) return;
+    rows.emplace_back(f.row_start, f.row_len);
+    pool_end = std::max<int64_t>(pool_end, int64_t{f.row_start} + f.row_len);
+  }
+  std::sort(rows.begin(), rows.end());
+  for (size_t i = 1; i < rows.size(); ++i)
+    if (rows[i - 1].first + rows[i - 1].second > rows[i].first) return;
+  DBuf<int> span;
+  span.alloc(1);
+  HIPCHECK(hipMemsetAsync(span.p, 0, sizeof(int), stream));
+  mi::launch_tof_live_span(entries.p, entry_d.p, n, tofc, span.p, stream);
+  int s = 0;
+  HIPCHECK(hipMemcpyAsync(&s, span.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  tof_live_span = s;
+  int fnt, fw, bw;
+  if (tof_rows_mode == 1 && ntof <= 12 && s <= mi::kTofFpNarrow) {
+    fnt = 12;
+    fw = mi::kTofFpNarrow;
+  } else if (s <= mi::kTofFpWide) {
+    fnt = 16;
+    fw = mi::kTofFpWide;
+  } else {
+    return;
+  }
+  if (tof_rows_mode == 1 && s <= mi::kTofBpNarrow) bw = mi::kTofBpNarrow;
+  else if (s <= mi::kTofBpWide) bw = mi::kTofBpWide;
+  else return;
+  tof_fp_rec.alloc(pool_end * mi::tof_fp_words(fw));
+  mi::launch_tof_fp_records(fams.p, static_cast<int>(h_fams.size()), lors.p, pool.p, dims, tofc, fnt, fw,
+                            tof_fp_rec.p, stream);
+  tof_bp_rec.alloc(n * mi::tof_bp_words(bw));
+  mi::launch_tof_bp_records(entries.p, entry_d.p, n, tofc, bw, tof_bp_rec.p, stream);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(stream));
+  tof_fp_nt = fnt;
+  tof_fp_w = fw;
+  tof_bp_w = bw;
+  entry_d.release();  // read only by the on-the-fly back projection
 }
 
 // The slab table (pmrt_slab.hip): for one plane per axial position, families
@@ -892,6 +964,8 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   y.release(); add.release(); mult.release(); ones.release(); vg_small.release(); x.release(); sens.clear();
   vg_off.release(); vg_len.release(); partial.release(); prior_partial.release();
   lors.release(); entry_d.release(); erf_tab.release(); sel_tasks.release();
+  tof_fp_rec.release(); tof_bp_rec.release();
+  tof_fp_nt = tof_fp_w = tof_bp_w = tof_live_span = 0;
   have_data = have_sens = have_x = have_image = false;
   plan.reset();
   plan = std::make_unique<mi::Plan>(g, nsub);
@@ -1033,7 +1107,9 @@ int mi_get_info(mi_ctx* ctx, mi_info* out) {
                                              ctx->stage_img.bytes() + ctx->stage_sino.bytes() + ctx->y.bytes() +
                                              ctx->add.bytes() + ctx->mult.bytes() + ctx->ones.bytes() + ctx->x.bytes() +
                                              ctx->slab_hdr.bytes() + ctx->slab_ent.bytes() + ctx->slab_rng.bytes() +
-                                             ctx->slab_tiles.bytes());
+                                             ctx->slab_tiles.bytes() + ctx->lors.bytes() + ctx->entry_d.bytes() +
+                                             ctx->erf_tab.bytes() + ctx->tof_fp_rec.bytes() +
+                                             ctx->tof_bp_rec.bytes());
     for (auto& s : ctx->sens) out->device_bytes += static_cast<int64_t>(s->bytes());
     out->num_subsets = ctx->plan->num_subsets();
     ctx->plan->symmetry_flags(out->symmetry_flags);
@@ -1047,6 +1123,9 @@ int mi_get_info(mi_ctx* ctx, mi_info* out) {
       out->slab_segment_groups = ctx->slab.ngroups;
       out->slab_lds_bytes = ctx->slab.slab_floats * 4;
     }
+    out->tof_fp_window = ctx->tof_fp_w;
+    out->tof_bp_window = ctx->tof_bp_w;
+    out->tof_live_span = ctx->tof_live_span;
   });
 }
 
@@ -1999,6 +2078,10 @@ int mi_set_tuning(mi_ctx* ctx, int param, int value) {
       case MI_TUNE_SLAB_LDS_KB:
         if (value < 16 || value > 160) throw ApiError(MI_ERR_ARGUMENT, "slab LDS budget out of range");
         ctx->slab_lds_kb = value;
+        break;
+      case MI_TUNE_TOF_ROWS:
+        if (value < 0 || value > 2) throw ApiError(MI_ERR_ARGUMENT, "TOF row mode must be 0, 1 or 2");
+        ctx->tof_rows_mode = value;
         break;
       default: throw ApiError(MI_ERR_ARGUMENT, "unknown tuning parameter");
     }

@@ -147,6 +147,44 @@ void launch_entry_tof_distance(const Family* d_fams, int fam_begin, int fam_end,
                                const TofConsts& tof, float* d_entry_d, hipStream_t stream);
 void launch_gather_floats(const float* d_src, const uint32_t* d_idx, float* d_dst, int64_t n, hipStream_t stream);
 
+// Stored TOF rows: each element's TOF weights computed once (as the
+// on-the-fly kernels compute them) and kept as a window of consecutive TOF
+// bins.
+//   forward record of a row element, 2 + W words (tof_fp_words):
+//     [0] col | c << 20  (c: first bin of the window)
+//     [1] z << 16 | r  (r: elements from this one to the end of its run of
+//         equal c in the row, at most 65535)
+//     [2..] the weights of TOF bins c .. c + W - 1
+//   back record of a table entry, 3 + W words (tof_bp_words):
+//     [0] sino_base  [1] k  [2] nax | ppa << 16 | klo << 20 | nk << 25
+//     [3..] the weights of TOF bins klo .. klo + nk - 1, zeros up to W
+// A forward window starts at c = min(klo, nt - W) so it stays inside the nt
+// accumulators of the kernel; bins outside the live range carry weight 0.
+constexpr int kTofFpNarrow = 6, kTofFpWide = 14;  // forward window widths (8- and 16-word records)
+constexpr int kTofBpNarrow = 5, kTofBpWide = 13;  // back window widths (8- and 16-word records)
+constexpr int tof_fp_words(int w) { return 2 + w; }
+constexpr int tof_bp_words(int w) { return 3 + w; }
+// widest live range (khi - klo + 1 over bins with a non-zero weight) of the
+// table entries [0, n) into *d_span (atomic max; *d_span zeroed by the caller)
+void launch_tof_live_span(const BpEntry* d_entries, const float* d_entry_d, int64_t n, const TofConsts& tof,
+                          int* d_span, hipStream_t stream);
+// forward records of the families' rows (rows must not overlap); nt, w: the
+// forward kernel's accumulator count and window width; the run lengths are
+// filled in by the second launch
+void launch_tof_fp_records(const Family* d_fams, int nfam, const FamilyLor* d_lors, const int2* d_pool,
+                           const ImageDims& dims, const TofConsts& tof, int nt, int w, int32_t* d_rec,
+                           hipStream_t stream);
+void launch_tof_bp_records(const BpEntry* d_entries, const float* d_entry_d, int64_t n, const TofConsts& tof, int w,
+                           int32_t* d_rec, hipStream_t stream);
+void launch_forward_project_tof_rows(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams,
+                                     const int32_t* d_rec, int nt, int w, const float* d_img, float* d_out,
+                                     const ImageDims& dims, const TofConsts& tof, const FpData& data,
+                                     hipStream_t stream);
+void launch_back_project_tof_rows(BpMode mode, int ppa, const int64_t* d_list_off, const int32_t* d_rec, int w,
+                                  int subset_begin, int subset_end, const float* d_proj, float* d_img,
+                                  const ImageDims& dims, const TofConsts& tof, const BpData& data,
+                                  const BpTuning& tune, hipStream_t stream);
+
 // ---- grouped kernels for many problems in one launch (pmrt_batch.hip)
 // One problem of a batch as the grouped kernels see it (device pointers into
 // that problem's own buffers).

@@ -204,6 +204,15 @@ __device__ inline __amdgpu_buffer_rsrc_t run_buffer(cptr<float> base, unsigned f
 __device__ inline float load_run(__amdgpu_buffer_rsrc_t run, int byte_off) {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(run, byte_off, 0, 0));
 }
+// A lane byte offset kept whole for load_run. Offsets of a lane's z slots
+// differ by constants (256 bytes a slot), and the compiler would otherwise
+// load slot q + 1 as slot q's register offset plus an immediate offset; the
+// range check does not wrap that sum, so where slot q's offset is negative
+// (a plane below the run) an in-range slot q + 1 would read 0.
+__device__ inline int whole_offset(int off) {
+  asm("" : "+v"(off));
+  return off;
+}
 #endif
 
 // i clamped to [0, n - 1]

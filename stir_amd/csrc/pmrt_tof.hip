@@ -421,6 +421,327 @@ __global__ void gather_floats_kernel(const float* __restrict__ src, const uint32
     dst[i] = src[idx[i]];
 }
 
+// ---- stored TOF rows
+
+// the first and last TOF bin with a non-zero weight (klo > khi when none)
+template <int NT>
+__device__ inline void live_range(const float (&tw)[NT], int& klo, int& khi) {
+  klo = NT;
+  khi = -1;
+#pragma unroll
+  for (int k = 0; k < NT; ++k)
+    if (tw[k] != 0.F) {
+      klo = min(klo, k);
+      khi = k;
+    }
+}
+
+__global__ void tof_live_span_kernel(const BpEntry* __restrict__ ent, const float* __restrict__ ent_d, int64_t n,
+                                     TofConsts T, int* __restrict__ span) {
+  int best = 0;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    float tw[kMaxTofBins];
+    tof_weights<kMaxTofBins>(ent[i].w, ent_d[i], T, tw);
+    int klo, khi;
+    live_range(tw, klo, khi);
+    best = max(best, khi - klo + 1);
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off, kWave));
+  if (threadIdx.x % kWave == 0) atomicMax(span, best);
+}
+
+// one wave per family, lanes over its row; the same weights as fp_tof_task
+__global__ void __launch_bounds__(kWave) tof_fp_records_kernel(const Family* __restrict__ fams, int nfam,
+                                                               const FamilyLor* __restrict__ lors,
+                                                               const int2* __restrict__ pool, ImageDims d,
+                                                               TofConsts T, int nt, int w, int32_t* __restrict__ rec) {
+  const int fi = blockIdx.x;
+  if (fi >= nfam) return;
+  const Family f = fams[fi];
+  const FamilyLor lor = lors[fi];
+  const int words = tof_fp_words(w);
+  for (int e = threadIdx.x; e < f.row_len; e += kWave) {
+    const int2 el = pool[f.row_start + e];
+    int x, y, z;
+    decode(el.x, x, y, z);
+    float tw[kMaxTofBins];
+    tof_weights<kMaxTofBins>(__int_as_float(el.y), tof_distance(lor, z, y, x, T), T, tw);
+    int klo, khi;
+    live_range(tw, klo, khi);
+    const int c = khi < klo ? 0 : min(klo, nt - w);
+    int32_t* r = rec + (static_cast<int64_t>(f.row_start) + e) * words;
+    r[0] = ((y - d.min_y) * d.nx + (x - d.min_x)) | (c << 20);
+    r[1] = static_cast<int32_t>(static_cast<uint32_t>(f.zsign * z) << 16);
+    for (int j = 0; j < w; ++j) r[2 + j] = __float_as_int(c + j < kMaxTofBins ? tw[c + j] : 0.F);
+  }
+}
+
+// run lengths of equal window starts, from the end of each row (one thread
+// per family)
+__global__ void tof_fp_runs_kernel(const Family* __restrict__ fams, int nfam, int w, int32_t* __restrict__ rec) {
+  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fi >= nfam) return;
+  const Family f = fams[fi];
+  const int words = tof_fp_words(w);
+  int32_t* r = rec + static_cast<int64_t>(f.row_start) * words;
+  int run = 0, next_c = -1;
+  for (int e = f.row_len - 1; e >= 0; --e) {
+    const int c = r[static_cast<int64_t>(e) * words] >> 20;
+    run = (c == next_c && run < 0xffff) ? run + 1 : 1;
+    next_c = c;
+    int32_t& h = r[static_cast<int64_t>(e) * words + 1];
+    h = (h & static_cast<int32_t>(0xffff0000u)) | run;
+  }
+}
+
+__global__ void tof_bp_records_kernel(const BpEntry* __restrict__ ent, const float* __restrict__ ent_d, int64_t n,
+                                      TofConsts T, int w, int32_t* __restrict__ rec) {
+  const int words = tof_bp_words(w);
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const BpEntry E = ent[i];
+    float tw[kMaxTofBins];
+    tof_weights<kMaxTofBins>(E.w, ent_d[i], T, tw);
+    int klo, khi;
+    live_range(tw, klo, khi);
+    const int nk = khi < klo ? 0 : khi - klo + 1;
+    if (nk == 0) klo = 0;
+    int32_t* r = rec + i * words;
+    r[0] = E.sino_base;
+    r[1] = E.k;
+    r[2] = (E.nax_ppa & 0xfffff) | (klo << 20) | (nk << 25);
+    for (int j = 0; j < w; ++j) r[3 + j] = __float_as_int(j < nk ? tw[klo + j] : 0.F);
+  }
+}
+
+// a * b, except that a zero operand gives +0 whatever the other one is
+// (v_mul_legacy_f32)
+__device__ float mul_legacy(float a, float b) __asm("llvm.amdgcn.fmul.legacy");
+
+template <int N>
+struct alignas(16) Words {
+  int32_t v[N];
+};
+
+// acc[q][C + j] += v[q] * w[j] for the W bins of a window starting at C; the
+// legacy multiply gives +0 for a zero weight whatever the image value (an
+// element outside the reference's TOF row adds nothing, as it is skipped
+// there), and x + (+0) == x for every accumulator value other than -0, which
+// a sum started at +0 never takes
+template <int C, int NT, int W, int NSLOT>
+__device__ inline void window_add(float (&acc)[NSLOT][NT], const float (&v)[NSLOT], const int32_t* wb) {
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const float wj = __int_as_float(wb[j]);
+#pragma unroll
+    for (int q = 0; q < NSLOT; ++q) acc[q][C + j] = acc[q][C + j] + mul_legacy(v[q], wj);
+  }
+}
+
+constexpr int kRowGroup = 4;  // elements whose image gathers are issued together
+
+// the elements [e, stop) of a run whose window starts at bin C
+template <int C, int NT, int W, int NSLOT>
+__device__ inline void fp_tof_run(cptr<int32_t> row, int e, int stop, const float* __restrict__ img,
+                                  const ImageDims& d, const int (&zb4)[NSLOT], float (&acc)[NSLOT][NT]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  constexpr int R = tof_fp_words(W);
+  const cptr<Words<R>> rec = reinterpret_cast<cptr<Words<R>>>(row);
+  // an image voxel outside the column's nz planes (or a negative plane)
+  // reads 0 from the range-checked column buffer and adds +0
+  for (; e + kRowGroup <= stop; e += kRowGroup) {
+    Words<R> E[kRowGroup];
+    float v[kRowGroup][NSLOT];
+#pragma unroll
+    for (int u = 0; u < kRowGroup; ++u) {
+      E[u] = rec[e + u];
+      const __amdgpu_buffer_rsrc_t column =
+          run_buffer(as_const(img), static_cast<unsigned>(__mul24(E[u].v[0] & 0xfffff, d.nzp)), d.nz);
+      const int z4 = (E[u].v[1] >> 16) * 4;
+#pragma unroll
+      for (int q = 0; q < NSLOT; ++q) v[u][q] = load_run(column, z4 + zb4[q]);
+    }
+#pragma unroll
+    for (int u = 0; u < kRowGroup; ++u) window_add<C, NT, W, NSLOT>(acc, v[u], E[u].v + 2);
+  }
+  for (; e < stop; ++e) {
+    const Words<R> E = rec[e];
+    const __amdgpu_buffer_rsrc_t column =
+        run_buffer(as_const(img), static_cast<unsigned>(__mul24(E.v[0] & 0xfffff, d.nzp)), d.nz);
+    const int z4 = (E.v[1] >> 16) * 4;
+    float v[NSLOT];
+#pragma unroll
+    for (int q = 0; q < NSLOT; ++q) v[q] = load_run(column, z4 + zb4[q]);
+    window_add<C, NT, W, NSLOT>(acc, v, E.v + 2);
+  }
+#endif
+}
+
+// One family per wave over stored records: lanes over axial positions, each
+// element's image column gathered once and added to the accumulators of its
+// window, in row order (the reference's per-bin sum order). The window start
+// is dispatched once per run of elements that share it.
+template <int MODE, int NT, int W, int NSLOT>
+__device__ void fp_tof_rows_task(const Family& f, const int32_t* __restrict__ rec, const float* __restrict__ img,
+                                 float* __restrict__ out, const ImageDims& d, const TofConsts& T, const FpData& data,
+                                 int t, int lane) {
+  constexpr int R = tof_fp_words(W);
+  static_assert(NT - W < 16, "window start beyond the dispatch");
+  int zb4[NSLOT];
+  bool valid[NSLOT];
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) {
+    const int a = lane + kWave * q;
+    valid[q] = a < f.nax;
+    zb4[q] = 4 * (f.zadd0 + f.ppa * a - d.min_z);
+  }
+  float acc[NSLOT][NT];
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q)
+#pragma unroll
+    for (int k = 0; k < NT; ++k) acc[q][k] = 0.F;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const cptr<int32_t> row = as_const(rec) + static_cast<int64_t>(f.row_start) * R;
+  const int n = f.row_len;
+  for (int e = 0; e < n;) {
+    const int c = row[static_cast<int64_t>(e) * R] >> 20;
+    const int stop = e + (row[static_cast<int64_t>(e) * R + 1] & 0xffff);
+    switch (c) {
+#define MI_TW(C)                                                                               \
+  case C:                                                                                      \
+    if constexpr (C <= NT - W) fp_tof_run<C, NT, W, NSLOT>(row, e, stop, img, d, zb4, acc); \
+    break;
+      MI_TW(0) MI_TW(1) MI_TW(2) MI_TW(3) MI_TW(4) MI_TW(5) MI_TW(6) MI_TW(7)
+      MI_TW(8) MI_TW(9) MI_TW(10) MI_TW(11) MI_TW(12) MI_TW(13) MI_TW(14) MI_TW(15)
+#undef MI_TW
+      default: break;
+    }
+    e = stop;
+  }
+#endif
+  double ll = 0.0;
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) {
+    if (!valid[q]) continue;
+    const int64_t i0 = static_cast<int64_t>(f.sino_base) + lane + kWave * q;
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+      if (k < T.ntof)
+        fp_epilogue<MODE>(acc[q][k], static_cast<int64_t>(k) * T.block + i0, k * T.nvg + f.viewgram, data, out, ll);
+  }
+  if constexpr (fp_accumulates_loglik(MODE)) {
+    for (int off = kWave / 2; off > 0; off >>= 1) ll += __shfl_xor(ll, off, kWave);
+    if (lane == 0) data.partial[t] = ll;
+  }
+}
+
+template <int MODE, int NT, int W>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_tof_rows_kernel(const FpTask* __restrict__ tasks,
+                                                                            int ntasks,
+                                                                            const Family* __restrict__ fams,
+                                                                            const int32_t* __restrict__ rec,
+                                                                            const float* __restrict__ img,
+                                                                            float* __restrict__ out, ImageDims d,
+                                                                            TofConsts T, FpData data) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int t = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (t >= ntasks) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  const Family f = fams[tasks[t].first];
+  if (f.nax <= kWave) fp_tof_rows_task<MODE, NT, W, 1>(f, rec, img, out, d, T, data, t, lane);
+  else fp_tof_rows_task<MODE, NT, W, 2>(f, rec, img, out, d, T, data, t, lane);
+}
+
+constexpr int kEntryGroup = 2;  // table entries whose projection-data gathers are issued together
+
+// Voxel-driven back projection over stored records: one wave per image
+// column, as bp_tof_kernel. Each entry gathers the runs of its W window bins
+// at once; bins past its live range get an empty buffer (no read, value 0)
+// and weight 0, and the legacy multiply makes a zero weight add +0 whatever
+// the projection value.
+template <int MODE, int PPA, int NSLOT, int W>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_rows_kernel(const int64_t* __restrict__ list_off,
+                                                                            const int32_t* __restrict__ rec, int sb,
+                                                                            int se, const float* __restrict__ proj,
+                                                                            float* __restrict__ img, ImageDims d,
+                                                                            int nwork, int2 tile, int remap,
+                                                                            int64_t block, BpData data) {
+  constexpr int R = tof_bp_words(W);
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int w = (remap ? xcd_remap(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kWavesPerBlock + wave;
+  if (w >= nwork) return;
+  const int col = tiled_column(w, d, tile);
+  if (col < 0) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  float acc[NSLOT];
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) acc[q] = 0.F;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const cptr<Words<R>> table = reinterpret_cast<cptr<Words<R>>>(as_const(rec));
+  for (int s = sb; s < se; ++s) {
+    const int64_t beg = list_off[static_cast<int64_t>(s) * d.ncols + col];
+    const int64_t end = list_off[static_cast<int64_t>(s) * d.ncols + col + 1];
+    int64_t e = beg;
+    for (; e < end; e += kEntryGroup) {
+      float p[kEntryGroup][W][NSLOT];
+      float wt[kEntryGroup][W];
+#pragma unroll
+      for (int u = 0; u < kEntryGroup; ++u) {
+        // a repeated last entry adds nothing: its weights are taken as 0
+        const bool real = e + u < end;
+        const Words<R> E = table[real ? e + u : e];
+        const int base = E.v[0], k = E.v[1], np = E.v[2] & 0xfffff;
+        const int klo = (E.v[2] >> 20) & 31, nk = real ? static_cast<int>(static_cast<unsigned>(E.v[2]) >> 25) : 0;
+        const int nax = np & 0xffff;
+        int off[NSLOT];
+#pragma unroll
+        for (int q = 0; q < NSLOT; ++q) {
+          const int zi = d.min_z + lane + kWave * q;
+          if constexpr (PPA == 1) {
+            off[q] = whole_offset(4 * (zi - k));  // outside [0, nax) the run buffer reads 0
+          } else {
+            bool ok;
+            const unsigned a = entry_axial<PPA>(k, np, zi, ok);
+            off[q] = ok ? static_cast<int>(a << 2) : -4;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          wt[u][j] = j < nk ? __int_as_float(E.v[3 + j]) : 0.F;
+          const __amdgpu_buffer_rsrc_t run = run_buffer(
+              as_const(proj), static_cast<unsigned>(static_cast<int64_t>(klo + j) * block + base), j < nk ? nax : 0);
+#pragma unroll
+          for (int q = 0; q < NSLOT; ++q) p[u][j][q] = load_run(run, off[q]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kEntryGroup; ++u)
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+#pragma unroll
+          for (int q = 0; q < NSLOT; ++q) acc[q] = acc[q] + mul_legacy(wt[u][j], p[u][j][q]);
+    }
+  }
+#endif
+  float* outc = img + static_cast<int64_t>(col) * d.nzp;
+#pragma unroll
+  for (int q = 0; q < NSLOT; ++q) {
+    const int z = lane + kWave * q;
+    if (z >= d.nzp) continue;
+    if constexpr (MODE == BP_PLAIN) {
+      outc[z] = z < d.nz ? acc[q] : 0.F;
+    } else {
+      if (z < d.nz) {
+        const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
+        const float u = osem_factor(acc[q], data.sens[idx], data.threshold, data.thr_lo, data.thr_hi);
+        data.estimate[idx] = data.estimate[idx] * u;
+        if (data.update) data.update[idx] = u;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 void launch_forward_project_tof(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams,
@@ -501,6 +822,98 @@ void launch_gather_floats(const float* d_src, const uint32_t* d_idx, float* d_ds
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
   hipLaunchKernelGGL(gather_floats_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, d_src, d_idx,
                      d_dst, n);
+}
+
+namespace {
+unsigned elementwise_blocks(int64_t n) { return static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 65536)); }
+}  // namespace
+
+void launch_tof_live_span(const BpEntry* d_entries, const float* d_entry_d, int64_t n, const TofConsts& tof,
+                          int* d_span, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(tof_live_span_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_entries, d_entry_d, n,
+                     tof, d_span);
+}
+
+void launch_tof_fp_records(const Family* d_fams, int nfam, const FamilyLor* d_lors, const int2* d_pool,
+                           const ImageDims& dims, const TofConsts& tof, int nt, int w, int32_t* d_rec,
+                           hipStream_t stream) {
+  if (nfam <= 0) return;
+  hipLaunchKernelGGL(tof_fp_records_kernel, dim3(nfam), dim3(kWave), 0, stream, d_fams, nfam, d_lors, d_pool, dims, tof,
+                     nt, w, d_rec);
+  hipLaunchKernelGGL(tof_fp_runs_kernel, dim3((nfam + 255) / 256), dim3(256), 0, stream, d_fams, nfam, w, d_rec);
+}
+
+void launch_tof_bp_records(const BpEntry* d_entries, const float* d_entry_d, int64_t n, const TofConsts& tof, int w,
+                           int32_t* d_rec, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(tof_bp_records_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, stream, d_entries, d_entry_d, n,
+                     tof, w, d_rec);
+}
+
+void launch_forward_project_tof_rows(FpMode mode, const FpTask* d_tasks, int ntasks, const Family* d_fams,
+                                     const int32_t* d_rec, int nt, int w, const float* d_img, float* d_out,
+                                     const ImageDims& dims, const TofConsts& tof, const FpData& data,
+                                     hipStream_t stream) {
+  if (ntasks <= 0) return;
+  const dim3 grid((ntasks + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+#define MI_FPR(M, NT, W)                                                                                              \
+  hipLaunchKernelGGL((fp_tof_rows_kernel<M, NT, W>), grid, block, 0, stream, d_tasks, ntasks, d_fams, d_rec, d_img, \
+                     d_out, dims, tof, data)
+#define MI_FPR_CASE(M)                                            \
+  case M:                                                         \
+    if (nt == 12 && w == kTofFpNarrow) MI_FPR(M, 12, kTofFpNarrow); \
+    else MI_FPR(M, 16, kTofFpWide);                               \
+    break;
+  switch (mode) {
+    MI_FPR_CASE(FP_PLAIN)
+    MI_FPR_CASE(FP_RATIO)
+    MI_FPR_CASE(FP_GRADIENT)
+    MI_FPR_CASE(FP_LOGLIK)
+    MI_FPR_CASE(FP_GRADIENT_LOGLIK)
+    MI_FPR_CASE(FP_RATIO_LOGLIK)
+    MI_FPR_CASE(FP_DRATIO)
+    MI_FPR_CASE(FP_SCALE)
+    MI_FPR_CASE(FP_LM_RATIO)
+    MI_FPR_CASE(FP_LM_VALUE)
+  }
+#undef MI_FPR_CASE
+#undef MI_FPR
+}
+
+void launch_back_project_tof_rows(BpMode mode, int ppa, const int64_t* d_list_off, const int32_t* d_rec, int w,
+                                  int subset_begin, int subset_end, const float* d_proj, float* d_img,
+                                  const ImageDims& dims, const TofConsts& tof, const BpData& data,
+                                  const BpTuning& tune, hipStream_t stream) {
+  const int nslot = (dims.nzp + kWave - 1) / kWave;
+  const int2 tile = make_int2(tune.tile_x > 0 ? tune.tile_x : 1, tune.tile_y > 0 ? tune.tile_y : 1);
+  const int nwork = ((dims.nx + tile.x - 1) / tile.x) * ((dims.ny + tile.y - 1) / tile.y) * tile.x * tile.y;
+  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+#define MI_BPR(M, P, S, W)                                                                                     \
+  hipLaunchKernelGGL((bp_tof_rows_kernel<M, P, S, W>), grid, block, 0, stream, d_list_off, d_rec, subset_begin, \
+                     subset_end, d_proj, d_img, dims, nwork, tile, tune.xcd_remap, tof.block, data)
+#define MI_BPR_W(M, P, S)                         \
+  if (w == kTofBpNarrow) MI_BPR(M, P, S, kTofBpNarrow); \
+  else MI_BPR(M, P, S, kTofBpWide);
+#define MI_BPR_SLOTS(M, P)            \
+  switch (nslot) {                    \
+    case 1: MI_BPR_W(M, P, 1) break;  \
+    case 2: MI_BPR_W(M, P, 2) break;  \
+    case 3: MI_BPR_W(M, P, 3) break;  \
+    default: MI_BPR_W(M, P, 4) break; \
+  }
+  if (mode == BP_PLAIN) {
+    if (ppa == 1) { MI_BPR_SLOTS(BP_PLAIN, 1) }
+    else if (ppa == 2) { MI_BPR_SLOTS(BP_PLAIN, 2) }
+    else { MI_BPR_SLOTS(BP_PLAIN, 0) }
+  } else {
+    if (ppa == 1) { MI_BPR_SLOTS(BP_OSEM_UPDATE, 1) }
+    else if (ppa == 2) { MI_BPR_SLOTS(BP_OSEM_UPDATE, 2) }
+    else { MI_BPR_SLOTS(BP_OSEM_UPDATE, 0) }
+  }
+#undef MI_BPR_SLOTS
+#undef MI_BPR_W
+#undef MI_BPR
 }
 
 }  // namespace mi

@@ -88,22 +88,26 @@ def test_osem(case):
     assert rel <= 1e-4
 
 
-def test_tof_span3():
-    """Time of flight on a one-plane table (the 8-ring scanner with 9 TOF bins
-    of 89.04 mm, 400 ps FWHM): forward bit-exact, back projection within 1e-5
-    of the FP64 oracle's maximum, OSEM with TOF sensitivities within 1e-4."""
+@pytest.mark.parametrize("mrd,ppa", [(4, 1), (5, 0)])
+def test_tof_span3(mrd, ppa):
+    """Time of flight on a one-plane table and on a mixed one (the 8-ring
+    scanner with 9 TOF bins of 89.04 mm, 400 ps FWHM, MRD 4 and 5), with the
+    stored TOF rows: forward bit-exact, back projection within 1e-5 of the
+    FP64 oracle's maximum, OSEM with TOF sensitivities within 1e-4."""
     from dataclasses import replace
     from stir_amd import phantoms
     sc = replace(Scanner.cylindrical(8, 192, 300.0, 4.0, name="cyl8x192_tof"), timing_resolution_ps=400.0,
                  size_of_timing_pos_ps=594.0, max_num_timing_poss=9)
-    pdi = ProjDataInfoCylindrical.construct(sc, span=3, max_delta=4, num_views=96, num_tangential_poss=128).with_tof(1)
+    pdi = ProjDataInfoCylindrical.construct(sc, span=3, max_delta=mrd, num_views=96, num_tangential_poss=128).with_tof(1)
     img = VoxelsOnCartesianGrid(15, 128, 128, 2.0, 2.0, 2.0)
     pm = RayTracingMatrixParameters()
-    print(f"[span3 tof] {pdi.num_bins} bins", flush=True)
+    print(f"[span3 tof] MRD {mrd}: {pdi.num_bins} bins", flush=True)
     P = ProjectorByBinPairMI355X(ProjMatrixByBinUsingRayTracing(pm))
     P.set_up(pdi, img, num_subsets=4)
     try:
-        assert P.info().planes_per_axial_position == 1
+        info = P.info()
+        assert info.planes_per_axial_position == ppa
+        assert (info.tof_fp_window, info.tof_bp_window) == (6, 5)
         O = OraclePlan(pdi, img, pm)
         rng = np.random.default_rng(13)
         x = rng.uniform(0, 1, img.shape).astype(np.float32)

@@ -8,13 +8,14 @@ float order; bars 1e-5 relative); back projection within 1e-5 of the
 maximum; OSEM images within 1e-4 relative; objective value within 1e-6 and
 sub-gradient within 1e-5 of the maximum. Property checks: the sum over TOF
 bins of the TOF forward projection equals the non-TOF projection (1e-5),
-and the adjoint identity holds with FP64 sums.
+and the adjoint identity holds with FP64 sums. Every test runs with the TOF
+weights stored at set-up (narrow and wide windows) and computed on the fly.
 """
 import numpy as np
 import pytest
 import torch
 
-from stir_amd import phantoms
+from stir_amd import _native, phantoms
 from stir_amd.autograd import PoissonLogLikelihoodLoss
 from stir_amd.geometry import RayTracingMatrixParameters, config0, config0_tof
 from stir_amd.projectors import (OSMAPOSLReconstruction, PoissonLogLikelihoodWithLinearModelForMeanAndProjData,
@@ -25,13 +26,26 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def tof():
+def tof_oracle():
     pdi, img = config0_tof()
     pm = RayTracingMatrixParameters()
-    print(f"[tof c0] set-up: {pdi.num_bins} bins ({pdi.num_tof_bins} TOF bins), image {img.shape}", flush=True)
+    return pdi, img, pm, OraclePlan(pdi, img, pm)
+
+
+# TOF weights: stored records with the narrowest windows (the default), with
+# wide windows, and computed on the fly in every projection
+ROW_MODES = {"stored": 1, "stored_wide": 2, "on_the_fly": 0}
+
+
+@pytest.fixture(scope="module", params=sorted(ROW_MODES))
+def tof(request, tof_oracle):
+    pdi, img, pm, O = tof_oracle
+    print(f"[tof c0 {request.param}] set-up: {pdi.num_bins} bins ({pdi.num_tof_bins} TOF bins), image {img.shape}",
+          flush=True)
     P = ProjectorByBinPairMI355X(ProjMatrixByBinUsingRayTracing(pm))
+    P.context.set_tuning(_native.MI_TUNE_TOF_ROWS, ROW_MODES[request.param])
     P.set_up(pdi, img, num_subsets=4)
-    O = OraclePlan(pdi, img, pm)
+    P.row_mode = request.param
     yield pdi, img, P, O
     P.close()
 
@@ -41,6 +55,12 @@ def test_tof_setup(tof):
     info = P.info()
     assert info.num_bins == pdi.num_bins == O.num_bins
     assert tuple(info.symmetry_flags) == O.symmetry_flags() == (0, 0, 0, 0, 1)
+    # 400 ps FWHM (sigma 25.5 mm) against 89 mm bins: the 4-sigma-sqrt(2)
+    # support (288 mm) meets at most 5 bins (measured when the rows are stored)
+    if P.row_mode != "on_the_fly":
+        assert 3 <= info.tof_live_span <= 5
+    windows = {"stored": (6, 5), "stored_wide": (14, 13), "on_the_fly": (0, 0)}[P.row_mode]
+    assert (info.tof_fp_window, info.tof_bp_window) == windows
 
 
 def test_tof_forward_projection_bit_exact(tof):

@@ -46,6 +46,10 @@ def test_tof_mmr_setup(tof_mmr):
     assert pdi.num_tof_bins == 11
     assert P.info().num_bins == pdi.num_bins == O.num_bins == 11 * 72557856
     assert tuple(P.info().symmetry_flags) == O.symmetry_flags() == (0, 0, 0, 0, 1)
+    # the projections run from the stored TOF rows (narrow windows)
+    info = P.info()
+    print(f"[tof mmr] live span {info.tof_live_span}, windows {info.tof_fp_window}/{info.tof_bp_window}", flush=True)
+    assert (info.tof_fp_window, info.tof_bp_window) == (6, 5)
 
 
 def test_tof_mmr_subset_projections_match_oracle(tof_mmr):
