@@ -666,7 +666,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_rows_kernel(cons
                                                                             int se, const float* __restrict__ proj,
                                                                             float* __restrict__ img, ImageDims d,
                                                                             int nwork, int2 tile, int remap,
-                                                                            int64_t block, BpData data) {
+                                                                            int64_t block, int accumulate,
+                                                                            BpData data) {
   constexpr int R = tof_bp_words(W);
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
   const int w = (remap ? xcd_remap(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kWavesPerBlock + wave;
@@ -730,7 +731,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_rows_kernel(cons
     const int z = lane + kWave * q;
     if (z >= d.nzp) continue;
     if constexpr (MODE == BP_PLAIN) {
-      outc[z] = z < d.nz ? acc[q] : 0.F;
+      outc[z] = z < d.nz ? (accumulate ? outc[z] + acc[q] : acc[q]) : 0.F;
     } else {
       if (z < d.nz) {
         const int64_t idx = static_cast<int64_t>(col) * d.nzp + z;
@@ -889,9 +890,16 @@ void launch_back_project_tof_rows(BpMode mode, int ppa, const int64_t* d_list_of
   const int2 tile = make_int2(tune.tile_x > 0 ? tune.tile_x : 1, tune.tile_y > 0 ? tune.tile_y : 1);
   const int nwork = ((dims.nx + tile.x - 1) / tile.x) * ((dims.ny + tile.y - 1) / tile.y) * tile.x * tile.y;
   const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
-#define MI_BPR(M, P, S, W)                                                                                     \
-  hipLaunchKernelGGL((bp_tof_rows_kernel<M, P, S, W>), grid, block, 0, stream, d_list_off, d_rec, subset_begin, \
-                     subset_end, d_proj, d_img, dims, nwork, tile, tune.xcd_remap, tof.block, data)
+  // a plain projection of several subsets runs one launch per subset, each
+  // adding to the image: the waves in flight then share one subset's
+  // projection data in L2 (11 TOF blocks make the whole data set many times
+  // the L2 and MALL)
+  const bool split = mode == BP_PLAIN && subset_end - subset_begin > 1;
+  for (int s = subset_begin; s < subset_end; ++s) {
+    const int sb = split ? s : subset_begin, se = split ? s + 1 : subset_end, accumulate = split && s > subset_begin;
+#define MI_BPR(M, P, S, W)                                                                                    \
+  hipLaunchKernelGGL((bp_tof_rows_kernel<M, P, S, W>), grid, block, 0, stream, d_list_off, d_rec, sb, se, d_proj, \
+                     d_img, dims, nwork, tile, tune.xcd_remap, tof.block, accumulate, data)
 #define MI_BPR_W(M, P, S)                         \
   if (w == kTofBpNarrow) MI_BPR(M, P, S, kTofBpNarrow); \
   else MI_BPR(M, P, S, kTofBpWide);
@@ -910,6 +918,8 @@ void launch_back_project_tof_rows(BpMode mode, int ppa, const int64_t* d_list_of
     if (ppa == 1) { MI_BPR_SLOTS(BP_OSEM_UPDATE, 1) }
     else if (ppa == 2) { MI_BPR_SLOTS(BP_OSEM_UPDATE, 2) }
     else { MI_BPR_SLOTS(BP_OSEM_UPDATE, 0) }
+  }
+    if (!split) break;
   }
 #undef MI_BPR_SLOTS
 #undef MI_BPR_W
