@@ -670,7 +670,10 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_rows_kernel(cons
                                                                             BpData data) {
   constexpr int R = tof_bp_words(W);
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int w = (remap ? xcd_remap(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kWavesPerBlock + wave;
+  const int b = remap == 1 ? xcd_remap(blockIdx.x, gridDim.x)
+                : remap > 1  ? xcd_chunk_remap(blockIdx.x, remap)
+                             : static_cast<int>(blockIdx.x);
+  const int w = b * kWavesPerBlock + wave;
   if (w >= nwork) return;
   const int col = tiled_column(w, d, tile);
   if (col < 0) return;
@@ -889,7 +892,12 @@ void launch_back_project_tof_rows(BpMode mode, int ppa, const int64_t* d_list_of
   const int nslot = (dims.nzp + kWave - 1) / kWave;
   const int2 tile = make_int2(tune.tile_x > 0 ? tune.tile_x : 1, tune.tile_y > 0 ? tune.tile_y : 1);
   const int nwork = ((dims.nx + tile.x - 1) / tile.x) * ((dims.ny + tile.y - 1) / tile.y) * tile.x * tile.y;
-  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  int nblocks = (nwork + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (tune.xcd_remap > 1) {  // chunked remap: whole rounds of 8 chunks
+    const int round = 8 * tune.xcd_remap;
+    nblocks = (nblocks + round - 1) / round * round;
+  }
+  const dim3 grid(nblocks), block(kWave * kWavesPerBlock);
   // a plain projection of several subsets runs one launch per subset, each
   // adding to the image: the waves in flight then share one subset's
   // projection data in L2 (11 TOF blocks make the whole data set many times
