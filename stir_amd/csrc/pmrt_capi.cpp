@@ -334,7 +334,7 @@ struct mi_ctx {
                                    st);
     else
       mi::launch_back_project(mode, uniform_ppa, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj,
-                              d_img, dims, data, bp_tuning, st);
+                              d_img, dims, data, bp_tuning, st, block_bins);
     if (timing) { HIPCHECK(hipEventRecord(e1, st)); bp_events.emplace_back(e0, e1); }
   }
 

@@ -125,9 +125,11 @@ struct BpTuning {
 
 // ppa: planes per axial position shared by all families (1 or 2 select a
 // specialised kernel), 0 when it varies between segments.
+// proj_values: the number of floats at d_proj (0 if unknown), used to pick
+// 32-bit run-buffer arithmetic when they lie inside one 4 GB-aligned window
 void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const BpEntry* d_entries, int subset_begin,
                          int subset_end, const float* d_proj, float* d_img, const ImageDims& dims, const BpData& data,
-                         const BpTuning& tune, hipStream_t stream);
+                         const BpTuning& tune, hipStream_t stream, int64_t proj_values = 0);
 // 2-D transpose used for the STIR <-> device layout conversions:
 // dst[c * dst_ld + r] = src[r * src_ld + c] for r < rows, c < cols.
 void launch_transpose(const float* src, float* dst, int rows, int cols, int src_ld, int dst_ld, hipStream_t stream);

@@ -201,6 +201,15 @@ __device__ inline float load_byteoff(cptr<float> base, unsigned off) {
 __device__ inline __amdgpu_buffer_rsrc_t run_buffer(cptr<float> base, unsigned first, int n) {
   return __builtin_amdgcn_make_buffer_rsrc((void*)(base + first), static_cast<short>(0), 4 * n, 0x00020000);
 }
+// run_buffer for a buffer that lies inside one 4 GB-aligned window of the
+// address space (checked by the launcher): the run's start is formed with one
+// 32-bit add to the low half of the base address, the high half kept.
+__device__ inline __amdgpu_buffer_rsrc_t run_buffer_lo32(cptr<float> base, unsigned first, int n) {
+  const uint64_t b = reinterpret_cast<uint64_t>(base);
+  const uint32_t lo = static_cast<uint32_t>(b) + (first << 2);
+  const uint64_t p = (b & 0xffffffff00000000ull) | lo;
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(p), static_cast<short>(0), 4 * n, 0x00020000);
+}
 __device__ inline float load_run(__amdgpu_buffer_rsrc_t run, int byte_off) {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(run, byte_off, 0, 0));
 }

@@ -41,8 +41,10 @@ constexpr int kRowSlack = 16;   // elements after the last row copy (read-ahead 
 // elements of the next kUnroll group are requested before the current
 // group's gathers are issued (the copies end in kUnroll elements of slack, so
 // reading a group past the row's end stays inside the allocation; those
-// elements are never used).
-template <int NSLOT>
+// elements are never used). LO32: the image lies inside one 4 GB-aligned
+// window, so a column's buffer start is one 32-bit add (4 scalar instructions
+// per element instead of 7; the CU's one scalar unit is shared by its waves).
+template <int NSLOT, bool LO32>
 __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& L, cptr<float> img, const ImageDims& d,
                                   float (&acc)[NSLOT]) {
   // elements per group: 16 gathers in flight whatever the slot count, so
@@ -73,7 +75,8 @@ __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& 
       for (int j = 0; j < U; ++j) {
         const unsigned col = static_cast<unsigned>(cur[j].x) >> 12;
         const int zu = cur[j].x & 0xfff;
-        const __amdgpu_buffer_rsrc_t column = run_buffer(img, col * static_cast<unsigned>(d.nzp), d.nz);
+        const __amdgpu_buffer_rsrc_t column = LO32 ? run_buffer_lo32(img, col * static_cast<unsigned>(d.nzp), d.nz)
+                                                   : run_buffer(img, col * static_cast<unsigned>(d.nzp), d.nz);
         w[j] = __int_as_float(cur[j].y);
 #pragma unroll
         for (int q = 0; q < NSLOT; ++q) v[j][q] = load_run(column, __mul24(zs4[q], zu) + zb4[q]);
@@ -101,7 +104,7 @@ __device__ inline void fp_row_ops(cptr<int2> row, int n, const LaneBins<NSLOT>& 
   }
 }
 
-template <int MODE, int NSLOT>
+template <int MODE, int NSLOT, bool LO32 = false>
 __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, cptr<int2> pool, cptr<float> img,
                                float* __restrict__ out, const ImageDims& d, const FpData& data, int t, int lane,
                                cptr<int2> ops, int64_t nelem) {
@@ -121,7 +124,7 @@ __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, cp
     (void)f;
   }
   float acc[NSLOT];
-  fp_row_ops<NSLOT>(ops + static_cast<int64_t>(f1.inplane) * nelem + f1.row_start, f1.row_len, L, img, d, acc);
+  fp_row_ops<NSLOT, LO32>(ops + static_cast<int64_t>(f1.inplane) * nelem + f1.row_start, f1.row_len, L, img, d, acc);
   double ll = 0.0;
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q)
@@ -134,7 +137,7 @@ __device__ inline void fp_task(const Family& f1, const Family& f2, bool pair, cp
 
 // One wave per task; kWavesPerBlock tasks per workgroup. The number of axial
 // slots per lane (1..4) follows the task's total number of bins.
-template <int MODE>
+template <int MODE, bool LO32>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask* __restrict__ tasks, int ntasks,
                                                                    const Family* __restrict__ fams,
                                                                    const int2* __restrict__ pool,
@@ -154,10 +157,10 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask*
   const int nslot = (total + kWave - 1) / kWave;
   cptr<int2> rows = ops ? as_const(ops) : nullptr;
   switch (nslot) {
-    case 1: fp_task<MODE, 1>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
-    case 2: fp_task<MODE, 2>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
-    case 3: fp_task<MODE, 3>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
-    default: fp_task<MODE, 4>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
+    case 1: fp_task<MODE, 1, LO32>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
+    case 2: fp_task<MODE, 2, LO32>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
+    case 3: fp_task<MODE, 3, LO32>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
+    default: fp_task<MODE, 4, LO32>(f1, f2, pair, as_const(pool), as_const(img), out, d, data, t, lane, rows, nelem); break;
   }
 }
 
@@ -167,7 +170,9 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask*
 // for all slots; kUnroll entries have their projection-data reads in flight
 // together and are accumulated in table order.
 // Back projection of one image column (wave-uniform col) over subsets [sb, se).
-template <int MODE, int PPA, int NSLOT, bool SKIP>
+// LO32: the projection data lie inside one 4 GB-aligned window (a run's
+// buffer start is one 32-bit add).
+template <int MODE, int PPA, int NSLOT, bool SKIP, bool LO32 = false>
 __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> ent, int sb, int se, cptr<float> proj,
                                  float* __restrict__ img, const ImageDims& d, const BpData& data, int lane) {
   float acc[NSLOT];
@@ -206,7 +211,9 @@ __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> 
         // path for the one-plane entries of mixed tables, behind a per-entry
         // test, made those tables' back projection slower.)
         if constexpr (PPA == 1 && !SKIP) {
-          const __amdgpu_buffer_rsrc_t run = run_buffer(proj, static_cast<unsigned>(E.sino_base), E.nax_ppa & 0xffff);
+          const __amdgpu_buffer_rsrc_t run =
+              LO32 ? run_buffer_lo32(proj, static_cast<unsigned>(E.sino_base), E.nax_ppa & 0xffff)
+                   : run_buffer(proj, static_cast<unsigned>(E.sino_base), E.nax_ppa & 0xffff);
 #pragma unroll
           for (int q = 0; q < NSLOT; ++q) {
             ok[j][q] = true;
@@ -417,7 +424,7 @@ __device__ inline void bp_column_parity(int col, cptr<int64_t> list_off, cptr<Bp
   }
 }
 
-template <int MODE, int PPA, int NSLOT, bool SKIP>
+template <int MODE, int PPA, int NSLOT, bool SKIP, bool LO32>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t* __restrict__ list_off,
                                                                    const BpEntry* __restrict__ ent, int sb, int se,
                                                                    const float* __restrict__ proj,
@@ -437,8 +444,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t
     bp_column_parity<MODE, 32>(col, as_const(list_off), as_const(ent), sb, se, as_const(proj), img, d, data, lane,
                                slots + wave * kWave);
   } else
-    bp_column<MODE, PPA, NSLOT, SKIP>(col, as_const(list_off), as_const(ent), sb, se, as_const(proj), img, d, data,
-                                      lane);
+    bp_column<MODE, PPA, NSLOT, SKIP, LO32>(col, as_const(list_off), as_const(ent), sb, se, as_const(proj), img, d,
+                                            data, lane);
 }
 
 // ---- batches: one work item per wave, the problem's descriptor read
@@ -604,11 +611,19 @@ void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, cons
     nblocks = (nblocks + round - 1) / round * round;
   }
   const dim3 grid(nblocks), block(kWave * kWavesPerBlock);
+  // the image inside one 4 GB-aligned window of the address space
+  const uint64_t first = reinterpret_cast<uint64_t>(d_img);
+  const uint64_t last = first + static_cast<uint64_t>(dims.ncols) * dims.nzp * sizeof(float) - 1;
+  const bool lo32 = (first >> 32) == (last >> 32);
   switch (mode) {
-#define MI_FP_CASE(M)                                                                                            \
-  case M:                                                                                                        \
-    hipLaunchKernelGGL(fp_kernel<M>, grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, dims, \
-                       data, xcd_chunk, d_ops, nelem);                                                           \
+#define MI_FP_CASE(M)                                                                                              \
+  case M:                                                                                                          \
+    if (lo32)                                                                                                      \
+      hipLaunchKernelGGL((fp_kernel<M, true>), grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img, d_out, \
+                         dims, data, xcd_chunk, d_ops, nelem);                                                     \
+    else                                                                                                           \
+      hipLaunchKernelGGL((fp_kernel<M, false>), grid, block, 0, stream, d_tasks, ntasks, d_fams, d_pool, d_img,      \
+                         d_out, dims, data, xcd_chunk, d_ops, nelem);                                              \
     break;
     MI_FP_CASE(FP_PLAIN)
     MI_FP_CASE(FP_RATIO)
@@ -626,7 +641,7 @@ void launch_forward_project(FpMode mode, const FpTask* d_tasks, int ntasks, cons
 
 void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const BpEntry* d_entries, int subset_begin,
                          int subset_end, const float* d_proj, float* d_img, const ImageDims& dims, const BpData& data,
-                         const BpTuning& tune, hipStream_t stream) {
+                         const BpTuning& tune, hipStream_t stream, int64_t proj_values) {
   const int nslot = (dims.nzp + kWave - 1) / kWave;
   const int2 tile = make_int2(tune.tile_x > 0 ? tune.tile_x : 1, tune.tile_y > 0 ? tune.tile_y : 1);
   const int nwork = ((dims.nx + tile.x - 1) / tile.x) * ((dims.ny + tile.y - 1) / tile.y) * tile.x * tile.y;
@@ -636,9 +651,17 @@ void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const 
     nblocks = (nblocks + round - 1) / round * round;
   }
   const dim3 grid(nblocks), block(kWave * kWavesPerBlock);
-#define MI_BP_LAUNCH(M, P, S, K)                                                                                       \
-  hipLaunchKernelGGL((bp_kernel<M, P, S, K>), grid, block, 0, stream, d_list_off, d_entries, subset_begin, subset_end, \
-                     d_proj, d_img, dims, nwork, tile, tune.xcd_remap, data)
+  // the projection data inside one 4 GB-aligned window of the address space
+  // (the one-plane kernel forms its run buffers with 32-bit adds then)
+  const uint64_t first = reinterpret_cast<uint64_t>(d_proj);
+  const bool lo32 = proj_values > 0 && (first >> 32) == ((first + static_cast<uint64_t>(proj_values) * 4 - 1) >> 32);
+#define MI_BP_LAUNCH(M, P, S, K)                                                                                    \
+  if (lo32)                                                                                                         \
+    hipLaunchKernelGGL((bp_kernel<M, P, S, K, true>), grid, block, 0, stream, d_list_off, d_entries, subset_begin, \
+                       subset_end, d_proj, d_img, dims, nwork, tile, tune.xcd_remap, data);                         \
+  else                                                                                                              \
+    hipLaunchKernelGGL((bp_kernel<M, P, S, K, false>), grid, block, 0, stream, d_list_off, d_entries, subset_begin, \
+                       subset_end, d_proj, d_img, dims, nwork, tile, tune.xcd_remap, data)
 #define MI_BP_SKIP(M, P, S)             \
   if (tune.slot_skip) MI_BP_LAUNCH(M, P, S, true); \
   else MI_BP_LAUNCH(M, P, S, false);

@@ -541,8 +541,10 @@ __device__ inline void window_add(float (&acc)[NSLOT][NT], const float (&v)[NSLO
 
 constexpr int kRowGroup = 4;  // elements whose image gathers are issued together
 
-// the elements [e, stop) of a run whose window starts at bin C
-template <int C, int NT, int W, int NSLOT>
+// the elements [e, stop) of a run whose window starts at bin C; LO32: the
+// image lies inside one 4 GB-aligned window (a column's buffer start is one
+// 32-bit add)
+template <int C, int NT, int W, int NSLOT, bool LO32>
 __device__ inline void fp_tof_run(cptr<int32_t> row, int e, int stop, const float* __restrict__ img,
                                   const ImageDims& d, const int (&zb4)[NSLOT], float (&acc)[NSLOT][NT]) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -556,8 +558,9 @@ __device__ inline void fp_tof_run(cptr<int32_t> row, int e, int stop, const floa
 #pragma unroll
     for (int u = 0; u < kRowGroup; ++u) {
       E[u] = rec[e + u];
+      const unsigned colf = static_cast<unsigned>(__mul24(E[u].v[0] & 0xfffff, d.nzp));
       const __amdgpu_buffer_rsrc_t column =
-          run_buffer(as_const(img), static_cast<unsigned>(__mul24(E[u].v[0] & 0xfffff, d.nzp)), d.nz);
+          LO32 ? run_buffer_lo32(as_const(img), colf, d.nz) : run_buffer(as_const(img), colf, d.nz);
       const int z4 = (E[u].v[1] >> 16) * 4;
 #pragma unroll
       for (int q = 0; q < NSLOT; ++q) v[u][q] = load_run(column, z4 + zb4[q]);
@@ -567,8 +570,9 @@ __device__ inline void fp_tof_run(cptr<int32_t> row, int e, int stop, const floa
   }
   for (; e < stop; ++e) {
     const Words<R> E = rec[e];
+    const unsigned colf = static_cast<unsigned>(__mul24(E.v[0] & 0xfffff, d.nzp));
     const __amdgpu_buffer_rsrc_t column =
-        run_buffer(as_const(img), static_cast<unsigned>(__mul24(E.v[0] & 0xfffff, d.nzp)), d.nz);
+        LO32 ? run_buffer_lo32(as_const(img), colf, d.nz) : run_buffer(as_const(img), colf, d.nz);
     const int z4 = (E.v[1] >> 16) * 4;
     float v[NSLOT];
 #pragma unroll
@@ -582,7 +586,7 @@ __device__ inline void fp_tof_run(cptr<int32_t> row, int e, int stop, const floa
 // element's image column gathered once and added to the accumulators of its
 // window, in row order (the reference's per-bin sum order). The window start
 // is dispatched once per run of elements that share it.
-template <int MODE, int NT, int W, int NSLOT>
+template <int MODE, int NT, int W, int NSLOT, bool LO32>
 __device__ void fp_tof_rows_task(const Family& f, const int32_t* __restrict__ rec, const float* __restrict__ img,
                                  float* __restrict__ out, const ImageDims& d, const TofConsts& T, const FpData& data,
                                  int t, int lane) {
@@ -610,7 +614,7 @@ __device__ void fp_tof_rows_task(const Family& f, const int32_t* __restrict__ re
     switch (c) {
 #define MI_TW(C)                                                                               \
   case C:                                                                                      \
-    if constexpr (C <= NT - W) fp_tof_run<C, NT, W, NSLOT>(row, e, stop, img, d, zb4, acc); \
+    if constexpr (C <= NT - W) fp_tof_run<C, NT, W, NSLOT, LO32>(row, e, stop, img, d, zb4, acc); \
     break;
       MI_TW(0) MI_TW(1) MI_TW(2) MI_TW(3) MI_TW(4) MI_TW(5) MI_TW(6) MI_TW(7)
       MI_TW(8) MI_TW(9) MI_TW(10) MI_TW(11) MI_TW(12) MI_TW(13) MI_TW(14) MI_TW(15)
@@ -636,7 +640,7 @@ __device__ void fp_tof_rows_task(const Family& f, const int32_t* __restrict__ re
   }
 }
 
-template <int MODE, int NT, int W>
+template <int MODE, int NT, int W, bool LO32>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_tof_rows_kernel(const FpTask* __restrict__ tasks,
                                                                             int ntasks,
                                                                             const Family* __restrict__ fams,
@@ -649,8 +653,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_tof_rows_kernel(cons
   if (t >= ntasks) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   const Family f = fams[tasks[t].first];
-  if (f.nax <= kWave) fp_tof_rows_task<MODE, NT, W, 1>(f, rec, img, out, d, T, data, t, lane);
-  else fp_tof_rows_task<MODE, NT, W, 2>(f, rec, img, out, d, T, data, t, lane);
+  if (f.nax <= kWave) fp_tof_rows_task<MODE, NT, W, 1, LO32>(f, rec, img, out, d, T, data, t, lane);
+  else fp_tof_rows_task<MODE, NT, W, 2, LO32>(f, rec, img, out, d, T, data, t, lane);
 }
 
 constexpr int kEntryGroup = 2;  // table entries whose projection-data gathers are issued together
@@ -861,13 +865,20 @@ void launch_forward_project_tof_rows(FpMode mode, const FpTask* d_tasks, int nta
                                      hipStream_t stream) {
   if (ntasks <= 0) return;
   const dim3 grid((ntasks + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
-#define MI_FPR(M, NT, W)                                                                                              \
-  hipLaunchKernelGGL((fp_tof_rows_kernel<M, NT, W>), grid, block, 0, stream, d_tasks, ntasks, d_fams, d_rec, d_img, \
-                     d_out, dims, tof, data)
+  // the image inside one 4 GB-aligned window of the address space
+  const uint64_t first = reinterpret_cast<uint64_t>(d_img);
+  const bool lo32 = (first >> 32) == ((first + static_cast<uint64_t>(dims.ncols) * dims.nzp * 4 - 1) >> 32);
+#define MI_FPR(M, NT, W)                                                                                           \
+  if (lo32)                                                                                                         \
+    hipLaunchKernelGGL((fp_tof_rows_kernel<M, NT, W, true>), grid, block, 0, stream, d_tasks, ntasks, d_fams, d_rec, \
+                       d_img, d_out, dims, tof, data);                                                              \
+  else                                                                                                              \
+    hipLaunchKernelGGL((fp_tof_rows_kernel<M, NT, W, false>), grid, block, 0, stream, d_tasks, ntasks, d_fams,      \
+                       d_rec, d_img, d_out, dims, tof, data)
 #define MI_FPR_CASE(M)                                            \
   case M:                                                         \
-    if (nt == 12 && w == kTofFpNarrow) MI_FPR(M, 12, kTofFpNarrow); \
-    else MI_FPR(M, 16, kTofFpWide);                               \
+    if (nt == 12 && w == kTofFpNarrow) { MI_FPR(M, 12, kTofFpNarrow); } \
+    else { MI_FPR(M, 16, kTofFpWide); }                               \
     break;
   switch (mode) {
     MI_FPR_CASE(FP_PLAIN)
