@@ -441,10 +441,10 @@ int mi_batch_synchronize(mi_batch* batch);
 /* ---- launch tuning ---------------------------------------------------------- */
 /* No counterpart in the reference. Launch shape of the back projection: image
  * columns are visited in tiles of MI_TUNE_BP_TILE_X x MI_TUNE_BP_TILE_Y
- * (default 16 x 16; 1 x 1 is raster order); MI_TUNE_BP_XCD_REMAP maps
+ * (default 8 x 64; 1 x 1 is raster order); MI_TUNE_BP_XCD_REMAP maps
  * workgroups to XCDs: 0 in hardware order, 1 in one contiguous range per XCD,
  * k > 1 in chunks of k workgroups (4 columns each) dealt to the XCDs in turn
- * (default 64: one 16 x 16 tile per chunk); MI_TUNE_BP_SLOT_SKIP skips z slots
+ * (default 64: half an 8 x 64 tile per chunk); MI_TUNE_BP_SLOT_SKIP skips z slots
  * an entry cannot reach (default off). Results do not depend on these
  * settings. MI_TUNE_FP_XCD_CHUNK maps forward-projection workgroups (4 tasks
  * each) the same way: 0 one contiguous range per XCD, k > 0 chunks of k

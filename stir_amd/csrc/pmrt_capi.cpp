@@ -688,8 +688,9 @@ void mi_ctx::build_table() {
 }
 
 // Stored TOF rows when every family has a row of its own, the planes per
-// axial position fit the record's 4 bits and the widest live range fits a
-// window; otherwise the on-the-fly kernels stay in use.
+// axial position fit the record's 4 bits, the widest live range fits a
+// window and the records fit in three quarters of the free device memory;
+// otherwise the on-the-fly kernels stay in use.
 void mi_ctx::build_tof_rows(int64_t n) {
   tof_fp_rec.release();
   tof_bp_rec.release();
@@ -727,6 +728,13 @@ void mi_ctx::build_tof_rows(int64_t n) {
   if (tof_rows_mode == 1 && s <= mi::kTofBpNarrow) bw = mi::kTofBpNarrow;
   else if (s <= mi::kTofBpWide) bw = mi::kTofBpWide;
   else return;
+  // the records only where they fit beside everything else (a quarter of the
+  // free memory is left for the data, images and sensitivities set up later)
+  const size_t need = sizeof(int32_t) * (static_cast<size_t>(pool_end) * mi::tof_fp_words(fw) +
+                                         static_cast<size_t>(n) * mi::tof_bp_words(bw));
+  size_t free_b = 0, total_b = 0;
+  HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+  if (need > free_b / 4 * 3) return;
   tof_fp_rec.alloc(pool_end * mi::tof_fp_words(fw));
   mi::launch_tof_fp_records(fams.p, static_cast<int>(h_fams.size()), lors.p, pool.p, dims, tofc, fnt, fw,
                             tof_fp_rec.p, stream);
