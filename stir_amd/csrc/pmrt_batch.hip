@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_group_kernel(const P
   const int wave = static_cast<int>(threadIdx.x) / kWave;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   const int a = lane % G;
-  const int wi = (static_cast<int>(blockIdx.x) * kWavesPerBlock + wave) * F + lane / G;
+  const int wi = (batch_block() * kWavesPerBlock + wave) * F + lane / G;
   WorkItem W{0, -1};
   if (wi < nwork) W = work[wi];
   const bool item = W.index >= 0;
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_group_kernel(const P
   const int wave = static_cast<int>(threadIdx.x) / kWave;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   const int zl = lane % G;
-  const int wi = (static_cast<int>(blockIdx.x) * kWavesPerBlock + wave) * F + lane / G;
+  const int wi = (batch_block() * kWavesPerBlock + wave) * F + lane / G;
   WorkItem W{0, -1};
   if (wi < nwork) W = work[wi];
   const bool item = W.index >= 0;
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_rowgroup_kernel(cons
                                                                             const int32_t* __restrict__ item_fams,
                                                                             int nitems) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int wi = static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
+  const int wi = batch_block() * kWavesPerBlock + wave;
   if (wi >= nitems) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   const RowGroupItem it = items[wi];
@@ -259,9 +259,9 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_rowgroup_kernel(cons
   }
 }
 
-int group_blocks(int nwork, int G) {
+unsigned group_blocks(int nwork, int G) {
   const int per_block = kWavesPerBlock * (kWave / G);
-  return (nwork + per_block - 1) / per_block;
+  return batch_grid((nwork + per_block - 1) / per_block);
 }
 
 }  // namespace
@@ -287,7 +287,7 @@ void launch_fp_group(FpMode mode, int G, const ProblemDesc* d_descs, const WorkI
 void launch_fp_rowgroup(FpMode mode, const ProblemDesc* d_descs, const RowGroupItem* d_items, const int32_t* d_item_fams,
                         int nitems, hipStream_t stream) {
   if (nitems <= 0) return;
-  const dim3 grid((nitems + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  const dim3 grid(batch_grid((nitems + kWavesPerBlock - 1) / kWavesPerBlock)), block(kWave * kWavesPerBlock);
   if (mode == FP_RATIO)
     hipLaunchKernelGGL(fp_rowgroup_kernel<FP_RATIO>, grid, block, 0, stream, d_descs, d_items, d_item_fams, nitems);
   else

@@ -2502,8 +2502,9 @@ int mi_batch_tune(mi_batch* b, int* choice) {
     HIPCHECK(hipSetDevice(b->device));
     batch_wait(b);
     struct Cand { int rg, gmax, widen, pack, tx, ty; };
-    const Cand cands[] = {{32, 32, 0, 2, 32, 8}, {32, 32, 0, 0, 32, 8}, {16, 32, 0, 2, 32, 8}, {0, 32, 0, 2, 32, 8},
-                          {0, 0, 0, 0, 32, 8},   {32, 32, 0, 2, 8, 8},  {32, 32, 0, 2, 16, 16}, {32, 32, 0, 1, 32, 8}};
+    const Cand cands[] = {{32, 32, 0, 2, 32, 8},  {32, 32, 0, 0, 32, 8}, {16, 32, 0, 2, 32, 8}, {0, 32, 0, 2, 32, 8},
+                          {0, 0, 0, 0, 32, 8},    {32, 32, 0, 2, 8, 8},  {32, 32, 0, 2, 16, 16}, {32, 32, 0, 1, 32, 8},
+                          {16, 32, 0, 2, 8, 64},  {16, 32, 0, 2, 8, 32}};
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));

@@ -38,6 +38,18 @@ __device__ inline int xcd_chunk_remap(int bid, int chunk) {
   return ((i / chunk) * kNumXcd + xcd) * chunk + i % chunk;
 }
 
+// The batch kernels deal their blocks to the XCDs in chunks of kBatchChunk
+// (as fp_kernel / bp_kernel do): neighbouring work items (a tile's columns,
+// the families of one (view, tangential position)) share one XCD's L2. Their
+// grids are padded to whole rounds (batch_grid); blocks past the work find no
+// item.
+constexpr int kBatchChunk = 32;
+__device__ inline int batch_block() { return xcd_chunk_remap(static_cast<int>(blockIdx.x), kBatchChunk); }
+inline unsigned batch_grid(int nblocks) {
+  const int round = kNumXcd * kBatchChunk;
+  return static_cast<unsigned>((nblocks + round - 1) / round * round);
+}
+
 // Address spaces. Tables and inputs that a kernel only reads are accessed
 // through the constant address space, so that reads at wave-uniform
 // addresses become scalar loads; outputs through the global one. Kernel

@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_batch_kernel(const P
                                                                          const FpTask* const* __restrict__ task_lists,
                                                                          const WorkItem* __restrict__ work, int nwork) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int wi = static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
+  const int wi = batch_block() * kWavesPerBlock + wave;
   if (wi >= nwork) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   const int p = __builtin_amdgcn_readfirstlane(work[wi].problem);
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_batch_kernel(const P
                                                                          const WorkItem* __restrict__ work, int nwork,
                                                                          int subset, int threshold) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int wi = static_cast<int>(blockIdx.x) * kWavesPerBlock + wave;
+  const int wi = batch_block() * kWavesPerBlock + wave;
   if (wi >= nwork) return;
   const int lane = static_cast<int>(threadIdx.x % kWave);
   const int p = __builtin_amdgcn_readfirstlane(work[wi].problem);
@@ -689,7 +689,7 @@ void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const 
 void launch_fp_batch(FpMode mode, int nslot, const ProblemDesc* d_descs, const FpTask* const* d_task_lists,
                      const WorkItem* d_work, int nwork, hipStream_t stream) {
   if (nwork <= 0) return;
-  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  const dim3 grid(batch_grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock)), block(kWave * kWavesPerBlock);
 #define MI_FPB(M, S) \
   hipLaunchKernelGGL((fp_batch_kernel<M, S>), grid, block, 0, stream, d_descs, d_task_lists, d_work, nwork)
 #define MI_FPB_S(M)              \
@@ -708,7 +708,7 @@ void launch_fp_batch(FpMode mode, int nslot, const ProblemDesc* d_descs, const F
 void launch_bp_batch(BpMode mode, int ppa, int nslot, const ProblemDesc* d_descs, const WorkItem* d_work, int nwork,
                      int subset, int threshold, hipStream_t stream) {
   if (nwork <= 0) return;
-  const dim3 grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock), block(kWave * kWavesPerBlock);
+  const dim3 grid(batch_grid((nwork + kWavesPerBlock - 1) / kWavesPerBlock)), block(kWave * kWavesPerBlock);
 #define MI_BPB(M, P, S, NP)                                                                                         \
   hipLaunchKernelGGL((bp_batch_kernel<M, P, S, NP>), grid, block, 0, stream, d_descs, d_work, nwork, subset, threshold)
   // nslot < 0: one slot, -nslot parts
