@@ -324,8 +324,10 @@ struct mi_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timing) { e0 = take_event(); e1 = take_event(); HIPCHECK(hipEventRecord(e0, st)); }
     if (ntof > 1 && !nontof && tof_bp_w > 0)
-      mi::launch_back_project_tof_rows(mode, uniform_ppa, list_off.p, tof_bp_rec.p, tof_bp_w, subset_begin(subset),
-                                       subset_end(subset), d_proj, d_img, dims, tofc, data, tof_bp_tuning, st);
+      mi::launch_back_project_tof_rows(mode, uniform_ppa, list_off.p, tof_bp_rec.p, tof_bp_w,
+                                       static_cast<int64_t>(tof_bp_rec.n / mi::tof_bp_words(tof_bp_w)) - 1,
+                                       subset_begin(subset), subset_end(subset), d_proj, d_img, dims, tofc, data,
+                                       tof_bp_tuning, st);
     else if (ntof > 1 && !nontof)
       mi::launch_back_project_tof(mode, uniform_ppa, list_off.p, entries.p, entry_d.p, subset_begin(subset),
                                   subset_end(subset), d_proj, d_img, dims, tofc, data, tof_bp_tuning, st);
@@ -688,7 +690,8 @@ void mi_ctx::build_table() {
 }
 
 // Stored TOF rows when every family has a row of its own, the planes per
-// axial position fit the record's 4 bits, the widest live range fits a
+// axial position fit the record's 4 bits, the TOF data hold fewer than 2^31
+// bins, the widest live range fits a
 // window and the records fit in three quarters of the free device memory;
 // otherwise the on-the-fly kernels stay in use.
 void mi_ctx::build_tof_rows(int64_t n) {
@@ -699,6 +702,7 @@ void mi_ctx::build_tof_rows(int64_t n) {
   std::vector<std::pair<int64_t, int64_t>> rows;
   rows.reserve(h_fams.size());
   int64_t pool_end = 0;
+  if (total_bins() >= (int64_t{1} << 31)) return;  // back records hold a bin index in 31 bits
   for (const mi::Family& f : h_fams) {
     if (f.ppa < 1 || f.ppa > 15) return;
     rows.emplace_back(f.row_start, f.row_len);
@@ -738,7 +742,8 @@ void mi_ctx::build_tof_rows(int64_t n) {
   tof_fp_rec.alloc(pool_end * mi::tof_fp_words(fw));
   mi::launch_tof_fp_records(fams.p, static_cast<int>(h_fams.size()), lors.p, pool.p, dims, tofc, fnt, fw,
                             tof_fp_rec.p, stream);
-  tof_bp_rec.alloc(n * mi::tof_bp_words(bw));
+  tof_bp_rec.alloc((n + 1) * mi::tof_bp_words(bw));  // + the all-zero record read past a list's end
+  HIPCHECK(hipMemsetAsync(tof_bp_rec.p + n * mi::tof_bp_words(bw), 0, sizeof(int32_t) * mi::tof_bp_words(bw), stream));
   mi::launch_tof_bp_records(entries.p, entry_d.p, n, tofc, bw, tof_bp_rec.p, stream);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(stream));

@@ -160,7 +160,8 @@ void launch_gather_floats(const float* d_src, const uint32_t* d_idx, float* d_ds
 //         equal c in the row, at most 65535)
 //     [2..] the weights of TOF bins c .. c + W - 1
 //   back record of a table entry, 3 + W words (tof_bp_words):
-//     [0] sino_base  [1] k  [2] nax | ppa << 16 | klo << 20 | nk << 25
+//     [0] klo * block + sino_base (the run of the window's first bin)
+//     [1] k  [2] nax | ppa << 16 | klo << 20 | nk << 25
 //     [3..] the weights of TOF bins klo .. klo + nk - 1, zeros up to W
 // A forward window starts at c = min(klo, nt - W) so it stays inside the nt
 // accumulators of the kernel; bins outside the live range carry weight 0.
@@ -184,9 +185,10 @@ void launch_forward_project_tof_rows(FpMode mode, const FpTask* d_tasks, int nta
                                      const int32_t* d_rec, int nt, int w, const float* d_img, float* d_out,
                                      const ImageDims& dims, const TofConsts& tof, const FpData& data,
                                      hipStream_t stream);
+// d_rec holds num_entries records followed by one all-zero record
 void launch_back_project_tof_rows(BpMode mode, int ppa, const int64_t* d_list_off, const int32_t* d_rec, int w,
-                                  int subset_begin, int subset_end, const float* d_proj, float* d_img,
-                                  const ImageDims& dims, const TofConsts& tof, const BpData& data,
+                                  int64_t num_entries, int subset_begin, int subset_end, const float* d_proj,
+                                  float* d_img, const ImageDims& dims, const TofConsts& tof, const BpData& data,
                                   const BpTuning& tune, hipStream_t stream);
 
 // ---- grouped kernels for many problems in one launch (pmrt_batch.hip)

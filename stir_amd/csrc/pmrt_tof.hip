@@ -508,7 +508,7 @@ __global__ void tof_bp_records_kernel(const BpEntry* __restrict__ ent, const flo
     const int nk = khi < klo ? 0 : khi - klo + 1;
     if (nk == 0) klo = 0;
     int32_t* r = rec + i * words;
-    r[0] = E.sino_base;
+    r[0] = static_cast<int32_t>(klo * T.block + E.sino_base);  // the run of the window's first bin
     r[1] = E.k;
     r[2] = (E.nax_ppa & 0xfffff) | (klo << 20) | (nk << 25);
     for (int j = 0; j < w; ++j) r[3 + j] = __float_as_int(j < nk ? tw[klo + j] : 0.F);
@@ -671,7 +671,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_rows_kernel(cons
                                                                             float* __restrict__ img, ImageDims d,
                                                                             int nwork, int2 tile, int remap,
                                                                             int64_t block, int accumulate,
-                                                                            BpData data) {
+                                                                            int64_t sentinel, BpData data) {
   constexpr int R = tof_bp_words(W);
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
   const int b = remap == 1 ? xcd_remap(blockIdx.x, gridDim.x)
@@ -696,11 +696,11 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_rows_kernel(cons
       float wt[kEntryGroup][W];
 #pragma unroll
       for (int u = 0; u < kEntryGroup; ++u) {
-        // a repeated last entry adds nothing: its weights are taken as 0
-        const bool real = e + u < end;
-        const Words<R> E = table[real ? e + u : e];
+        // past the list's end: the all-zero record after the table (no live
+        // bin, weights 0), so it adds nothing
+        const Words<R> E = table[e + u < end ? e + u : sentinel];
         const int base = E.v[0], k = E.v[1], np = E.v[2] & 0xfffff;
-        const int klo = (E.v[2] >> 20) & 31, nk = real ? static_cast<int>(static_cast<unsigned>(E.v[2]) >> 25) : 0;
+        const int nk = static_cast<int>(static_cast<unsigned>(E.v[2]) >> 25);
         const int nax = np & 0xffff;
         int off[NSLOT];
 #pragma unroll
@@ -714,13 +714,17 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_tof_rows_kernel(cons
             off[q] = ok ? static_cast<int>(a << 2) : -4;
           }
         }
+        // the record holds the run of bin klo; the run of bin klo + j starts
+        // one TOF block after that of bin klo + j - 1: one 64-bit step per bin
+        uint64_t run_addr = reinterpret_cast<uint64_t>(proj) + 4 * static_cast<uint64_t>(static_cast<uint32_t>(base));
 #pragma unroll
         for (int j = 0; j < W; ++j) {
-          wt[u][j] = j < nk ? __int_as_float(E.v[3 + j]) : 0.F;
-          const __amdgpu_buffer_rsrc_t run = run_buffer(
-              as_const(proj), static_cast<unsigned>(static_cast<int64_t>(klo + j) * block + base), j < nk ? nax : 0);
+          wt[u][j] = __int_as_float(E.v[3 + j]);  // stored as 0 past the live range
+          const __amdgpu_buffer_rsrc_t run = __builtin_amdgcn_make_buffer_rsrc(
+              reinterpret_cast<void*>(run_addr), static_cast<short>(0), j < nk ? 4 * nax : 0, 0x00020000);
 #pragma unroll
           for (int q = 0; q < NSLOT; ++q) p[u][j][q] = load_run(run, off[q]);
+          run_addr += 4 * static_cast<uint64_t>(block);
         }
       }
 #pragma unroll
@@ -897,8 +901,8 @@ void launch_forward_project_tof_rows(FpMode mode, const FpTask* d_tasks, int nta
 }
 
 void launch_back_project_tof_rows(BpMode mode, int ppa, const int64_t* d_list_off, const int32_t* d_rec, int w,
-                                  int subset_begin, int subset_end, const float* d_proj, float* d_img,
-                                  const ImageDims& dims, const TofConsts& tof, const BpData& data,
+                                  int64_t num_entries, int subset_begin, int subset_end, const float* d_proj,
+                                  float* d_img, const ImageDims& dims, const TofConsts& tof, const BpData& data,
                                   const BpTuning& tune, hipStream_t stream) {
   const int nslot = (dims.nzp + kWave - 1) / kWave;
   const int2 tile = make_int2(tune.tile_x > 0 ? tune.tile_x : 1, tune.tile_y > 0 ? tune.tile_y : 1);
@@ -918,7 +922,7 @@ void launch_back_project_tof_rows(BpMode mode, int ppa, const int64_t* d_list_of
     const int sb = split ? s : subset_begin, se = split ? s + 1 : subset_end, accumulate = split && s > subset_begin;
 #define MI_BPR(M, P, S, W)                                                                                    \
   hipLaunchKernelGGL((bp_tof_rows_kernel<M, P, S, W>), grid, block, 0, stream, d_list_off, d_rec, sb, se, d_proj, \
-                     d_img, dims, nwork, tile, tune.xcd_remap, tof.block, accumulate, data)
+                     d_img, dims, nwork, tile, tune.xcd_remap, tof.block, accumulate, num_entries, data)
 #define MI_BPR_W(M, P, S)                         \
   if (w == kTofBpNarrow) MI_BPR(M, P, S, kTofBpNarrow); \
   else MI_BPR(M, P, S, kTofBpWide);
