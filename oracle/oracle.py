@@ -76,6 +76,8 @@ def load_library(path: str):
                                      ctypes.c_int, _f32p]
     L.or_subset_ratio.argtypes = [vp, _f32p, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   _f32p]
+    L.or_hessian_times_input.restype = ctypes.c_int
+    L.or_hessian_times_input.argtypes = [vp, _f32p, vp, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
     L.or_log_likelihood.restype = ctypes.c_double
     L.or_log_likelihood.argtypes = [vp, _f32p, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.or_osem.argtypes = [vp, _f32p, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -357,6 +359,25 @@ class OraclePlan:
         out = np.zeros(self.num_bins, np.float32)
         self._L.or_subset_ratio(self._p, y, bp[0] if bp else None, x.reshape(-1), subset, num_subsets,
                               int(zero_end_planes), int(add_sensitivity), out)
+        return out
+
+    def hessian_times_input(self, y, current, inp, subset, num_subsets, bkg=None, approximate=False, out=None):
+        """out -= BP(q) for the subset's data-term Hessian product (exact: q =
+        divide_and_truncate(y * A inp, (A current + b)^2); approximate: q =
+        divide_and_truncate(A inp, y / m / m)). Raises ValueError where the
+        reference aborts (a negative value in A inp)."""
+        y = self._sino(y); inp = self._img(inp)
+        cur = None if approximate else self._img(current)
+        bp = _opt_ptr(None if bkg is None else self._sino(bkg))
+        if out is None:
+            out = np.zeros(self.img.shape, np.float32)
+        assert out.dtype == np.float32 and out.flags.c_contiguous
+        rc = self._L.or_hessian_times_input(self._p, y, bp[0] if bp else None,
+                                            None if cur is None else cur.ctypes.data_as(ctypes.c_void_p),
+                                            inp.reshape(-1), subset, num_subsets, int(bool(approximate)),
+                                            out.reshape(-1))
+        if rc:
+            raise ValueError("forward projection of the input has negative values")
         return out
 
     def log_likelihood(self, y, x, bkg=None, subset=-1, num_subsets=1, zero_end_planes=False) -> float:

@@ -1368,6 +1368,94 @@ void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, cons
   free(q);
 }
 
+/* divide_and_truncate of numerator / denominator over one viewgram
+ * (recon_array_functions.cxx:172-266, rim_truncation_sino = 0): the small
+ * value is 1e-6 times the numerator viewgram's maximum (not below 0). */
+static void divide_and_truncate_vg2(const or_plan* p, const float* num_in, const float* den, float* q, int seg,
+                                    int view, int kk)
+{
+  const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+  float mx = -FLT_MAX;
+  for (int a = 0; a < nax; ++a)
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+      const float v = num_in[tbin(p, seg, view, a, t, kk)];
+      if (v > mx) mx = v;
+    }
+  const float small_num = maxf_(mx * 0.000001F, 0.F);
+  for (int a = 0; a < nax; ++a)
+    for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+      const int64_t bi = tbin(p, seg, view, a, t, kk);
+      const float nv = num_in[bi];
+      const float max_q = 10000.F;
+      q[bi] = nv <= small_num ? 0.F : (nv > max_q * den[bi] ? max_q : nv / den[bi]);
+    }
+}
+
+/* The data-term Hessian products of one subset, accumulated as the reference
+ * does (output -= BP(q)), over the subset's viewgrams (all TOF bins):
+ *   exact (accumulate_sub_Hessian_times_input_without_penalty,
+ *   ...AndProjData.cxx:1066-1218): ybar = FP(current) + b,
+ *     q = divide_and_truncate(y * FP(input), ybar * ybar);
+ *   approximate (actual_add_multiplication_with_approximate_sub_Hessian_
+ *   without_penalty, :957-1064): the data normalised twice (BinNormalisation::
+ *   apply divides by the bin efficiency, at least 1e-20, BinNormalisation.cxx:
+ *   92-105; the multiplicative factors are those efficiencies),
+ *     q = divide_and_truncate(FP(input), y / m / m).
+ * The data are used as stored (no end-plane zeroing: the reference reads them
+ * with get_related_viewgrams) and no multiplicative factor enters the exact
+ * form. Returns 1, with out untouched, when FP(input) has a negative value
+ * (the reference aborts then, :1036-1050,1147-1151). */
+int or_hessian_times_input(const or_plan* p, const float* y, const float* bkg, const float* current,
+                           const float* input, int subset, int nsub, int approximate, float* out)
+{
+  const int64_t nb = p->num_bins * p->ntof;
+  float* fin = (float*)calloc((size_t)nb, sizeof(float));
+  float* num = (float*)calloc((size_t)nb, sizeof(float));
+  float* den = (float*)calloc((size_t)nb, sizeof(float));
+  float* q = (float*)calloc((size_t)nb, sizeof(float));
+  const int64_t nv = (int64_t)p->g.nz * p->g.ny * p->g.nx;
+  float* bp = (float*)calloc((size_t)nv, sizeof(float));
+  int negative = 0;
+  or_forward_project_omp(p, input, fin, subset, nsub);
+  for (int64_t i = 0; i < nb; ++i)
+    if (fin[i] < 0) negative = 1;
+  if (!negative) {
+    if (!approximate) or_forward_project_omp(p, current, den, subset, nsub);
+    vs_t* vs; const int n = all_vs(p, subset, nsub, &vs);
+    for (int k = 0; k < n; ++k) {
+      const int seg = vs[k].seg, view = vs[k].view;
+      const int nax = p->g.num_axial_poss[SEGI(p, seg)];
+      for (int kk = 0; kk < p->ntof; ++kk) {
+        for (int a = 0; a < nax; ++a)
+          for (int t = p->g.min_tangential_pos_num; t <= p->max_tang; ++t) {
+            const int64_t i = tbin(p, seg, view, a, t, kk);
+            if (approximate) {
+              float d = y[i];
+              if (p->mult) {
+                const float e = maxf_(1.E-20F, p->mult[i]);
+                d = d / e;
+                d = d / e;
+              }
+              num[i] = fin[i];
+              den[i] = d;
+            } else {
+              float ybar = den[i];
+              if (bkg) ybar = ybar + bkg[i];
+              num[i] = y[i] * fin[i];
+              den[i] = ybar * ybar;
+            }
+          }
+        divide_and_truncate_vg2(p, num, den, q, seg, view, kk);
+      }
+    }
+    free(vs);
+    or_back_project(p, q, bp, subset, nsub);
+    for (int64_t j = 0; j < nv; ++j) out[j] = out[j] - bp[j];
+  }
+  free(fin); free(num); free(den); free(q); free(bp);
+  return negative;
+}
+
 /* accumulate_loglikelihood -- recon_array_functions.cxx:336-377 via
  * RPC_process_related_viewgrams_accumulate_loglikelihood (...AndProjData.cxx:1484-1515) */
 double or_log_likelihood(const or_plan* p, const float* y, const float* bkg, const float* x,

@@ -162,6 +162,12 @@ void or_subset_ratio(const or_plan* p, const float* y, const float* bkg, const f
                      int subset, int num_subsets, int zero_end_planes, int add_sensitivity, float* q);
 /* compute_sub_gradient_without_penalty(_plus_sensitivity):
  * grad = BP_subset(y/ybar) (+sens) or BP_subset(y/ybar - 1). */
+/* The data-term Hessian products of one subset (accumulate_sub_Hessian_times_
+ * input_without_penalty when approximate == 0, add_multiplication_with_
+ * approximate_sub_Hessian_without_penalty when 1): out -= BP(q). Returns 1
+ * (out untouched) when the forward projection of input has a negative value. */
+int or_hessian_times_input(const or_plan* p, const float* y, const float* bkg, const float* current,
+                           const float* input, int subset, int nsub, int approximate, float* out);
 void or_subset_gradient(const or_plan* p, const float* y, const float* bkg, const float* x,
                         int subset, int num_subsets, int zero_end_planes, int add_sensitivity,
                         float* grad);

@@ -1,0 +1,159 @@
+"""Oracle pins for the objective's data-term Hessian products (CPU):
+accumulate_sub_Hessian_times_input_without_penalty and
+add_multiplication_with_approximate_sub_Hessian_without_penalty
+(PoissonLogLikelihoodWithLinearModelForMeanAndProjData.cxx:957-1218).
+
+Pins, none of them a restatement of the formula under test:
+* the exact product against central differences of the subset gradient
+  (itself pinned by the golden image, closed forms and finite differences of
+  the log-likelihood): H v = d/dt grad(x + t v) at t = 0, as the reference's
+  own objective test compares Hessian and gradient differences
+  (test_PoissonLogLikelihoodWithLinearModelForMeanAndProjData.cxx:178-372);
+* symmetry <u, H v> = <v, H u> and negative definiteness <v, H v> < 0 (H is
+  -A^T D A with D >= 0): a transposed or one-sided operator fails them;
+* where the data equal the model (y = A x + b) the exact and the approximate
+  products agree (y / ybar^2 = 1 / y);
+* the approximate product with multiplicative factors m = 2 (exact in float)
+  is 4 times the one without: the data enter as y / m / m (y * m^2, y / m or
+  ignoring m give 1/4, 2 or 1);
+* a negative value in the forward projection of the input aborts, as in the
+  reference.
+Each pin is then run against oracle builds with one plausible mistake planted
+in the C source (ybar not squared, the background left out of ybar, one
+division by the factors, the product added instead of subtracted): every
+planted mistake must fail its pin.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as oracle_mod
+from oracle.oracle import OraclePlan
+from stir_amd.geometry import ProjDataInfoCylindrical, RayTracingMatrixParameters, Scanner, VoxelsOnCartesianGrid
+
+NSUB = 4
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    sc = Scanner.cylindrical(4, 64, 80.0, 4.0)
+    pdi = ProjDataInfoCylindrical.construct(sc, span=1, max_delta=3, num_views=32, num_tangential_poss=32)
+    img = VoxelsOnCartesianGrid(7, 32, 32, 2.0, 2.0, 2.0)
+    O = OraclePlan(pdi, img, RayTracingMatrixParameters())
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0.5, 1.5, img.shape).astype(np.float32)
+    bkg = np.full(pdi.num_bins, 0.1, np.float32)
+    ybar = O.forward_project(x) + bkg
+    y = rng.poisson(ybar * 20.0).astype(np.float32) / 20.0 + 0.05  # no zero counts: no truncation fires
+    return pdi, img, O, x, y, bkg
+
+
+def _dot(a, b):
+    return float(np.dot(a.reshape(-1).astype(np.float64), b.reshape(-1).astype(np.float64)))
+
+
+def test_exact_product_is_the_derivative_of_the_gradient(tiny):
+    pdi, img, O, x, y, bkg = tiny
+    rng = np.random.default_rng(12)
+    v = rng.uniform(0.0, 1.0, img.shape).astype(np.float32)
+    for s in (0, 3):
+        h = O.hessian_times_input(y, x, v, s, NSUB, bkg=bkg)
+        eps = 1e-2
+        gp = O.subset_gradient(y, (x + eps * v).astype(np.float32), s, NSUB, bkg=bkg).astype(np.float64)
+        gm = O.subset_gradient(y, (x - eps * v).astype(np.float32), s, NSUB, bkg=bkg).astype(np.float64)
+        fd = (gp - gm) / (2 * eps)
+        err = np.abs(h - fd).max() / np.abs(fd).max()
+        print(f"subset {s}: |H v - central difference| / max = {err:.2e}")
+        assert err < 5e-4
+
+
+def test_symmetric_and_negative_definite(tiny):
+    pdi, img, O, x, y, bkg = tiny
+    rng = np.random.default_rng(13)
+    u = rng.uniform(0.0, 1.0, img.shape).astype(np.float32)
+    v = rng.uniform(0.0, 1.0, img.shape).astype(np.float32)
+    for approximate in (False, True):
+        hu = O.hessian_times_input(y, x, u, 1, NSUB, bkg=bkg, approximate=approximate)
+        hv = O.hessian_times_input(y, x, v, 1, NSUB, bkg=bkg, approximate=approximate)
+        a, b = _dot(u, hv), _dot(v, hu)
+        assert abs(a - b) <= 1e-5 * abs(a)
+        assert _dot(v, hv) < 0 and _dot(u, hu) < 0
+
+
+def test_exact_equals_approximate_where_the_data_equal_the_model(tiny):
+    pdi, img, O, x, _, bkg = tiny
+    y = O.forward_project(x) + bkg  # float32 model: ybar = y bit for bit
+    v = np.random.default_rng(14).uniform(0.0, 1.0, img.shape).astype(np.float32)
+    he = O.hessian_times_input(y, x, v, 2, NSUB, bkg=bkg)
+    ha = O.hessian_times_input(y, x, v, 2, NSUB, approximate=True)
+    assert np.abs(he - ha).max() <= 1e-5 * np.abs(ha).max()
+
+
+def test_approximate_divides_the_data_by_the_factors_twice(tiny):
+    pdi, img, O, x, y, bkg = tiny
+    v = np.random.default_rng(15).uniform(0.0, 1.0, img.shape).astype(np.float32)
+    h1 = O.hessian_times_input(y, x, v, 0, NSUB, approximate=True)
+    m = np.full(pdi.num_bins, 2.0, np.float32)
+    O.set_multiplicative(m)
+    try:
+        h2 = O.hessian_times_input(y, x, v, 0, NSUB, approximate=True)
+    finally:
+        O.set_multiplicative(None)
+    ratio = np.abs(h2).max() / np.abs(h1).max()
+    print(f"approximate product with m = 2 / without: {ratio}")
+    assert np.allclose(h2, 4.0 * h1, rtol=1e-6, atol=0)
+
+
+def test_accumulates_into_the_output(tiny):
+    pdi, img, O, x, y, bkg = tiny
+    v = np.random.default_rng(16).uniform(0.0, 1.0, img.shape).astype(np.float32)
+    h = O.hessian_times_input(y, x, v, 0, NSUB, bkg=bkg)
+    start = np.full(img.shape, 3.0, np.float32)
+    out = O.hessian_times_input(y, x, v, 0, NSUB, bkg=bkg, out=start.copy())
+    assert np.array_equal(out, start + h)
+
+
+def test_negative_forward_projection_aborts(tiny):
+    pdi, img, O, x, y, bkg = tiny
+    v = -np.ones(img.shape, np.float32)
+    out = np.zeros(img.shape, np.float32)
+    with pytest.raises(ValueError):
+        O.hessian_times_input(y, x, v, 0, NSUB, bkg=bkg, out=out)
+    assert not out.any()
+
+
+# ---------------------------------------------------------------------------
+# planted mistakes: (pin that must catch it, source text, replacement)
+ORACLE_SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+MUTANTS = {
+    "ybar_not_squared": (test_exact_product_is_the_derivative_of_the_gradient,
+                         "              den[i] = ybar * ybar;", "              den[i] = ybar;"),
+    "no_background": (test_exact_product_is_the_derivative_of_the_gradient,
+                      "              if (bkg) ybar = ybar + bkg[i];", "              (void)bkg;"),
+    "factors_once": (test_approximate_divides_the_data_by_the_factors_twice,
+                     "                d = d / e;\n                d = d / e;", "                d = d / e;"),
+    "added_not_subtracted": (test_symmetric_and_negative_definite,
+                             "    for (int64_t j = 0; j < nv; ++j) out[j] = out[j] - bp[j];",
+                             "    for (int64_t j = 0; j < nv; ++j) out[j] = out[j] + bp[j];"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MUTANTS))
+def test_planted_mistake_fails_its_pin(tiny, tmp_path, name):
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc is not available")
+    pin, old, new = MUTANTS[name]
+    src = open(os.path.join(ORACLE_SRC, "pmrt_oracle.c")).read()
+    assert src.count(old) == 1, f"mutation site of {name} not found exactly once"
+    (tmp_path / "pmrt_oracle.c").write_text(src.replace(old, new))
+    shutil.copy(os.path.join(ORACLE_SRC, "pmrt_oracle.h"), tmp_path / "pmrt_oracle.h")
+    so = tmp_path / "liborpmrt_mutant.so"
+    subprocess.run(["gcc", "-O2", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-shared",
+                    "-o", str(so), str(tmp_path / "pmrt_oracle.c"), "-lm"], check=True)
+    pdi, img, _, x, y, bkg = tiny
+    O = OraclePlan(pdi, img, RayTracingMatrixParameters(), library=oracle_mod.load_library(str(so)))
+    with pytest.raises(AssertionError):
+        pin((pdi, img, O, x, y, bkg))
