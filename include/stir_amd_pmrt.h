@@ -365,6 +365,24 @@ int mi_relative_difference_prior_dev(mi_ctx* ctx, const float* d_image, const fl
                                      float beta, float gamma, float epsilon, int only_2D, float* d_gradient,
                                      double* d_value, void* stream);
 
+/* The data term's Hessian products on device images (STIR layout),
+ * accumulated into d_output (d_output -= A^T q) for one subset, or all
+ * subsets in one pass with subset = -1:
+ *  approximate = 0: accumulate_sub_Hessian_times_input_without_penalty
+ *    (PoissonLogLikelihoodWithLinearModelForMeanAndProjData.cxx:1066-1218),
+ *    q = divide_and_truncate(y A d_input, (A d_current + b)^2);
+ *  approximate = 1: add_multiplication_with_approximate_sub_Hessian_without_penalty
+ *    (:957-1064), q = divide_and_truncate(A d_input, y / m / m) with m the
+ *    multiplicative factors of mi_obj_set_multiplicative (at least 1e-20;
+ *    none: y), d_current unused.
+ * y and b are the data of mi_obj_set_data as given (end planes not zeroed);
+ * divide_and_truncate works per viewgram (recon_array_functions.cxx:172-266).
+ * A negative value in A d_input is an error (MI_ERR_ARGUMENT) that leaves
+ * d_output unchanged, as the reference aborts. Projection data only (not
+ * list mode); synchronizes the stream. */
+int mi_obj_hessian_times_input_dev(mi_ctx* ctx, int approximate, const float* d_current, const float* d_input,
+                                   int subset, float* d_output, void* stream);
+
 /* Prior Hessian products on device images (STIR layout), accumulated into
  * d_output: accumulate_Hessian_times_input (QuadraticPrior.cxx:604-690,
  * RelativeDifferencePrior.cxx:535-614) of d_input at the estimate d_image,

@@ -32,6 +32,7 @@ EXPORTED_SYMBOLS = (
     "mi_relative_difference_prior_dev", "mi_batch_set_row_groups", "mi_lm_set_events",
     "mi_prior_hessian_times_input_dev", "mi_basic_view_segments_in_subset", "mi_set_image",
     "mi_forward_project_view_segments", "mi_back_project_view_segments", "mi_quadratic_prior_noalloc_dev",
+    "mi_obj_hessian_times_input_dev",
 )
 
 MI_PRIOR_NONE, MI_PRIOR_QUADRATIC, MI_PRIOR_RELATIVE_DIFFERENCE = 0, 1, 2
@@ -153,6 +154,7 @@ def load():
     L.mi_lm_set_events.argtypes = [vp, _i32p, ctypes.c_int64, vp, ctypes.c_float]
     L.mi_prior_hessian_times_input_dev.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float,
                                                    ctypes.c_float, c_int, vp, vp]
+    L.mi_obj_hessian_times_input_dev.argtypes = [vp, c_int, vp, vp, c_int, vp, vp]
     L.mi_basic_view_segments_in_subset.argtypes = [vp, c_int, c_int, _i32p, _i32p, c_int, ctypes.POINTER(c_int)]
     L.mi_set_image.argtypes = [vp, _f32p]
     L.mi_forward_project_view_segments.argtypes = [vp, c_int, _i32p, _i32p, _f32p]

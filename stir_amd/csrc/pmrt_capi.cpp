@@ -184,6 +184,14 @@ struct mi_ctx {
   DBuf<float> img, img2, sino, ratio, stage_img, stage_sino;
   // objective function
   DBuf<float> y, add, mult, ones, vg_small, x;
+  // the measured data and additive term as given, kept when "zero end planes
+  // of segment 0" zeroes y and add (the Hessian products read them unzeroed)
+  DBuf<float> y_raw, add_raw;
+  // the Hessian products: each subset's viewgrams (TOF bins included) and
+  // the flag of a negative forward projection
+  DBuf<int32_t> subset_vgs;
+  std::vector<int> subset_vg_off;
+  DBuf<int32_t> hess_flag;
   std::vector<std::unique_ptr<DBuf<float>>> sens;
   DBuf<int64_t> vg_off;
   DBuf<int32_t> vg_len;
@@ -976,6 +984,7 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   img.release(); img2.release(); sino.release(); ratio.release(); stage_img.release(); stage_sino.release();
   y.release(); add.release(); mult.release(); ones.release(); vg_small.release(); x.release(); sens.clear();
   vg_off.release(); vg_len.release(); partial.release(); prior_partial.release();
+  subset_vgs.release(); subset_vg_off.clear(); hess_flag.release(); y_raw.release(); add_raw.release();
   lors.release(); entry_d.release(); erf_tab.release(); sel_tasks.release();
   tof_fp_rec.release(); tof_bp_rec.release();
   tof_fp_nt = tof_fp_w = tof_bp_w = tof_live_span = 0;
@@ -1437,6 +1446,19 @@ int mi_obj_set_data(mi_ctx* ctx, const float* measured, const float* additive, f
     ctx->add_const = additive_constant;
     ctx->zero_end_planes = zero_end_planes;
     ctx->lm = false;
+    if (zero_end_planes) {
+      ctx->y_raw.alloc(nb);
+      HIPCHECK(hipMemcpyAsync(ctx->y_raw.p, ctx->y.p, nb * sizeof(float), hipMemcpyDeviceToDevice, st));
+      if (ctx->add.p) {
+        ctx->add_raw.alloc(nb);
+        HIPCHECK(hipMemcpyAsync(ctx->add_raw.p, ctx->add.p, nb * sizeof(float), hipMemcpyDeviceToDevice, st));
+      } else {
+        ctx->add_raw.release();
+      }
+    } else {
+      ctx->y_raw.release();
+      ctx->add_raw.release();
+    }
     ctx->ones.alloc(nb);
     mi::launch_fill(ctx->ones.p, nb, 1.F, st);
     if (zero_end_planes) {
@@ -2029,6 +2051,60 @@ int mi_relative_difference_prior_dev(mi_ctx* ctx, const float* d_image, const fl
         mi::launch_sum_partials(ctx->prior_partial.p, mi::prior_num_partials(ctx->stir_prior_image()),
                                 static_cast<double>(beta), d_value, st);
     }
+    HIPCHECK(hipGetLastError());
+  });
+}
+
+int mi_obj_hessian_times_input_dev(mi_ctx* ctx, int approximate, const float* d_current, const float* d_input,
+                                   int subset, float* d_output, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_input || !d_output || (!d_current && !approximate)) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
+    ctx->require_setup();
+    ctx->check_subset(subset);
+    if (ctx->lm) throw ApiError(MI_ERR_STATE, "the Hessian products work on projection data, not list mode");
+    if (!ctx->have_data) throw ApiError(MI_ERR_STATE, "mi_obj_set_data has not been called");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->pick(stream);
+    const int nsub = ctx->plan->num_subsets();
+    const int nvg1 = ctx->plan->num_viewgrams();
+    if (ctx->subset_vg_off.empty()) {
+      // the viewgrams of each subset, every TOF bin (find_basic_viewgram_indices_in_subset
+      // with their related viewgrams)
+      std::vector<int32_t> h;
+      ctx->subset_vg_off.push_back(0);
+      for (int s = 0; s < nsub; ++s) {
+        for (int k = 0; k < ctx->ntof; ++k)
+          for (const auto& vs : ctx->plan->subset_view_segments(s))
+            h.push_back(k * nvg1 + ctx->plan->viewgram_index(vs.second, vs.first));
+        ctx->subset_vg_off.push_back(static_cast<int>(h.size()));
+      }
+      ctx->subset_vgs.alloc(h.size());
+      HIPCHECK(hipMemcpy(ctx->subset_vgs.p, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      ctx->hess_flag.alloc(1);
+    }
+    const int v0 = ctx->subset_vg_off[ctx->subset_begin(subset)], v1 = ctx->subset_vg_off[ctx->subset_end(subset)];
+    // A input and, for the exact product, A current
+    ctx->image_to_device(d_input, ctx->img.p, st);
+    ctx->fp(mi::FP_PLAIN, ctx->img.p, ctx->ratio.p, subset, mi::FpData{}, st);
+    if (!approximate) {
+      ctx->image_to_device(d_current, ctx->img.p, st);
+      ctx->fp(mi::FP_PLAIN, ctx->img.p, ctx->sino.p, subset, mi::FpData{}, st);
+    }
+    HIPCHECK(hipMemsetAsync(ctx->hess_flag.p, 0, sizeof(int32_t), st));
+    const float* y = ctx->y_raw.p ? ctx->y_raw.p : ctx->y.p;
+    const float* add = ctx->add_raw.p ? ctx->add_raw.p : ctx->add.p;
+    mi::launch_hessian_ratio(ctx->subset_vgs.p + v0, v1 - v0, ctx->vg_off.p, ctx->vg_len.p, y, ctx->ratio.p,
+                             ctx->sino.p, add, ctx->add_const, ctx->mult_user.p, approximate, ctx->ratio.p,
+                             ctx->hess_flag.p, st);
+    int32_t negative = 0;
+    HIPCHECK(hipMemcpyAsync(&negative, ctx->hess_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (negative)
+      // ...AndProjData.cxx:1043-1046,1143-1146: the output is left as it was
+      throw ApiError(MI_ERR_ARGUMENT, "forward projection of input contains negatives: the result would be incorrect");
+    ctx->bp(mi::BP_PLAIN, ctx->ratio.p, ctx->img2.p, subset, mi::BpData{}, st);
+    ctx->image_from_device(ctx->img2.p, ctx->stage_img.p, st);
+    mi::launch_subtract(d_output, ctx->stage_img.p, ctx->stage_img.n, st);
     HIPCHECK(hipGetLastError());
   });
 }

@@ -312,6 +312,15 @@ void launch_fill(float* d, int64_t n, float v, hipStream_t stream);
 void launch_viewgram_max(const float* d_proj, const int64_t* d_vg_off, const int32_t* d_vg_len, int nvg, float* d_small,
                          hipStream_t stream);
 void launch_zero_end_planes(float* d_proj, const int64_t* d_off, int count, int run, int stride, hipStream_t stream);
+// the objective's data-term Hessian products (one workgroup per viewgram):
+// q = divide_and_truncate(y * fin, (fcur + b)^2) (exact) or
+// divide_and_truncate(fin, y / e / e) (approximate, e = max(1e-20, d_mult) or
+// 1) over the nvg viewgrams d_vgs lists (the first nvg without a list); d_q
+// may be d_fin; *d_negative set when fin holds a negative value
+void launch_hessian_ratio(const int32_t* d_vgs, int nvg, const int64_t* d_vg_off, const int32_t* d_vg_len,
+                          const float* d_y, const float* d_fin, const float* d_fcur, const float* d_add,
+                          float add_const, const float* d_mult, int approximate, float* d_q, int* d_negative,
+                          hipStream_t stream);
 
 // ---- objective-function helpers (pmrt_objective.hip)
 // QuadraticPrior weights w[dz+1][dy+1][dx+1].

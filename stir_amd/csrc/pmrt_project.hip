@@ -553,6 +553,64 @@ __global__ void __launch_bounds__(256) viewgram_max_kernel(const float* __restri
   }
 }
 
+// Projection data of the objective's data-term Hessian products, one
+// workgroup per viewgram: q = divide_and_truncate(num, den) over the viewgram
+// with the small value 1e-6 times the numerator's maximum
+// (recon_array_functions.cxx:172-266), where
+//   exact (...AndProjData.cxx:1066-1218): num = y * fin, den = (fcur + b)^2;
+//   approximate (:957-1064): num = fin, den = y / e / e with e the bin
+//   efficiency (the multiplicative factor, at least 1e-20; none: den = y).
+// fin = A input, fcur = A current; q may be fin. Workgroup i takes viewgram
+// vgs[i] (viewgram i without a list). *negative is set when fin holds a
+// negative value (the reference aborts then).
+__global__ void __launch_bounds__(256) hessian_ratio_kernel(const int32_t* __restrict__ vgs,
+                                                            const int64_t* __restrict__ off,
+                                                            const int32_t* __restrict__ len, const float* y,
+                                                            const float* fin, const float* fcur, const float* add,
+                                                            float add_const, const float* mult, int approximate,
+                                                            float* q, int* negative) {
+  __shared__ float red[256];
+  const int vg = vgs ? vgs[blockIdx.x] : static_cast<int>(blockIdx.x);
+  const int64_t o = off[vg];
+  const int n = len[vg];
+  float m = -FLT_MAX;
+  bool neg = false;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float t = fin[o + i];
+    neg = neg || t < 0.F;
+    m = fmaxf(m, approximate ? t : y[o + i] * t);
+  }
+  if (neg) *negative = 1;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const float sv = red[0] * 0.000001F;
+  const float small_value = (0.F < sv) ? sv : 0.F;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int64_t j = o + i;
+    const float t = fin[j];
+    float num, den;
+    if (approximate) {
+      num = t;
+      den = y[j];
+      if (mult) {
+        const float e = (1.E-20F < mult[j]) ? mult[j] : 1.E-20F;
+        den = den / e;
+        den = den / e;
+      }
+    } else {
+      num = y[j] * t;
+      const float ybar = fcur[j] + (add ? add[j] : add_const);
+      den = ybar * ybar;
+    }
+    const float max_quotient = 10000.F;
+    q[j] = num <= small_value ? 0.F : (num > max_quotient * den ? max_quotient : num / den);
+  }
+}
+
 // zero the first and last axial position of `count` runs of length `run`
 __global__ void zero_end_planes_kernel(float* proj, const int64_t* off, int count, int run) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -749,6 +807,15 @@ void launch_viewgram_max(const float* d_proj, const int64_t* d_vg_off, const int
                          hipStream_t stream) {
   if (nvg <= 0) return;
   hipLaunchKernelGGL(viewgram_max_kernel, dim3(nvg), dim3(256), 0, stream, d_proj, d_vg_off, d_vg_len, d_small);
+}
+
+void launch_hessian_ratio(const int32_t* d_vgs, int nvg, const int64_t* d_vg_off, const int32_t* d_vg_len,
+                          const float* d_y, const float* d_fin, const float* d_fcur, const float* d_add,
+                          float add_const, const float* d_mult, int approximate, float* d_q, int* d_negative,
+                          hipStream_t stream) {
+  if (nvg <= 0) return;
+  hipLaunchKernelGGL(hessian_ratio_kernel, dim3(nvg), dim3(256), 0, stream, d_vgs, d_vg_off, d_vg_len, d_y, d_fin,
+                     d_fcur, d_add, add_const, d_mult, approximate, d_q, d_negative);
 }
 
 void launch_zero_end_planes(float* d_proj, const int64_t* d_off, int count, int run, int stride, hipStream_t stream) {

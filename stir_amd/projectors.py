@@ -407,6 +407,90 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
                       "mi_obj_log_likelihood")
         return float(v.value)
 
+    # ------------------------------------------------------------------
+    # Hessian products (GeneralisedObjectiveFunction.cxx:253-470). Each
+    # updates ``output`` in place (output -= ...) and returns it; images are
+    # numpy arrays or float32 CUDA tensors (then nothing crosses to the host).
+    def accumulate_sub_Hessian_times_input_without_penalty(self, output, current_image_estimate, input,
+                                                            subset_num: int):
+        """``...AndProjData.cxx:1066-1218``: output -= A^T divide_and_truncate(y A input, (A current + b)^2)."""
+        return self._hessian(output, current_image_estimate, input, subset_num, False, 0.0)
+
+    def add_multiplication_with_approximate_sub_Hessian_without_penalty(self, output, input, subset_num: int):
+        """``...AndProjData.cxx:957-1064``: output -= A^T divide_and_truncate(A input, y / m / m)."""
+        return self._hessian(output, None, input, subset_num, True, 0.0)
+
+    def accumulate_sub_Hessian_times_input(self, output, current_image_estimate, input, subset_num: int):
+        """The subset term and the prior's Hessian product / num_subsets (``:398-425``)."""
+        return self._hessian(output, current_image_estimate, input, subset_num, False, 1.0 / self._num_subsets())
+
+    def add_multiplication_with_approximate_sub_Hessian(self, output, input, subset_num: int):
+        """The subset term and the prior's approximate product / num_subsets (``:280-311``)."""
+        return self._hessian(output, None, input, subset_num, True, 1.0 / self._num_subsets())
+
+    def accumulate_Hessian_times_input_without_penalty(self, output, current_image_estimate, input):
+        """Every subset (``:383-396``), in one pass over the data."""
+        return self._hessian(output, current_image_estimate, input, -1, False, 0.0)
+
+    def add_multiplication_with_approximate_Hessian_without_penalty(self, output, input):
+        """Every subset (``:313-326``), in one pass over the data."""
+        return self._hessian(output, None, input, -1, True, 0.0)
+
+    def accumulate_Hessian_times_input(self, output, current_image_estimate, input):
+        """Every subset and the prior's Hessian product once (``:353-381``)."""
+        return self._hessian(output, current_image_estimate, input, -1, False, 1.0)
+
+    def add_multiplication_with_approximate_Hessian(self, output, input):
+        """Every subset's ``add_multiplication_with_approximate_sub_Hessian`` (``:328-339``):
+        the prior's approximate product / num_subsets subtracted num_subsets times."""
+        return self._hessian(output, None, input, -1, True, 1.0 / self._num_subsets(), self._num_subsets())
+
+    def _num_subsets(self) -> int:
+        return max(1, int(self.projector.num_subsets))
+
+    def _hessian(self, output, current, inp, subset, approximate, prior_scale, prior_repeats=1):
+        import torch
+
+        nvox = self.projector.img.num_voxels
+
+        def dev(a, what):
+            if _is_torch(a):
+                if not a.is_cuda or a.dtype != torch.float32 or a.numel() != nvox:
+                    raise ValueError(f"{what} must be a float32 CUDA tensor with the image's number of voxels")
+                return a.contiguous()
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            if a.size != nvox:
+                raise ValueError(f"{what} has the wrong number of voxels")
+            return torch.from_numpy(a.reshape(-1)).cuda()
+
+        if not _is_torch(output) and not (isinstance(output, np.ndarray) and output.dtype == np.float32
+                                          and output.flags.c_contiguous):
+            raise ValueError("output must be a contiguous float32 array (it is updated in place)")
+        v = dev(inp, "input")
+        x = None if approximate else dev(current, "current image estimate")
+        out = dev(output, "output")
+        if _is_torch(output) and out.data_ptr() != output.data_ptr():
+            raise ValueError("output must be contiguous (it is updated in place)")
+        st = torch.cuda.current_stream(v.device).cuda_stream
+        _native.check(_native.load().mi_obj_hessian_times_input_dev(
+            self._ctx.handle, int(approximate), None if x is None else ctypes.c_void_p(x.data_ptr()),
+            ctypes.c_void_p(v.data_ptr()), int(subset), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(st)),
+            "mi_obj_hessian_times_input_dev")
+        prior = getattr(self, "prior", None)
+        if prior_scale and prior is not None and prior.penalisation_factor != 0.0:
+            tmp = torch.zeros_like(v)
+            prior._loss(self.projector).hessian_times_input(x if x is not None else v, v, tmp, approximate)
+            tmp = tmp.reshape(-1)
+            for _ in range(prior_repeats):
+                if prior_scale == 1.0:
+                    out.sub_(tmp)
+                else:
+                    out.sub_(tmp / self._num_subsets())
+        if _is_torch(output):
+            return output
+        output.reshape(-1)[:] = out.cpu().numpy()
+        return output
+
 
 class PoissonLogLikelihoodWithLinearModelForMeanAndListModeDataWithProjMatrixByBin(
         PoissonLogLikelihoodWithLinearModelForMeanAndProjData):
