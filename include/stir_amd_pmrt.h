@@ -120,6 +120,10 @@ typedef struct {
   int32_t tof_fp_window;
   int32_t tof_bp_window;
   int32_t tof_live_span;
+  /* 1 when back projections run over 2 x 2 column blocks (MI_TUNE_BP_QUAD),
+   * and the number of merged entries of that table */
+  int32_t bp_quad;
+  int64_t num_quad_entries;
 } mi_info;
 
 typedef struct mi_ctx mi_ctx;
@@ -485,10 +489,25 @@ int mi_batch_synchronize(mi_batch* batch);
  * same bit for bit in all three.
  * These take effect at the next mi_set_up (they decide the table's layout);
  * the sums of one setting are deterministic, other settings agree to float
- * rounding. */
+ * rounding.
+ * MI_TUNE_BP_QUAD 1 builds, at mi_set_up, a second table for non-TOF
+ * one-plane tables that merges the column lists of 2 x 2 blocks of image
+ * columns, and back-projects one block per wave (each projection-data run
+ * gathered once for the block's four columns: 0.58 of the gathers at mMR
+ * size, measured no faster); 0 (default) keeps the column kernel. Between a
+ * set-up with 1 and the next, 0 and 1 switch the kernel at the next launch.
+ * The two give the same sums bit for bit;
+ * MI_TUNE_BP_QUAD_UNROLL (4, 6 or 8, default 8) sets the merged entries
+ * whose gathers each wave keeps in flight.
+ * MI_TUNE_BP_SPLIT_SUBSETS 1 back-projects several subsets (a plain back
+ * projection of all subsets) in one launch per subset, each continuing the
+ * sums the previous one left in the image (the same sums bit for bit as one
+ * launch; a working set of one subset's projection data; 2 % faster at mMR
+ * size); 0 (default) in one launch. */
 enum { MI_TUNE_BP_TILE_X = 1, MI_TUNE_BP_TILE_Y = 2, MI_TUNE_BP_XCD_REMAP = 3, MI_TUNE_BP_SLOT_SKIP = 4,
        MI_TUNE_FP_XCD_CHUNK = 5, MI_TUNE_BP_SLAB = 6, MI_TUNE_SLAB_TILE_X = 7, MI_TUNE_SLAB_TILE_Y = 8,
-       MI_TUNE_SLAB_COLUMNS_PER_WAVE = 9, MI_TUNE_SLAB_LDS_KB = 10, MI_TUNE_TOF_ROWS = 11 };
+       MI_TUNE_SLAB_COLUMNS_PER_WAVE = 9, MI_TUNE_SLAB_LDS_KB = 10, MI_TUNE_TOF_ROWS = 11, MI_TUNE_BP_QUAD = 12,
+       MI_TUNE_BP_QUAD_UNROLL = 13, MI_TUNE_BP_SPLIT_SUBSETS = 14 };
 int mi_set_tuning(mi_ctx* ctx, int param, int value);
 
 /* ---- diagnostics ----------------------------------------------------------- */

@@ -53,6 +53,9 @@ class PriorStruct(ctypes.Structure):
 MI_TUNE_BP_TILE_X, MI_TUNE_BP_TILE_Y, MI_TUNE_BP_XCD_REMAP, MI_TUNE_BP_SLOT_SKIP, MI_TUNE_FP_XCD_CHUNK = 1, 2, 3, 4, 5
 MI_TUNE_BP_SLAB, MI_TUNE_SLAB_TILE_X, MI_TUNE_SLAB_TILE_Y, MI_TUNE_SLAB_COLUMNS_PER_WAVE, MI_TUNE_SLAB_LDS_KB = 6, 7, 8, 9, 10
 MI_TUNE_TOF_ROWS = 11
+MI_TUNE_BP_QUAD = 12
+MI_TUNE_BP_QUAD_UNROLL = 13
+MI_TUNE_BP_SPLIT_SUBSETS = 14
 
 
 class NativeError(RuntimeError):
@@ -84,6 +87,8 @@ class InfoStruct(ctypes.Structure):
         ("tof_fp_window", ctypes.c_int32),
         ("tof_bp_window", ctypes.c_int32),
         ("tof_live_span", ctypes.c_int32),
+        ("bp_quad", ctypes.c_int32),
+        ("num_quad_entries", ctypes.c_int64),
     ]
 
 

@@ -130,6 +130,15 @@ struct mi_ctx {
     slab_view_of.release(); slab_nviews.release(); slab_tiles.release(); slab_segs.release();
     slab = mi::SlabArgs{};
   }
+  // back projection over 2 x 2 column blocks (pmrt_quad.hip) for one-plane
+  // tables: MI_TUNE_BP_QUAD (1: build the block table at set-up and use it;
+  // default 0: it halves the gathers but measured no faster at mMR size)
+  int quad_enable = 0;
+  DBuf<int64_t> quad_off;  // per (subset, block), size nsub * blocks + 1
+  DBuf<mi::QuadEntry> quad_ent;
+  int64_t quad_entries = 0;
+  bool use_quad() const { return quad_enable && quad_ent.p != nullptr; }
+  void build_quad_table(const uint32_t* d_perm, int64_t n, const int64_t* d_famoff);
   std::vector<mi::Family> h_fams;
   std::unordered_map<int64_t, int> fam_lookup;  // (seg,view,tang,a0) -> family
   int64_t num_basic_elements = 0;
@@ -342,6 +351,9 @@ struct mi_ctx {
     else if (use_slab)
       mi::launch_back_project_slab(mode, slab, subset_begin(subset), subset_end(subset), d_proj, d_img, dims, data,
                                    st);
+    else if (use_quad())
+      mi::launch_back_project_quad(mode, quad_off.p, quad_ent.p, subset_begin(subset), subset_end(subset), d_proj,
+                                   d_img, dims, data, bp_tuning, st, block_bins);
     else
       mi::launch_back_project(mode, uniform_ppa, list_off.p, entries.p, subset_begin(subset), subset_end(subset), d_proj,
                               d_img, dims, data, bp_tuning, st, block_bins);
@@ -364,6 +376,8 @@ struct mi_ctx {
     }
     if (ntof > 1 && tof_bp_w > 0)
       bp_bytes = img_bytes + proj_bytes + static_cast<int64_t>(tof_bp_rec.bytes() + list_off.bytes());
+    else if (use_quad())
+      bp_bytes = img_bytes + proj_bytes + static_cast<int64_t>(quad_ent.bytes() + quad_off.bytes());
     else
       bp_bytes = img_bytes + proj_bytes +
                  static_cast<int64_t>(entries.bytes() + list_off.bytes() + entry_d.bytes() + slab_hdr.bytes() +
@@ -693,8 +707,73 @@ void mi_ctx::build_table() {
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(stream));
+    tmp.release();
+    raw.release();
+    raw_d.release();
+    keys_in.release();
+    keys_out.release();
+    vals_in.release();
+    build_quad_table(vals_out.p, n, d_famoff.p);
   }
   build_tof_rows(n);
+}
+
+// The 2 x 2 column-block table (pmrt_quad.hip) of a one-plane table without
+// TOF, from the sorted column table (d_perm: each entry's generation index),
+// when it fits in three quarters of the free device memory; otherwise, or if
+// the merge finds a column whose element order it cannot keep, the column
+// kernel stays in use.
+void mi_ctx::build_quad_table(const uint32_t* d_perm, int64_t n, const int64_t* d_famoff) {
+  quad_off.release();
+  quad_ent.release();
+  if (!quad_enable || ntof != 1 || uniform_ppa != 1 || n <= 0 || h_fams.empty()) return;
+  const int nsub = plan->num_subsets();
+  const int nqx = (dims.nx + 1) / 2, nqy = (dims.ny + 1) / 2, nq = nqx * nqy;
+  const int64_t nlists = static_cast<int64_t>(nsub) * nq;
+  if (nlists >= (int64_t{1} << 31) || h_fams.size() >= (size_t{1} << 40)) return;
+  size_t free_b = 0, total_b = 0;
+  HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+  // keys, merged entries (at most one per column entry), counts and offsets
+  const int64_t need = 8 * n + int64_t(sizeof(mi::QuadEntry)) * n + 12 * nlists;
+  if (need > static_cast<int64_t>(free_b / 4 * 3)) return;
+  DBuf<uint64_t> key;
+  key.alloc(n);
+  mi::launch_quad_keys(d_perm, n, d_famoff, static_cast<int>(h_fams.size()), fams.p, pool.p, key.p, stream);
+  DBuf<int32_t> counts;
+  counts.alloc(nlists);
+  DBuf<int> bad;
+  bad.alloc(1);
+  HIPCHECK(hipMemsetAsync(bad.p, 0, sizeof(int), stream));
+  mi::launch_quad_merge(true, list_off.p, entries.p, key.p, dims, nqx, nq, nsub, nullptr, counts.p, nullptr, bad.p,
+                        stream);
+  quad_off.alloc(nlists + 1);
+  {
+    size_t tb = 0;
+    HIPCHECK(mi::exclusive_scan_counts(nullptr, tb, counts.p, quad_off.p, static_cast<int>(nlists), stream));
+    DBuf<char> tmp;
+    tmp.alloc(std::max<size_t>(tb, 1));
+    HIPCHECK(mi::exclusive_scan_counts(tmp.p, tb, counts.p, quad_off.p, static_cast<int>(nlists), stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  int64_t last_off = 0;
+  int32_t last_cnt = 0;
+  HIPCHECK(hipMemcpy(&last_off, quad_off.p + nlists - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(&last_cnt, counts.p + nlists - 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int64_t total = last_off + last_cnt;
+  HIPCHECK(hipMemcpy(quad_off.p + nlists, &total, sizeof(int64_t), hipMemcpyHostToDevice));
+  counts.release();
+  quad_ent.alloc(std::max<int64_t>(total, 1));
+  mi::launch_quad_merge(false, list_off.p, entries.p, key.p, dims, nqx, nq, nsub, quad_off.p, nullptr, quad_ent.p,
+                        bad.p, stream);
+  HIPCHECK(hipGetLastError());
+  int h_bad = 0;
+  HIPCHECK(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  if (h_bad) {
+    quad_off.release();
+    quad_ent.release();
+  }
+  quad_entries = total;
 }
 
 // Stored TOF rows when every family has a row of its own, the planes per
@@ -985,6 +1064,7 @@ void mi_ctx::set_up(const mi_geometry& g, int nsub) {
   y.release(); add.release(); mult.release(); ones.release(); vg_small.release(); x.release(); sens.clear();
   vg_off.release(); vg_len.release(); partial.release(); prior_partial.release();
   subset_vgs.release(); subset_vg_off.clear(); hess_flag.release(); y_raw.release(); add_raw.release();
+  quad_off.release(); quad_ent.release();
   lors.release(); entry_d.release(); erf_tab.release(); sel_tasks.release();
   tof_fp_rec.release(); tof_bp_rec.release();
   tof_fp_nt = tof_fp_w = tof_bp_w = tof_live_span = 0;
@@ -1148,6 +1228,8 @@ int mi_get_info(mi_ctx* ctx, mi_info* out) {
     out->tof_fp_window = ctx->tof_fp_w;
     out->tof_bp_window = ctx->tof_bp_w;
     out->tof_live_span = ctx->tof_live_span;
+    out->bp_quad = ctx->use_quad() && !ctx->use_slab && ctx->ntof == 1 ? 1 : 0;
+    out->num_quad_entries = ctx->quad_ent.p ? ctx->quad_entries : 0;
   });
 }
 
@@ -2171,6 +2253,19 @@ int mi_set_tuning(mi_ctx* ctx, int param, int value) {
       case MI_TUNE_TOF_ROWS:
         if (value < 0 || value > 2) throw ApiError(MI_ERR_ARGUMENT, "TOF row mode must be 0, 1 or 2");
         ctx->tof_rows_mode = value;
+        break;
+      case MI_TUNE_BP_QUAD:
+        if (value < 0 || value > 1) throw ApiError(MI_ERR_ARGUMENT, "the block back projection switch must be 0 or 1");
+        ctx->quad_enable = value;
+        break;
+      case MI_TUNE_BP_QUAD_UNROLL:
+        if (value != 4 && value != 6 && value != 8)
+          throw ApiError(MI_ERR_ARGUMENT, "the block back projection keeps 4, 6 or 8 entries in flight");
+        ctx->bp_tuning.quad_unroll = value;
+        break;
+      case MI_TUNE_BP_SPLIT_SUBSETS:
+        if (value < 0 || value > 1) throw ApiError(MI_ERR_ARGUMENT, "the subset split switch must be 0 or 1");
+        ctx->bp_tuning.split_subsets = value;
         break;
       default: throw ApiError(MI_ERR_ARGUMENT, "unknown tuning parameter");
     }

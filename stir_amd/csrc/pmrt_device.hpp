@@ -30,6 +30,19 @@ struct BpEntry {
 };
 static_assert(sizeof(BpEntry) == 16, "BpEntry layout");
 
+// One entry of the 2 x 2 column-block back-projection table (pmrt_quad.hip):
+// a family's run as in BpEntry and the weights of the block's columns
+// (x0, y0), (x0 + 1, y0), (x0, y0 + 1), (x0 + 1, y0 + 1), 0 for a column
+// without an element of the family at that plane.
+struct QuadEntry {
+  int32_t sino_base;
+  int32_t k;
+  int32_t nax_ppa;
+  int32_t pad;
+  float w[4];
+};
+static_assert(sizeof(QuadEntry) == 32, "QuadEntry layout");
+
 struct ImageDims {
   int32_t nz, ny, nx;
   int32_t min_z, min_y, min_x;
@@ -123,6 +136,8 @@ struct BpTuning {
   int tile_y = 64;
   int xcd_remap = 64;  // 0: hardware order, 1: contiguous ranges per XCD, k > 1: chunks of k blocks dealt to the XCDs
   int slot_skip = 0;
+  int quad_unroll = 8;  // 2 x 2 block kernel: merged entries in flight per wave (4, 6 or 8)
+  int split_subsets = 0;  // plain projection of several subsets: 1 one launch per subset
 };
 
 // ppa: planes per axial position shared by all families (1 or 2 select a
@@ -306,6 +321,18 @@ void launch_iota(uint32_t* d_vals, int64_t n, hipStream_t stream);
 void launch_count_nonzeros(const Family* d_fams, int nfam, const int2* d_pool, const ImageDims& dims,
                            unsigned long long* d_total, hipStream_t stream);
 void launch_gather_entries(const BpEntry* d_src, const uint32_t* d_idx, BpEntry* d_dst, int64_t n, hipStream_t stream);
+// ---- 2 x 2 column-block back projection (pmrt_quad.hip)
+// merge keys of the sorted column table (d_perm: generation index of each entry)
+void launch_quad_keys(const uint32_t* d_perm, int64_t n, const int64_t* d_famoff, int nfam, const Family* d_fams,
+                      const int2* d_pool, uint64_t* d_key, hipStream_t stream);
+// per (subset, block): count (d_qcount) or write (from d_qoff) the merged entries
+void launch_quad_merge(bool count, const int64_t* d_list_off, const BpEntry* d_ent, const uint64_t* d_key,
+                       const ImageDims& dims, int nqx, int nq, int nsub, const int64_t* d_qoff, int32_t* d_qcount,
+                       QuadEntry* d_qent, int* d_bad, hipStream_t stream);
+// one-plane tables only; tiles of tune.tile_x/2 x tune.tile_y/2 blocks
+void launch_back_project_quad(BpMode mode, const int64_t* d_qoff, const QuadEntry* d_qent, int subset_begin,
+                              int subset_end, const float* d_proj, float* d_img, const ImageDims& dims,
+                              const BpData& data, const BpTuning& tune, hipStream_t stream, int64_t proj_values);
 
 // ---- small utilities (pmrt_project.hip)
 void launch_fill(float* d, int64_t n, float v, hipStream_t stream);

@@ -172,14 +172,20 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) fp_kernel(const FpTask*
 // Back projection of one image column (wave-uniform col) over subsets [sb, se).
 // LO32: the projection data lie inside one 4 GB-aligned window (a run's
 // buffer start is one 32-bit add).
+// accumulate (BP_PLAIN): the sums start from the image's values (the sums of
+// the subsets before sb, left by the previous launch), so a projection of
+// several subsets in one launch per subset adds the same terms in the same
+// order as one launch.
 template <int MODE, int PPA, int NSLOT, bool SKIP, bool LO32 = false>
 __device__ inline void bp_column(int col, cptr<int64_t> list_off, cptr<BpEntry> ent, int sb, int se, cptr<float> proj,
-                                 float* __restrict__ img, const ImageDims& d, const BpData& data, int lane) {
+                                 float* __restrict__ img, const ImageDims& d, const BpData& data, int lane,
+                                 bool accumulate = false) {
   float acc[NSLOT];
   int zi4[NSLOT];  // 4 * plane of each slot
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) {
-    acc[q] = 0.F;
+    const int z = lane + kWave * q;
+    acc[q] = accumulate && z < d.nzp ? as_global(img)[static_cast<int64_t>(col) * d.nzp + z] : 0.F;
     zi4[q] = 4 * (d.min_z + lane + kWave * q);
     asm volatile("" : "+v"(zi4[q]));  // opaque: keeps zi4 - 4k one subtraction per entry
   }
@@ -429,7 +435,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t
                                                                    const BpEntry* __restrict__ ent, int sb, int se,
                                                                    const float* __restrict__ proj,
                                                                    float* __restrict__ img, ImageDims d, int nwork,
-                                                                   int2 tile, int remap, BpData data) {
+                                                                   int2 tile, int remap, BpData data, int accumulate) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
   const int b = remap == 1 ? xcd_remap(blockIdx.x, gridDim.x)
                 : remap > 1  ? xcd_chunk_remap(blockIdx.x, remap)
@@ -445,7 +451,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock) bp_kernel(const int64_t
                                slots + wave * kWave);
   } else
     bp_column<MODE, PPA, NSLOT, SKIP, LO32>(col, as_const(list_off), as_const(ent), sb, se, as_const(proj), img, d,
-                                            data, lane);
+                                            data, lane, accumulate != 0);
 }
 
 // ---- batches: one work item per wave, the problem's descriptor read
@@ -713,13 +719,20 @@ void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const 
   // (the one-plane kernel forms its run buffers with 32-bit adds then)
   const uint64_t first = reinterpret_cast<uint64_t>(d_proj);
   const bool lo32 = proj_values > 0 && (first >> 32) == ((first + static_cast<uint64_t>(proj_values) * 4 - 1) >> 32);
+  // a plain projection of several subsets: one launch per subset, each
+  // continuing the sums in the image (a working set of one subset's
+  // projection data), except for the two-plane parity kernel
+  const bool split = tune.split_subsets && mode == BP_PLAIN && subset_end - subset_begin > 1 && !(ppa == 2 && nslot == 1);
+  const int nlaunch = split ? subset_end - subset_begin : 1;
+  for (int li = 0; li < nlaunch; ++li) {
+  const int sb = split ? subset_begin + li : subset_begin, se = split ? sb + 1 : subset_end, acc = split && li > 0;
 #define MI_BP_LAUNCH(M, P, S, K)                                                                                    \
   if (lo32)                                                                                                         \
-    hipLaunchKernelGGL((bp_kernel<M, P, S, K, true>), grid, block, 0, stream, d_list_off, d_entries, subset_begin, \
-                       subset_end, d_proj, d_img, dims, nwork, tile, tune.xcd_remap, data);                         \
+    hipLaunchKernelGGL((bp_kernel<M, P, S, K, true>), grid, block, 0, stream, d_list_off, d_entries, sb, se,       \
+                       d_proj, d_img, dims, nwork, tile, tune.xcd_remap, data, acc);                                 \
   else                                                                                                              \
-    hipLaunchKernelGGL((bp_kernel<M, P, S, K, false>), grid, block, 0, stream, d_list_off, d_entries, subset_begin, \
-                       subset_end, d_proj, d_img, dims, nwork, tile, tune.xcd_remap, data)
+    hipLaunchKernelGGL((bp_kernel<M, P, S, K, false>), grid, block, 0, stream, d_list_off, d_entries, sb, se,      \
+                       d_proj, d_img, dims, nwork, tile, tune.xcd_remap, data, acc)
 #define MI_BP_SKIP(M, P, S)             \
   if (tune.slot_skip) MI_BP_LAUNCH(M, P, S, true); \
   else MI_BP_LAUNCH(M, P, S, false);
@@ -738,6 +751,7 @@ void launch_back_project(BpMode mode, int ppa, const int64_t* d_list_off, const 
     if (ppa == 1) { MI_BP_SLOTS(BP_OSEM_UPDATE, 1) }
     else if (ppa == 2) { MI_BP_SLOTS(BP_OSEM_UPDATE, 2) }
     else { MI_BP_SLOTS(BP_OSEM_UPDATE, 0) }
+  }
   }
 #undef MI_BP_SLOTS
 #undef MI_BP_SKIP

@@ -29,7 +29,7 @@ import pytest
 import torch
 
 from oracle.oracle import OraclePlan, quadratic_prior
-from stir_amd import phantoms
+from stir_amd import _native, phantoms
 from stir_amd.geometry import RayTracingMatrixParameters, config1
 from stir_amd.projectors import (OSMAPOSLReconstruction, PoissonLogLikelihoodWithLinearModelForMeanAndProjData,
                                  ProjectorByBinPairMI355X, ProjMatrixByBinUsingRayTracing)
@@ -122,6 +122,36 @@ def test_subset_projections_match_oracle(mmr):
         rel = float((np.abs(bgot - bref)[m] / bref[m]).max())
         print(f"back subset {s}: max err {err:.2e} of max, {rel:.2e} relative", flush=True)
         assert err <= 1e-5 and rel <= 1e-5
+
+
+def test_block_back_projection_bit_identical(mmr):
+    """The back projection over 2 x 2 column blocks (MI_TUNE_BP_QUAD) and the
+    one in one launch per subset (MI_TUNE_BP_SPLIT_SUBSETS) equal the default
+    (column kernel, one launch for all subsets) bit for bit."""
+    pdi, img, pm, P, O = mmr
+    assert P.info().bp_quad == 0
+    y = np.random.default_rng(23).uniform(0.0, 2.0, pdi.num_bins).astype(np.float32)
+    ref = {s: P.back_project(y, subset=s) for s in (7, -1)}
+    ctx = P.context
+    try:
+        ctx.set_tuning(_native.MI_TUNE_BP_SPLIT_SUBSETS, 1)
+        assert np.array_equal(P.back_project(y).view(np.uint32), ref[-1].view(np.uint32))
+    finally:
+        ctx.set_tuning(_native.MI_TUNE_BP_SPLIT_SUBSETS, 0)
+    Q = ProjectorByBinPairMI355X(ProjMatrixByBinUsingRayTracing(pm))
+    Q.context.set_tuning(_native.MI_TUNE_BP_QUAD, 1)
+    Q.set_up(pdi, img, num_subsets=NSUB)
+    try:
+        info = Q.info()
+        assert info.bp_quad == 1
+        print(f"mMR: {info.num_table_entries} column entries, {info.num_quad_entries} block entries "
+              f"({info.num_quad_entries / info.num_table_entries:.3f})")
+        for s in (7, -1):
+            same = np.mean(Q.back_project(y, subset=s).view(np.uint32) == ref[s].view(np.uint32))
+            print(f"subset {s}: bit-identical voxels {same:.6f}", flush=True)
+            assert same == 1.0
+    finally:
+        Q.close()
 
 
 def test_adjoint_identity(mmr):

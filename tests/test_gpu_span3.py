@@ -3,7 +3,8 @@
 oracle: the kernels' range-checked run and column reads (a plane an entry
 does not reach reads 0).
 
-Scanners: 8 rings x 192 crystals (15-plane image, one z slot per lane) and
+Scanners: 8 rings x 192 crystals (15-plane image, one z slot per lane; a
+128 x 128 and a 127 x 127 image) and
 40 rings x 64 crystals (79-plane image, two z slots per lane), span 3, MRD 4;
 the 8-ring scanner with MRD 5 (outer segments of one ring difference, 2
 planes per axial position: a mixed table); 2 mm voxels. Bars: forward projection within 1e-5 relative (bit-exact
@@ -13,6 +14,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import OraclePlan
+from stir_amd import _native
 from stir_amd.geometry import ProjDataInfoCylindrical, RayTracingMatrixParameters, Scanner, VoxelsOnCartesianGrid
 from stir_amd.projectors import (OSMAPOSLReconstruction, PoissonLogLikelihoodWithLinearModelForMeanAndProjData,
                                  ProjectorByBinPairMI355X, ProjMatrixByBinUsingRayTracing)
@@ -29,6 +31,9 @@ CASES = {
     # MRD 5: the outer segments hold ring difference 5 only (2 planes per axial position)
     "8rings_mixed": (Scanner.cylindrical(8, 192, 300.0, 4.0, name="cyl8x192"), 96, 128,
                      VoxelsOnCartesianGrid(15, 128, 128, 2.0, 2.0, 2.0), 5, 0),
+    # an odd number of columns in x and y: the last 2 x 2 blocks hold one column of each pair
+    "8rings_odd": (Scanner.cylindrical(8, 192, 300.0, 4.0, name="cyl8x192"), 96, 128,
+                   VoxelsOnCartesianGrid(15, 127, 127, 2.0, 2.0, 2.0), 4, 1),
 }
 
 
@@ -129,3 +134,44 @@ def test_tof_span3(mrd, ppa):
         assert rel <= 1e-4
     finally:
         P.close()
+
+
+def test_block_back_projection_bit_identical(case):
+    """With MI_TUNE_BP_QUAD one-plane tables back-project over 2 x 2 column
+    blocks; the sums equal the column kernel's bit for bit (plain, in one
+    launch and in one launch per subset, and OSEM-update epilogues)."""
+    name, pdi, img, P, O = case
+    assert P.info().bp_quad == 0  # off by default
+    Q = ProjectorByBinPairMI355X(ProjMatrixByBinUsingRayTracing(RayTracingMatrixParameters()))
+    ctx = Q.context
+    ctx.set_tuning(_native.MI_TUNE_BP_QUAD, 1)
+    Q.set_up(pdi, img, num_subsets=4)
+    try:
+        info = Q.info()
+        assert info.bp_quad == (1 if CASES[name][5] == 1 else 0)
+        if not info.bp_quad:
+            return
+        assert 0 < info.num_quad_entries < info.num_table_entries
+        print(f"{name}: {info.num_table_entries} column entries, {info.num_quad_entries} block entries "
+              f"({info.num_quad_entries / info.num_table_entries:.3f})")
+        y = np.random.default_rng(13).uniform(0.0, 2.0, pdi.num_bins).astype(np.float32)
+        for subset in (1, -1):
+            for split in (1, 0):
+                ctx.set_tuning(_native.MI_TUNE_BP_SPLIT_SUBSETS, split)
+                blk = Q.back_project(y, subset=subset)
+                ctx.set_tuning(_native.MI_TUNE_BP_QUAD, 0)
+                col = Q.back_project(y, subset=subset)
+                ctx.set_tuning(_native.MI_TUNE_BP_QUAD, 1)
+                assert np.array_equal(blk.view(np.uint32), col.view(np.uint32)), (subset, split)
+                assert np.array_equal(col.view(np.uint32), P.back_project(y, subset=subset).view(np.uint32))
+        ctx.set_tuning(_native.MI_TUNE_BP_SPLIT_SUBSETS, 0)
+        meas = np.random.default_rng(14).poisson(Q.forward_project(np.ones(img.shape, np.float32)) + 0.1)
+        obj = PoissonLogLikelihoodWithLinearModelForMeanAndProjData(Q)
+        obj.set_up(meas.astype(np.float32), 0.1)
+        outs = []
+        for quad in (1, 0):
+            ctx.set_tuning(_native.MI_TUNE_BP_QUAD, quad)
+            outs.append(OSMAPOSLReconstruction(obj, num_subiterations=2).reconstruct(np.ones(img.shape, np.float32)))
+        assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    finally:
+        Q.close()
