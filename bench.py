@@ -89,6 +89,10 @@ def main():
     ap.add_argument("--radius", type=float, default=328.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB read + 256 MiB write element-wise kernel first (known traffic for rocprof counters)")
+    ap.add_argument("--bp-quad", action="store_true",
+                    help="pair: back-project over 2 x 2 column blocks (MI_TUNE_BP_QUAD; same sums, for comparison)")
+    ap.add_argument("--bp-split", action="store_true",
+                    help="pair: back-project all subsets in one launch per subset (MI_TUNE_BP_SPLIT_SUBSETS)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "--gpus must be 1",
@@ -98,7 +102,7 @@ def main():
     import numpy as np
     import torch
 
-    from stir_amd import phantoms
+    from stir_amd import _native, phantoms
     from stir_amd.geometry import RayTracingMatrixParameters, config1
     from stir_amd.projectors import (OSMAPOSLReconstruction, PoissonLogLikelihoodWithLinearModelForMeanAndProjData,
                                      ProjectorByBinPairMI355X, ProjMatrixByBinUsingRayTracing)
@@ -118,6 +122,10 @@ def main():
     log(f"set-up: {pdi.num_bins} bins, image {img.shape}, {nsub} subsets")
     t0 = time.time()
     pair = ProjectorByBinPairMI355X(ProjMatrixByBinUsingRayTracing(pm))
+    if args.bp_quad:
+        pair.context.set_tuning(_native.MI_TUNE_BP_QUAD, 1)
+    if args.bp_split:
+        pair.context.set_tuning(_native.MI_TUNE_BP_SPLIT_SUBSETS, 1)
     pair.set_up(pdi, img, num_subsets=nsub)
     torch.cuda.synchronize()
     setup_s = time.time() - t0
@@ -161,7 +169,7 @@ def main():
     nvox = int(img.num_voxels)
     dominant = "bp" if bp_avg >= fp_avg else "fp"
     dom_ms = bp_avg if dominant == "bp" else fp_avg
-    kernel = "bp_kernel" if dominant == "bp" else "fp_kernel"
+    kernel = ("bp_quad_kernel" if args.bp_quad else "bp_kernel") if dominant == "bp" else "fp_kernel"
     # HBM bytes per launch under this design: the matrix data the kernel
     # streams (back: table + offsets; forward: rows per task + records) plus
     # one read of the input and one write of the output (mi_info)
