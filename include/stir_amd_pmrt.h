@@ -382,8 +382,13 @@ int mi_relative_difference_prior_dev(mi_ctx* ctx, const float* d_image, const fl
  * y and b are the data of mi_obj_set_data as given (end planes not zeroed);
  * divide_and_truncate works per viewgram (recon_array_functions.cxx:172-266).
  * A negative value in A d_input is an error (MI_ERR_ARGUMENT) that leaves
- * d_output unchanged, as the reference aborts. Projection data only (not
- * list mode); synchronizes the stream. */
+ * d_output unchanged, as the reference aborts; synchronizes the stream.
+ * List-mode data (mi_lm_set_events): approximate = 0 only, the list-mode
+ * objective's accumulate_sub_Hessian_times_input_without_penalty
+ * (...ListModeDataWithProjMatrixByBin.cxx:790-822,963-1003): every event of
+ * the subset with fwd = A_e d_current + b adds
+ * A_e^T (-1 / fwd^2 * A_e d_input) to d_output, except events with
+ * 1 > 1e4 fwd; no negativity check, no synchronization. */
 int mi_obj_hessian_times_input_dev(mi_ctx* ctx, int approximate, const float* d_current, const float* d_input,
                                    int subset, float* d_output, void* stream);
 

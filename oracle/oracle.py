@@ -92,6 +92,7 @@ def load_library(path: str):
     L.or_lm_gradient.restype = ctypes.c_double
     L.or_lm_gradient.argtypes = [vp, _i32p, ctypes.c_int64, vp, _f32p, ctypes.c_int, ctypes.c_int, vp]
     L.or_lm_osem.argtypes = [vp, _i32p, ctypes.c_int64, vp, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.or_lm_hessian.argtypes = [vp, _i32p, ctypes.c_int64, vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, _f32p]
     L.or_prior_hessian_times_input.argtypes = [ctypes.c_int, _f32p, _f32p, vp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                                ctypes.c_float, _f32p]
@@ -307,6 +308,18 @@ class OraclePlan:
                                  x.reshape(-1), subset, num_subsets,
                                  None if g is None else g.ctypes.data_as(ctypes.c_void_p))
         return float(v), g
+
+    def lm_hessian_times_input(self, events, x, rhs, subset=0, num_subsets=1, additive=None, out=None):
+        """out + the list-mode objective's subset Hessian times rhs at x (LM_Hessian event by event)."""
+        ev = np.ascontiguousarray(events, dtype=np.int32).reshape(-1, 5)
+        x = self._img(x); rhs = self._img(rhs)
+        add = None if additive is None else self._sino(additive)
+        if out is None:
+            out = np.zeros(self.img.shape, np.float32)
+        assert out.dtype == np.float32 and out.flags.c_contiguous
+        self._L.or_lm_hessian(self._p, ev, ev.shape[0], None if add is None else add.ctypes.data_as(ctypes.c_void_p),
+                              x.reshape(-1), rhs.reshape(-1), subset, num_subsets, out.reshape(-1))
+        return out
 
     def lm_osem(self, events, x0, num_subsets, num_subiterations, additive=None, start_subset=0):
         ev = np.ascontiguousarray(events, dtype=np.int32).reshape(-1, 5)

@@ -1810,6 +1810,38 @@ double or_lm_gradient(const or_plan* p, const int32_t* ev, int64_t n, const floa
   return value;
 }
 
+/* LM_Hessian over the subset's events, accumulated into out
+ * (actual_accumulate_sub_Hessian_times_input_without_penalty,
+ * ...ListModeDataWithProjMatrixByBin.cxx:790-822,845-863,963-1003): per event
+ * (bin value 1), fwd = row . x + the additive term; an event with
+ * 1 > 1e4 fwd is skipped; r = row . rhs, skipped when 0; otherwise
+ * row^T (-1 / (fwd * fwd) * r) is added to out. */
+void or_lm_hessian(const or_plan* p, const int32_t* ev, int64_t n, const float* add, const float* x,
+                   const float* rhs, int subset, int nsub, float* out)
+{
+  const float max_quotient = 10000.F;
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* e = ev + 5 * i;
+    if (!lm_event_in_range(p, e)) continue;
+    const int seg = e[0], view = e[1], ax = e[2], tang = e[3];
+    const int kk = p->ntof > 1 ? e[4] + p->ntof / 2 : -1;
+    if (nsub > 1) {
+      int bs = seg, bv = view, ba = ax, bt = tang;
+      (void)sym_op_for_bin(p, &bs, &bv, &ba, &bt);
+      if (bv % nsub != subset) continue;
+    }
+    const float add_term = add ? add[tbin(p, seg, view, ax, tang, p->ntof - 1)] : 0.F;
+    const float fwd = fp_bin(p, x, seg, view, ax, tang, kk) + add_term;
+    const float ev_value = 1.F;
+    if (ev_value > max_quotient * fwd) continue;
+    const float fwd_sq = fwd * fwd;
+    const float minus_y_over_fwd2 = -ev_value / fwd_sq;
+    const float r = fp_bin(p, rhs, seg, view, ax, tang, kk);
+    if (r == 0) continue;
+    bp_bin(p, out, minus_y_over_fwd2 * r, seg, view, ax, tang, kk);
+  }
+}
+
 /* OSMAPOSL on list-mode data, prior-free: subset sensitivities BP(m or 1)
  * without end-plane zeroing (add_subset_sensitivity, .cxx:661-720), update
  * x *= divide(sum over the subset's events of row^T/fwd, sens_s, 0),

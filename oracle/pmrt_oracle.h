@@ -220,6 +220,9 @@ double or_lm_gradient(const or_plan* p, const int32_t* ev, int64_t n, const floa
                       int num_subsets, float* grad);
 void or_lm_osem(const or_plan* p, const int32_t* ev, int64_t n, const float* add, float* x, int num_subsets,
                 int num_subiterations, int start_subset);
+/* list-mode Hessian product of the subset's events (LM_Hessian), accumulated into out */
+void or_lm_hessian(const or_plan* p, const int32_t* ev, int64_t n, const float* add, const float* x,
+                   const float* rhs, int subset, int nsub, float* out);
 
 #ifdef __cplusplus
 }

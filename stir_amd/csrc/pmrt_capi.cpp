@@ -2055,6 +2055,8 @@ int mi_lm_set_events(mi_ctx* ctx, const int32_t* events, int64_t num_events, con
     mi::launch_viewgram_max(ctx->y.p, ctx->vg_off.p, ctx->vg_len.p, nvg, ctx->vg_small.p, st);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
+    ctx->y_raw.release();
+    ctx->add_raw.release();
     ctx->lm = true;
     ctx->have_data = true;
     ctx->have_sens = false;
@@ -2143,7 +2145,10 @@ int mi_obj_hessian_times_input_dev(mi_ctx* ctx, int approximate, const float* d_
     if (!ctx || !d_input || !d_output || (!d_current && !approximate)) throw ApiError(MI_ERR_ARGUMENT, "NULL argument");
     ctx->require_setup();
     ctx->check_subset(subset);
-    if (ctx->lm) throw ApiError(MI_ERR_STATE, "the Hessian products work on projection data, not list mode");
+    if (ctx->lm && approximate)
+      // the list-mode objective does not overload the approximate product
+      // (GeneralisedObjectiveFunction.cxx:341-348)
+      throw ApiError(MI_ERR_STATE, "the list-mode objective has no approximate Hessian product");
     if (!ctx->have_data) throw ApiError(MI_ERR_STATE, "mi_obj_set_data has not been called");
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->pick(stream);
@@ -2171,6 +2176,17 @@ int mi_obj_hessian_times_input_dev(mi_ctx* ctx, int approximate, const float* d_
     if (!approximate) {
       ctx->image_to_device(d_current, ctx->img.p, st);
       ctx->fp(mi::FP_PLAIN, ctx->img.p, ctx->sino.p, subset, mi::FpData{}, st);
+    }
+    if (ctx->lm) {
+      // LM_Hessian over the histogrammed events (...ListModeDataWithProjMatrixByBin.cxx:790-822,963-1003):
+      // output += A^T q, q <= 0, no negativity check
+      mi::launch_lm_hessian(ctx->subset_vgs.p + v0, v1 - v0, ctx->vg_off.p, ctx->vg_len.p, ctx->y.p, ctx->ratio.p,
+                            ctx->sino.p, ctx->add.p, ctx->add_const, ctx->ratio.p, st);
+      ctx->bp(mi::BP_PLAIN, ctx->ratio.p, ctx->img2.p, subset, mi::BpData{}, st);
+      ctx->image_from_device(ctx->img2.p, ctx->stage_img.p, st);
+      mi::launch_accumulate(d_output, ctx->stage_img.p, ctx->stage_img.n, st);
+      HIPCHECK(hipGetLastError());
+      return;
     }
     HIPCHECK(hipMemsetAsync(ctx->hess_flag.p, 0, sizeof(int32_t), st));
     const float* y = ctx->y_raw.p ? ctx->y_raw.p : ctx->y.p;

@@ -348,6 +348,12 @@ void launch_hessian_ratio(const int32_t* d_vgs, int nvg, const int64_t* d_vg_off
                           const float* d_y, const float* d_fin, const float* d_fcur, const float* d_add,
                           float add_const, const float* d_mult, int approximate, float* d_q, int* d_negative,
                           hipStream_t stream);
+// the list-mode objective's Hessian product over histogrammed events y:
+// q = y * (-1 / fwd^2 * fin) with fwd = fcur + b, 0 where y = 0,
+// 1 > 1e4 fwd or fin = 0 (d_q may be d_fin)
+void launch_lm_hessian(const int32_t* d_vgs, int nvg, const int64_t* d_vg_off, const int32_t* d_vg_len,
+                       const float* d_y, const float* d_fin, const float* d_fcur, const float* d_add, float add_const,
+                       float* d_q, hipStream_t stream);
 
 // ---- objective-function helpers (pmrt_objective.hip)
 // QuadraticPrior weights w[dz+1][dy+1][dx+1].

@@ -617,6 +617,34 @@ __global__ void __launch_bounds__(256) hessian_ratio_kernel(const int32_t* __res
   }
 }
 
+// The list-mode objective's Hessian product (LM_Hessian,
+// ...ListModeDataWithProjMatrixByBin.cxx:790-822) over the events
+// histogrammed into y, one workgroup per viewgram vgs[i]: every event of a
+// bin (bin value 1) has fwd = fcur + b and adds row^T (-1 / (fwd * fwd) * fin)
+// unless 1 > 1e4 fwd or fin == 0, so q = y * (-1 / (fwd * fwd) * fin) or 0.
+__global__ void __launch_bounds__(256) lm_hessian_kernel(const int32_t* __restrict__ vgs,
+                                                         const int64_t* __restrict__ off,
+                                                         const int32_t* __restrict__ len, const float* y,
+                                                         const float* fin, const float* fcur, const float* add,
+                                                         float add_const, float* q) {
+  const int vg = vgs[blockIdx.x];
+  const int64_t o = off[vg];
+  const int n = len[vg];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int64_t j = o + i;
+    const float yv = y[j];
+    const float fwd = fcur[j] + (add ? add[j] : add_const);
+    const float r = fin[j];
+    float v = 0.F;
+    if (yv != 0.F && !(1.F > 10000.F * fwd) && r != 0.F) {
+      const float fwd_sq = fwd * fwd;
+      const float per_event = -1.F / fwd_sq;
+      v = yv * (per_event * r);
+    }
+    q[j] = v;
+  }
+}
+
 // zero the first and last axial position of `count` runs of length `run`
 __global__ void zero_end_planes_kernel(float* proj, const int64_t* off, int count, int run) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -830,6 +858,14 @@ void launch_hessian_ratio(const int32_t* d_vgs, int nvg, const int64_t* d_vg_off
   if (nvg <= 0) return;
   hipLaunchKernelGGL(hessian_ratio_kernel, dim3(nvg), dim3(256), 0, stream, d_vgs, d_vg_off, d_vg_len, d_y, d_fin,
                      d_fcur, d_add, add_const, d_mult, approximate, d_q, d_negative);
+}
+
+void launch_lm_hessian(const int32_t* d_vgs, int nvg, const int64_t* d_vg_off, const int32_t* d_vg_len,
+                       const float* d_y, const float* d_fin, const float* d_fcur, const float* d_add, float add_const,
+                       float* d_q, hipStream_t stream) {
+  if (nvg <= 0) return;
+  hipLaunchKernelGGL(lm_hessian_kernel, dim3(nvg), dim3(256), 0, stream, d_vgs, d_vg_off, d_vg_len, d_y, d_fin, d_fcur,
+                     d_add, add_const, d_q);
 }
 
 void launch_zero_end_planes(float* d_proj, const int64_t* d_off, int count, int run, int stride, hipStream_t stream) {

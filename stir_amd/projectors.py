@@ -500,7 +500,10 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndListModeDataWithProjMatrixByB
     an optional additive projection-data term and multiplicative factors for
     the sensitivities. Gradients and OSMAPOSL updates sum row^T / (row . x + b)
     over the subset's events (every event: the reference caps only the value);
-    ``compute_objective_function`` returns the reference's value, -sum log(row . x + b)."""
+    ``compute_objective_function`` returns the reference's value, -sum log(row . x + b).
+    ``accumulate_sub_Hessian_times_input_without_penalty`` (and the prior and
+    full-objective versions) give LM_Hessian's sum over the subset's events;
+    the approximate products raise, as the reference has none for list mode."""
 
     def set_up(self, events: np.ndarray, additive=0.0, multiplicative: Optional[np.ndarray] = None,
                use_subset_sensitivities: bool = True, compute_sensitivities: bool = True,

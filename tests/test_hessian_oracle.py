@@ -18,10 +18,18 @@ Pins, none of them a restatement of the formula under test:
   ignoring m give 1/4, 2 or 1);
 * a negative value in the forward projection of the input aborts, as in the
   reference.
+The list-mode objective's exact product (LM_Hessian event by event,
+...ListModeDataWithProjMatrixByBin.cxx:790-822,963-1003) is pinned the same
+way: against central differences of the list-mode gradient, against the
+projection-data product of the histogrammed events (no truncation fires on
+these data), and by a closed form at x = 0, where every event's estimate is
+the additive term a: the product is -BP(y FP(v)) / a^2 when 1 <= 1e4 a, and
+nothing when 1 > 1e4 a (the reference skips such events).
 Each pin is then run against oracle builds with one plausible mistake planted
 in the C source (ybar not squared, the background left out of ybar, one
-division by the factors, the product added instead of subtracted): every
-planted mistake must fail its pin.
+division by the factors, the product added instead of subtracted; for list
+mode: fwd not squared, the skip of capped events left out, the sign flipped):
+every planted mistake must fail its pin.
 """
 import os
 import shutil
@@ -125,6 +133,52 @@ def test_negative_forward_projection_aborts(tiny):
     assert not out.any()
 
 
+@pytest.fixture(scope="module")
+def tiny_lm(tiny):
+    pdi, img, O, x, y, bkg = tiny
+    from stir_amd import phantoms
+    x_true, counts, b0 = phantoms.config0_data(pdi, img, O.forward_project)
+    ev = pdi.events_from_counts(counts, seed=1)
+    return pdi, img, O, (x_true + 0.2).astype(np.float32), counts, np.full(pdi.num_bins, b0, np.float32), ev
+
+
+def test_lm_product_is_the_derivative_of_the_lm_gradient(tiny_lm):
+    pdi, img, O, x, counts, b, ev = tiny_lm
+    v = np.random.default_rng(17).uniform(0.0, 1.0, img.shape).astype(np.float32)
+    for s in (0, 2):
+        h = O.lm_hessian_times_input(ev, x, v, s, NSUB, additive=b)
+        eps = 1e-2
+        gp = O.lm_gradient(ev, (x + eps * v).astype(np.float32), s, NSUB, additive=b)[1].astype(np.float64)
+        gm = O.lm_gradient(ev, (x - eps * v).astype(np.float32), s, NSUB, additive=b)[1].astype(np.float64)
+        fd = (gp - gm) / (2 * eps)
+        err = np.abs(h - fd).max() / np.abs(fd).max()
+        print(f"list mode subset {s}: |H v - central difference| / max = {err:.2e}")
+        assert err < 5e-4
+        assert _dot(v, h) < 0
+
+
+def test_lm_product_equals_the_histogram_product(tiny_lm):
+    pdi, img, O, x, counts, b, ev = tiny_lm
+    v = np.random.default_rng(18).uniform(0.0, 1.0, img.shape).astype(np.float32)
+    for s in (1, 3):
+        h = O.lm_hessian_times_input(ev, x, v, s, NSUB, additive=b)
+        hp = O.hessian_times_input(counts, x, v, s, NSUB, bkg=b)
+        assert np.abs(h - hp).max() <= 1e-5 * np.abs(hp).max(), s
+
+
+def test_lm_product_closed_form_at_zero(tiny_lm):
+    pdi, img, O, x, counts, b, ev = tiny_lm
+    v = np.random.default_rng(19).uniform(0.0, 1.0, img.shape).astype(np.float32)
+    zero = np.zeros(img.shape, np.float32)
+    a = np.float32(1e-3)  # 1 <= 1e4 a: every event counts, with estimate a
+    h = O.lm_hessian_times_input(ev, zero, v, 0, 1, additive=np.full(pdi.num_bins, a, np.float32))
+    ref = -O.back_project(counts * O.forward_project(v)).astype(np.float64) / (float(a) * float(a))
+    assert np.abs(h - ref).max() <= 1e-5 * np.abs(ref).max()
+    # 1 > 1e4 a: every event is skipped
+    h = O.lm_hessian_times_input(ev, zero, v, 0, 1, additive=np.full(pdi.num_bins, 1e-5, np.float32))
+    assert not h.any()
+
+
 # ---------------------------------------------------------------------------
 # planted mistakes: (pin that must catch it, source text, replacement)
 ORACLE_SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
@@ -138,6 +192,13 @@ MUTANTS = {
     "added_not_subtracted": (test_symmetric_and_negative_definite,
                              "    for (int64_t j = 0; j < nv; ++j) out[j] = out[j] - bp[j];",
                              "    for (int64_t j = 0; j < nv; ++j) out[j] = out[j] + bp[j];"),
+    "lm_fwd_not_squared": (test_lm_product_closed_form_at_zero,
+                           "    const float fwd_sq = fwd * fwd;", "    const float fwd_sq = fwd;"),
+    "lm_capped_not_skipped": (test_lm_product_closed_form_at_zero,
+                              "    if (ev_value > max_quotient * fwd) continue;", "    (void)max_quotient;"),
+    "lm_sign": (test_lm_product_is_the_derivative_of_the_lm_gradient,
+                "    const float minus_y_over_fwd2 = -ev_value / fwd_sq;",
+                "    const float minus_y_over_fwd2 = ev_value / fwd_sq;"),
 }
 
 
@@ -155,5 +216,12 @@ def test_planted_mistake_fails_its_pin(tiny, tmp_path, name):
                     "-o", str(so), str(tmp_path / "pmrt_oracle.c"), "-lm"], check=True)
     pdi, img, _, x, y, bkg = tiny
     O = OraclePlan(pdi, img, RayTracingMatrixParameters(), library=oracle_mod.load_library(str(so)))
+    if name.startswith("lm_"):
+        from stir_amd import phantoms
+        x_true, counts, b0 = phantoms.config0_data(pdi, img, O.forward_project)
+        ev = pdi.events_from_counts(counts, seed=1)
+        args = (pdi, img, O, (x_true + 0.2).astype(np.float32), counts, np.full(pdi.num_bins, b0, np.float32), ev)
+    else:
+        args = (pdi, img, O, x, y, bkg)
     with pytest.raises(AssertionError):
-        pin((pdi, img, O, x, y, bkg))
+        pin(args)
