@@ -85,6 +85,12 @@ typedef struct {
   float tof_r_sqrt2_gauss_sigma;
   float tof_low[MI_MAX_TOF_BINS];
   float tof_high[MI_MAX_TOF_BINS];
+  /* "use actual detector boundaries" (ProjMatrixByBinUsingRayTracing.cxx:82,
+   * 300-387,608-634,760-762): phi and s of each bin's LOR from its two
+   * detectors, and twice the tangential spacing of multiple LORs. As in the
+   * reference it applies only to non-arc-corrected data without view mashing
+   * whose segments each hold one ring difference (else it is ignored). */
+  int32_t use_actual_detector_boundaries;
 } mi_geometry;
 
 /* Sizes of a set-up context. */

@@ -136,6 +136,7 @@ struct or_plan {
   int max_z, max_y, max_x;
   /* symmetries */
   int sym90, sym180, symseg, syms, symz;
+  int actual_bounds; /* use_actual_detector_boundaries after the set-up's checks */
   int ppa[OR_MAX_SEGMENTS];
   int ppr;
   float ap2z[OR_MAX_SEGMENTS];
@@ -330,14 +331,28 @@ static void add_adjacent_z(row* lor, float zf, float right)
   }
 }
 
-/* calculate_proj_matrix_elems_for_one_bin -- ProjMatrixByBinUsingRayTracing.cxx:582-829
- * (use_actual_detector_boundaries = false). */
+/* phi and s of the LOR between the two detectors of (view, tang) --
+ * ProjMatrixByBinUsingRayTracing.cxx:613-634, with the detector pair of
+ * ProjDataInfoCylindricalNoArcCorr::initialise_uncompressed_view_tangpos_to_det1det2
+ * (ProjDataInfoCylindricalNoArcCorr.cxx:164-200; >> of a negative int rounds
+ * towards minus infinity); both evaluated in double and rounded to float. */
+static void actual_boundary_lor(const or_plan* p, int view, int tang, float* s, float* phi)
+{
+  const int ndet = p->g.num_detectors_per_ring;
+  const int det1 = (view + (tang >> 1) + ndet) % ndet;
+  const int det2 = (view - ((tang + 1) >> 1) + ndet / 2) % ndet;
+  *phi = (float)((det1 + det2) * OR_PI / ndet - OR_PI / 2 + p->phi_offset);
+  *s = (float)(p->g.effective_ring_radius * sin((det1 - det2) * OR_PI / ndet + OR_PI / 2));
+}
+
+/* calculate_proj_matrix_elems_for_one_bin -- ProjMatrixByBinUsingRayTracing.cxx:582-829 */
 static int calc_row(const or_plan* p, int seg, int view, int ax, int tang, row* lor)
 {
   const or_geometry* g = &p->g;
   lor->n = 0;
-  const float s_in_mm = get_s(p, tang);
-  const float phi = get_phi(p, view);
+  float s_in_mm = get_s(p, tang);
+  float phi = get_phi(p, view);
+  if (p->actual_bounds) actual_boundary_lor(p, view, tang, &s_in_mm, &phi);
   const float cphi = cosf(phi);
   const float sphi = sinf(phi);
   const float tantheta = get_tantheta(p, seg, tang);
@@ -359,7 +374,8 @@ static int calc_row(const or_plan* p, int seg, int view, int ax, int tang, row* 
   } else {
     row tmp = {0};
     const int n = g->num_tangential_LORs;
-    const float s_inc = 1 * get_sampling_in_s(p, tang) / n;
+    /* twice the spacing with actual detector boundaries (:760-762) */
+    const float s_inc = (p->actual_bounds ? 2 : 1) * get_sampling_in_s(p, tang) / n;
     float cur = s_in_mm - s_inc * (n - 1) / 2.F;
     for (int k = 1; k <= n; ++k, cur += s_inc) {
       tmp.n = 0;
@@ -714,6 +730,12 @@ or_plan* or_plan_create(const or_geometry* g, char* err, int errlen)
   p->symseg = g->do_symmetry_swap_segment;
   p->syms = g->do_symmetry_swap_s;
   p->symz = g->do_symmetry_shift_z;
+  /* use_actual_detector_boundaries is reset when the data are arc-corrected,
+   * view-mashed or axially compressed (ProjMatrixByBinUsingRayTracing.cxx:300-327) */
+  p->actual_bounds = g->use_actual_detector_boundaries && !g->arc_corrected &&
+                     g->num_views * 2 == g->num_detectors_per_ring;
+  for (int si = 0; si < p->nseg; ++si)
+    if (g->min_ring_diff[si] != g->max_ring_diff[si]) p->actual_bounds = 0;
   p->ntof = g->num_tof_bins > 1 ? g->num_tof_bins : 1;
   if (p->ntof > OR_MAX_TOF) { snprintf(err, errlen, "too many TOF bins"); free(p); return NULL; }
   if (p->ntof > 1) {

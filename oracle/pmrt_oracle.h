@@ -22,10 +22,19 @@
  *     test_DataSymmetriesForBins_PET_CartesianGrid.cxx:112-122).
  *   - adjoint identity, finite-difference gradient, one-voxel EM closed
  *     form, OSEM likelihood monotonicity.
+ *   - use_actual_detector_boundaries: rows against nominal plans (even
+ *     tangential positions: the same scanner; odd ones: the scanner turned
+ *     by -pi/N; wrapped detector pairs: the z-mirrored LOR), the reset
+ *     rules, a planted detector-pair mistake (tests/test_actual_boundaries_oracle.py).
+ *   - Hessian products (projection data and list mode): central differences
+ *     of the gradients, symmetry and sign, closed forms, planted mistakes
+ *     (tests/test_hessian_oracle.py).
  * Parity unpinned: the choice between float and double libm overloads in
  * get_s/get_tantheta/get_costheta (depends on header inclusion order in
- * the reference build, which cannot be compiled here) and the OpenMP
- * reduction order of the reference back projector.
+ * the reference build, which cannot be compiled here), the OpenMP
+ * reduction order of the reference back projector, and with
+ * use_actual_detector_boundaries the doubled spacing of several tangential
+ * LORs (checked only against the GPU's restatement of the same rule).
  */
 #ifndef PMRT_ORACLE_H
 #define PMRT_ORACLE_H
@@ -82,6 +91,7 @@ typedef struct {
   float tof_r_sqrt2_gauss_sigma;
   float tof_low[OR_MAX_TOF];
   float tof_high[OR_MAX_TOF];
+  int32_t use_actual_detector_boundaries; /* ProjMatrixByBinUsingRayTracing.cxx:608-634 */
 } or_geometry;
 
 typedef struct or_plan or_plan;

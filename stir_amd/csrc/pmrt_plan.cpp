@@ -120,6 +120,11 @@ Plan::Plan(const mi_geometry& g, int num_subsets) : g_(g), nsub_(num_subsets) {
   symseg_ = g.do_symmetry_swap_segment != 0;
   syms_ = g.do_symmetry_swap_s != 0;
   symz_ = g.do_symmetry_shift_z != 0;
+  // use_actual_detector_boundaries is reset when the data are arc-corrected,
+  // view-mashed or axially compressed (ProjMatrixByBinUsingRayTracing.cxx:300-327)
+  actual_bounds_ = g.use_actual_detector_boundaries && !g.arc_corrected && g.num_views * 2 == g.num_detectors_per_ring;
+  for (int si = 0; si < nseg_; ++si)
+    if (g.min_ring_diff[si] != g.max_ring_diff[si]) actual_bounds_ = false;
   {
     const float zo = g.origin_z / g.vox_z;
     if (std::fabs(stir_round(zo) - zo) > 1.E-3F) fail("the z origin must be a multiple of the plane separation");
@@ -359,7 +364,18 @@ void Plan::build_basic_rays() {
           // calculate_proj_matrix_elems_for_one_bin (ProjMatrixByBinUsingRayTracing.cxx:582-732)
           BasicRay r{};
           r.s = get_s(tang);
-          const float phi = get_phi(view);
+          float phi = get_phi(view);
+          if (actual_bounds_) {
+            // the LOR between the bin's two detectors (:613-634), with the
+            // detector pair of ProjDataInfoCylindricalNoArcCorr.cxx:164-200
+            // (>> of a negative int rounds towards minus infinity), in double
+            // and rounded to float
+            const int ndet = g.num_detectors_per_ring;
+            const int det1 = (view + (tang >> 1) + ndet) % ndet;
+            const int det2 = (view - ((tang + 1) >> 1) + ndet / 2) % ndet;
+            phi = static_cast<float>((det1 + det2) * kPi / ndet - kPi / 2 + phi_offset_);
+            r.s = static_cast<float>(g.effective_ring_radius * std::sin((det1 - det2) * kPi / ndet + kPi / 2));
+          }
           r.cphi = std::cos(phi);  // float overloads
           r.sphi = std::sin(phi);
           r.tantheta = get_tantheta(seg, tang);
@@ -374,7 +390,7 @@ void Plan::build_basic_rays() {
           if (r.tantheta == 0) {
             if (std::fabs(stir_modulo((r.t + r.offset_in_z) / g.vox_z, 1.F) - .5) < .001) r.offset_in_z -= .1F * g.vox_z;
           }
-          r.s_inc = 1 * get_sampling_in_s(tang) / g.num_tangential_LORs;
+          r.s_inc = (actual_bounds_ ? 2 : 1) * get_sampling_in_s(tang) / g.num_tangential_LORs;  // :760-762
           basic_lookup_[((static_cast<size_t>(seg - g.min_segment_num) * g.num_views + view) * na + (symz_ ? 0 : ax)) *
                             (2 * tmax_ + 1) +
                         (tang + tmax_)] = static_cast<int>(rays_.size());

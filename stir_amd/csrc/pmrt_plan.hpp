@@ -183,6 +183,7 @@ class Plan {
   std::vector<int> ppa_;
   int ppr_;
   bool sym90_, sym180_, symseg_, syms_, symz_;
+  bool actual_bounds_ = false;  // use_actual_detector_boundaries after the set-up's checks
   int ntof_ = 1;
   float z_off_ = 0.F;
   std::vector<FamilyLor> fam_lor_;

@@ -407,8 +407,6 @@ class RayTracingMatrixParameters:
     def __post_init__(self):
         if self.num_tangential_LORs < 1:
             raise ValueError("num_tangential_LORs should be at least 1")
-        if self.use_actual_detector_boundaries:
-            raise NotImplementedError("use_actual_detector_boundaries is not supported")
 
 
 class GeometryStruct(ctypes.Structure):
@@ -454,6 +452,7 @@ class GeometryStruct(ctypes.Structure):
         ("tof_r_sqrt2_gauss_sigma", ctypes.c_float),
         ("tof_low", ctypes.c_float * MAX_TOF_BINS),
         ("tof_high", ctypes.c_float * MAX_TOF_BINS),
+        ("use_actual_detector_boundaries", ctypes.c_int32),
     ]
 
 
@@ -486,6 +485,7 @@ def make_geometry_struct(pdi: ProjDataInfoCylindrical, img: VoxelsOnCartesianGri
     g.origin_z, g.origin_y, g.origin_x = img.origin_z, img.origin_y, img.origin_x
     g.restrict_to_cylindrical_FOV = int(pm.restrict_to_cylindrical_FOV)
     g.num_tangential_LORs = pm.num_tangential_LORs
+    g.use_actual_detector_boundaries = int(pm.use_actual_detector_boundaries)
     g.do_symmetry_90degrees_min_phi = int(pm.do_symmetry_90degrees_min_phi)
     g.do_symmetry_180degrees_min_phi = int(pm.do_symmetry_180degrees_min_phi)
     g.do_symmetry_swap_segment = int(pm.do_symmetry_swap_segment)

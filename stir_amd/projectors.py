@@ -400,12 +400,53 @@ class PoissonLogLikelihoodWithLinearModelForMeanAndProjData:
                                                             out.reshape(-1)), "mi_obj_subset_gradient")
         return out
 
-    def compute_objective_function(self, current_estimate: np.ndarray, subset: int = -1) -> float:
+    def compute_objective_function_without_penalty(self, current_estimate: np.ndarray, subset: int = -1) -> float:
+        """The log-likelihood of one subset, or of all with subset -1."""
         self.set_estimate(current_estimate)
         v = ctypes.c_double()
         _native.check(_native.load().mi_obj_log_likelihood(self._ctx.handle, subset, ctypes.byref(v)),
                       "mi_obj_log_likelihood")
         return float(v.value)
+
+    # With a prior (set_prior) the penalised forms subtract it, as
+    # GeneralisedObjectiveFunction.cxx:113-252 does: the prior's value and
+    # gradient over num_subsets for one subset, whole for the full objective.
+    def _prior_active(self) -> bool:
+        prior = getattr(self, "prior", None)
+        return prior is not None and prior.penalisation_factor != 0.0
+
+    def compute_penalty(self, current_estimate: np.ndarray, subset: int = -1) -> float:
+        """``compute_penalty``: the prior's value (/ num_subsets for one subset); 0 without a prior."""
+        if not self._prior_active():
+            return 0.0
+        p = self.prior.compute_value(current_estimate, self.projector)
+        return p / self._num_subsets() if subset >= 0 else p
+
+    def compute_objective_function(self, current_estimate: np.ndarray, subset: int = -1) -> float:
+        """``compute_objective_function``: log-likelihood minus the penalty."""
+        return (self.compute_objective_function_without_penalty(current_estimate, subset)
+                - self.compute_penalty(current_estimate, subset))
+
+    def compute_sub_gradient(self, current_estimate: np.ndarray, subset: int) -> np.ndarray:
+        """``compute_sub_gradient``: the subset's gradient minus the prior's gradient / num_subsets."""
+        if subset < 0 or subset >= self._num_subsets():
+            raise ValueError("compute_sub_gradient subset_num out-of-range error")
+        g = self.compute_sub_gradient_without_penalty(current_estimate, subset)
+        if self._prior_active():
+            pg = self.prior.compute_gradient(current_estimate, self.projector).astype(np.float32)
+            g -= pg / np.float32(self._num_subsets())
+        return g
+
+    def compute_gradient_without_penalty(self, current_estimate: np.ndarray) -> np.ndarray:
+        """Every subset's gradient (all subsets in one pass)."""
+        return self.compute_sub_gradient_without_penalty(current_estimate, -1)
+
+    def compute_gradient(self, current_estimate: np.ndarray) -> np.ndarray:
+        """``compute_gradient``: every subset's gradient minus the prior's gradient."""
+        g = self.compute_gradient_without_penalty(current_estimate)
+        if self._prior_active():
+            g -= self.prior.compute_gradient(current_estimate, self.projector).astype(np.float32)
+        return g
 
     # ------------------------------------------------------------------
     # Hessian products (GeneralisedObjectiveFunction.cxx:253-470). Each
