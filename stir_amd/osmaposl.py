@@ -21,10 +21,12 @@ utility (``OSMAPOSL.cxx``, ``OSMAPOSLReconstruction.cxx``,
   subiteration number), ``start at subiteration number``, ``start at subset``,
   ``save estimates at subiteration intervals``, ``initial estimate``, ``enforce
   initial positivity condition``, ``MAP_model``, ``maximum relative change``,
-  ``minimum relative change``, ``output filename prefix``, ``disable output``.
+  ``minimum relative change``, ``output filename prefix``, ``disable output``,
+  ``uniformly randomise subset order`` (a new random permutation of the
+  subsets every cycle, IterativeReconstruction.cxx:575-638).
 
-Keys this path does not implement (filters, randomised subset order, other
-projectors, normalisations or priors) raise an error instead of being ignored.
+Keys this path does not implement (filters, other projectors, normalisations
+or priors) raise an error instead of being ignored.
 Sensitivities are always computed.
 """
 from __future__ import annotations
@@ -70,8 +72,6 @@ def _unsupported(par: Dict[str, Any]) -> None:
         v = _get(par, key)
         if v is not None and str(v).strip().lower() not in ("", "none"):
             raise ValueError(f"{key} := {v} is not supported")
-    if _truthy(_get(par, "uniformly randomise subset order", 0)):
-        raise ValueError("uniformly randomise subset order is not supported")
     if float(_get(par, "z zoom", 1.0)) != 1.0:
         raise ValueError("Z zoom other than 1 is not supported (image planes must be half the ring spacing)")
 
@@ -200,7 +200,9 @@ def set_up_from_par(par_path: str, device: int = 0):
                                    MAP_model=str(_get(par, "map_model", "additive")).strip(),
                                    minimum_relative_change=float(_get(par, "minimum relative change", 0.0)),
                                    maximum_relative_change=float(_get(par, "maximum relative change", FLT_MAX)),
-                                   start_subiteration_num=keys["start_subiteration"])
+                                   start_subiteration_num=keys["start_subiteration"],
+                                   uniformly_randomise_subset_order=_truthy(_get(par, "uniformly randomise subset order",
+                                                                                 0)))
     init_file = _get(par, "initial estimate")
     if init_file is not None and str(init_file).strip():
         iimg, initial = I.read_image(_path(base, init_file))
@@ -230,7 +232,7 @@ def run_par(par_path: str, output_dir: Optional[str] = None, device: int = 0) ->
     nsub = k["num_subsets"]
     last = k["num_subiterations"]
     for it in range(k["start_subiteration"], last + 1):
-        recon.update_estimate((it + k["start_subset"] - 1) % nsub, it)
+        recon.update_estimate(recon.get_subset_num(it), it)
         if (it % k["save_interval"] == 0 or it == last) and not k["disable_output"] and k["output_prefix"]:
             I.write_image(os.path.join(out, f"{k['output_prefix']}_{it}.hv"), img, obj.get_estimate())
     return obj.get_estimate(), s

@@ -636,14 +636,22 @@ class OSMAPOSLReconstruction:
     """OSMAPOSL (``OSMAPOSLReconstruction.cxx:266-516``) on the device: prior-free
     EM updates, or one-step-late MAP updates with the objective's prior.
     MAP_model "additive" (default) or "multiplicative"; minimum / maximum
-    relative change bound the multiplicative update after subiteration 1."""
+    relative change bound the multiplicative update after subiteration 1.
+    ``uniformly_randomise_subset_order``: every cycle of num_subsets
+    subiterations visits the subsets in a new random order
+    (``IterativeReconstruction.cxx:575-638``; the reference seeds rand() with
+    the time, here ``seed`` seeds numpy's generator, None for the time)."""
 
     def __init__(self, objective: PoissonLogLikelihoodWithLinearModelForMeanAndProjData,
                  num_subiterations: int, start_subset: int = 0, enforce_initial_positivity: bool = True,
                  MAP_model: str = "additive", minimum_relative_change: float = 0.0,
-                 maximum_relative_change: float = FLT_MAX, start_subiteration_num: int = 1):
+                 maximum_relative_change: float = FLT_MAX, start_subiteration_num: int = 1,
+                 uniformly_randomise_subset_order: bool = False, seed: Optional[int] = None):
         if MAP_model not in ("additive", "multiplicative"):
             raise ValueError(f"MAP model should have as value 'additive' or 'multiplicative', while it is {MAP_model!r}")
+        self.uniformly_randomise_subset_order = bool(uniformly_randomise_subset_order)
+        self._rng = np.random.default_rng(seed)
+        self._subset_array: List[int] = []
         self.objective = objective
         self.num_subiterations = num_subiterations
         self.start_subset = start_subset
@@ -666,9 +674,24 @@ class OSMAPOSLReconstruction:
             raise RuntimeError("OSMAPOSL cannot handle unbalanced subsets")
         self._set_options()
         self.objective.set_estimate(initial_image, enforce_positivity=self.enforce_initial_positivity)
-        _native.check(_native.load().mi_osem_run(self.objective._ctx.handle, self.start_subiteration_num,
-                                                 self.num_subiterations, self.start_subset), "mi_osem_run")
+        if not self.uniformly_randomise_subset_order:
+            _native.check(_native.load().mi_osem_run(self.objective._ctx.handle, self.start_subiteration_num,
+                                                     self.num_subiterations, self.start_subset), "mi_osem_run")
+            return self.objective.get_estimate()
+        for k in range(self.start_subiteration_num, self.start_subiteration_num + self.num_subiterations):
+            _native.check(_native.load().mi_osem_subset_step(self.objective._ctx.handle, self.get_subset_num(k), k),
+                          "mi_osem_subset_step")
         return self.objective.get_estimate()
+
+    def get_subset_num(self, subiteration_num: int) -> int:
+        """``get_subset_num`` (``IterativeReconstruction.cxx:629-638``): (k + start - 1) mod
+        num_subsets, or with random order a new permutation at the start of every cycle."""
+        n = max(1, int(self.objective.projector.num_subsets))
+        if not self.uniformly_randomise_subset_order:
+            return (subiteration_num + self.start_subset - 1) % n
+        if (subiteration_num - 1) % n == 0 or len(self._subset_array) != n:
+            self._subset_array = [int(v) for v in self._rng.permutation(n)]
+        return self._subset_array[(subiteration_num - 1) % n]
 
     def update_estimate(self, subset: int, subiteration_num: int) -> None:
         self._set_options()

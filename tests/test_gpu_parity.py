@@ -179,6 +179,31 @@ def test_osem_matches_oracle(c0):
     assert np.abs(got - ref).max() <= 1e-4 * ref.max()
 
 
+def test_osem_in_random_subset_order(c0):
+    """"uniformly randomise subset order": a new permutation every cycle of 4
+    subiterations; the oracle applies the same subsets one subiteration at a
+    time (default relative-change bounds: no thresholding), within 1e-4."""
+    pdi, img, pm, P, O = c0
+    x_true, y, bkg = phantoms.config0_data(pdi, img, O.forward_project)
+    obj = PoissonLogLikelihoodWithLinearModelForMeanAndProjData(P)
+    obj.set_up(y, bkg)
+    order = [OSMAPOSLReconstruction(obj, 8, uniformly_randomise_subset_order=True, seed=5)]
+    order = [order[0].get_subset_num(k) for k in range(1, 9)]
+    print("subset order", order)
+    assert sorted(order[:4]) == [0, 1, 2, 3] and sorted(order[4:]) == [0, 1, 2, 3]
+    assert order != [0, 1, 2, 3, 0, 1, 2, 3]
+    recon = OSMAPOSLReconstruction(obj, num_subiterations=8, uniformly_randomise_subset_order=True, seed=5)
+    got = recon.reconstruct(np.ones(img.shape, np.float32))
+    ref = np.ones(img.shape, np.float32)
+    b = np.full(pdi.num_bins, bkg, np.float32)
+    for s in order:
+        ref = O.osem(y, ref, 4, 1, bkg=b, start_subset=s)
+    mask = ref > 1e-3 * ref.max()
+    rel = np.abs(got - ref)[mask] / ref[mask]
+    print(f"random-order OSEM max rel err {rel.max():.3e}")
+    assert rel.max() <= 1e-4
+
+
 def test_gradient_and_loglik(c0):
     pdi, img, pm, P, O = c0
     x_true, y, bkg = phantoms.config0_data(pdi, img, O.forward_project)

@@ -119,3 +119,26 @@ def test_unsupported_geometry_is_rejected_on_host():
     g = make_geometry_struct(pdi, img, RayTracingMatrixParameters())
     with pytest.raises(_native.NativeError, match="origin"):
         _native.symmetry_operations(g, np.zeros((1, 4), np.int32))
+
+
+def test_random_subset_order_is_a_permutation_per_cycle():
+    """get_subset_num (IterativeReconstruction.cxx:629-638) without a device:
+    (k + start - 1) mod n in order, a fresh permutation every n subiterations
+    with "uniformly randomise subset order", reproducible with a seed."""
+    from stir_amd.projectors import OSMAPOSLReconstruction
+
+    class _Pair:
+        num_subsets = 5
+
+    class _Obj:
+        projector = _Pair()
+
+    plain = OSMAPOSLReconstruction(_Obj(), 10, start_subset=2)
+    assert [plain.get_subset_num(k) for k in range(1, 11)] == [(k + 1) % 5 for k in range(1, 11)]
+    r1 = OSMAPOSLReconstruction(_Obj(), 15, uniformly_randomise_subset_order=True, seed=7)
+    seq = [r1.get_subset_num(k) for k in range(1, 16)]
+    for c in range(3):
+        assert sorted(seq[5 * c:5 * c + 5]) == list(range(5))
+    assert len({tuple(seq[0:5]), tuple(seq[5:10]), tuple(seq[10:15])}) > 1
+    r2 = OSMAPOSLReconstruction(_Obj(), 15, uniformly_randomise_subset_order=True, seed=7)
+    assert [r2.get_subset_num(k) for k in range(1, 16)] == seq
