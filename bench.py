@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-map", action="store_true")
     ap.add_argument("--skip-listmode", action="store_true")
+    ap.add_argument("--skip-hessian", action="store_true")
     ap.add_argument("--tof-steps", type=int, default=3)
     ap.add_argument("--autograd-steps", type=int, default=3)
     ap.add_argument("--cpu-subsets", type=int, default=14,
@@ -237,7 +238,7 @@ def main():
     }
 
     obj = None
-    if not (args.skip_osem and args.skip_autograd and args.skip_map):
+    if not (args.skip_osem and args.skip_autograd and args.skip_map and args.skip_hessian):
         fwd = pair.forward_project(x).cpu().numpy()
         y, scale = phantoms.poisson_data(fwd, 2.0, 0.05, seed=101)
         del fwd
@@ -273,6 +274,9 @@ def main():
     if not args.skip_autograd:
         result["autograd"] = autograd_bench(pair, obj, img, pdi, dev, args.autograd_steps, ms_per_step)
 
+    if not args.skip_hessian:
+        result["hessian"] = hessian_bench(obj, img, dev, ms_per_step)
+
     # the list-mode objective takes over the pair's context (its data become
     # the events' histogram): it runs after everything that uses the
     # projection-data objective
@@ -288,6 +292,33 @@ def main():
     if not args.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(pdi, img, pm, x_true, nsub, args.cpu_subsets)
     print(json.dumps(result), flush=True)
+
+
+def hessian_bench(obj, img, dev, pair_ms, steps=3):
+    """The objective's Hessian-vector products over all subsets on device
+    images (accumulate_Hessian_times_input_without_penalty: two forward
+    projections, the per-viewgram quotient, one back projection; the
+    approximate product: one forward projection fewer), after one warm-up."""
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(7)
+    x = torch.from_numpy(rng.uniform(0.5, 1.5, img.shape).astype(np.float32)).to(dev)
+    v = torch.from_numpy(rng.uniform(0.0, 1.0, img.shape).astype(np.float32)).to(dev)
+    out = torch.zeros(img.shape, dtype=torch.float32, device=dev)
+    res = {}
+    for name, call in (("exact", lambda: obj.accumulate_Hessian_times_input_without_penalty(out, x, v)),
+                       ("approximate", lambda: obj.add_multiplication_with_approximate_Hessian_without_penalty(out, v))):
+        call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            call()
+        torch.cuda.synchronize()
+        res[f"{name}_ms"] = round((time.perf_counter() - t0) * 1000.0 / steps, 3)
+    res["exact_vs_pair"] = round(res["exact_ms"] / pair_ms, 3)
+    log(f"Hessian products: exact {res['exact_ms']:.2f} ms, approximate {res['approximate_ms']:.2f} ms")
+    return res
 
 
 def autograd_bench(pair, obj, img, pdi, dev, steps, pair_ms):
