@@ -371,9 +371,14 @@ class VoxelsOnCartesianGrid:
 
     @staticmethod
     def from_proj_data_info(pdi: ProjDataInfoCylindrical, zoom: float = 1.0, xy_size: int = -1,
-                            z_size: int = -1) -> "VoxelsOnCartesianGrid":
-        """``VoxelsOnCartesianGrid(proj_data_info, zoom, origin, sizes)``
-        (``VoxelsOnCartesianGrid.cxx:55-134,230-288``)."""
+                            z_size: int = -1, zoom_z: float = 1.0, z_offset: float = 0.0) -> "VoxelsOnCartesianGrid":
+        """``VoxelsOnCartesianGrid(proj_data_info, zooms, origin, sizes)``
+        (``VoxelsOnCartesianGrid.cxx:55-134,230-288``): voxel size (z sampling,
+        s sampling, s sampling) / (zoom_z, zoom, zoom); the default z size does
+        not depend on zoom_z; origin (z_offset, 0, 0) ("Z offset (in mm)",
+        ``ParseDiscretisedDensityParameters.cxx:44-50``)."""
+        if zoom <= 0 or zoom_z <= 0:
+            raise ValueError("zoom should be positive")
         rs = np.float32(pdi.scanner.ring_spacing)
         z_sampling = rs / np.float32(2)
         seg0 = -pdi.min_segment_num
@@ -388,7 +393,9 @@ class VoxelsOnCartesianGrid:
             # cover the FOV: 2 ceil(R_fov / voxel) + 1 (VoxelsOnCartesianGrid.cxx:252-267)
             r_fov = max(abs(pdi.get_s(pdi.max_tangential_pos_num)), abs(pdi.get_s(pdi.min_tangential_pos_num)))
             xy_size = 2 * int(math.ceil(float(np.float32(r_fov) / vxy))) + 1
-        return VoxelsOnCartesianGrid(z_size, xy_size, xy_size, float(z_sampling), float(vxy), float(vxy))
+        vz = z_sampling / np.float32(zoom_z)
+        return VoxelsOnCartesianGrid(z_size, xy_size, xy_size, float(vz), float(vxy), float(vxy),
+                                     origin_z=float(np.float32(z_offset)))
 
 
 @dataclass(frozen=True)

@@ -463,7 +463,7 @@ def ray_tracing_matrix_parameters(par: Dict[str, Any]) -> RayTracingMatrixParame
 def reconstruction_setup(par_path: str, data_dir: Optional[str] = None):
     """Problem of an OSMAPOSL ``.par`` file: (ProjDataInfoCylindrical of its
     input file, restricted to "maximum absolute segment number to process";
-    VoxelsOnCartesianGrid from zoom and xy size; RayTracingMatrixParameters;
+    VoxelsOnCartesianGrid from the zooms, sizes and z offset; RayTracingMatrixParameters;
     dict of the reconstruction keys)."""
     par = parse_par(par_path)
     base = data_dir or os.path.dirname(os.path.abspath(par_path))
@@ -474,7 +474,10 @@ def reconstruction_setup(par_path: str, data_dir: Optional[str] = None):
         pdi = pdi.restrict_segments(max_seg)
     zoom = float(find_key(par, "zoom", 1.0))
     xy = int(find_key(par, "xy output image size (in pixels)", -1))
-    img = VoxelsOnCartesianGrid.from_proj_data_info(pdi, zoom=zoom, xy_size=xy)
+    img = VoxelsOnCartesianGrid.from_proj_data_info(pdi, zoom=zoom, xy_size=xy,
+                                                    z_size=int(find_key(par, "z output image size (in pixels)", -1)),
+                                                    zoom_z=float(find_key(par, "z zoom", 1.0)),
+                                                    z_offset=float(find_key(par, "z offset (in mm)", 0.0)))
     keys = {
         "num_subsets": int(find_key(par, "number of subsets", 1)),
         "num_subiterations": int(find_key(par, "number of subiterations", 1)),

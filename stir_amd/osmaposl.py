@@ -15,8 +15,8 @@ utility (``OSMAPOSL.cxx``, ``OSMAPOSLReconstruction.cxx``,
 * projector pair ``Matrix`` with ``Matrix type := Ray Tracing`` or ``Separate
   Projectors`` with the ``MI355X Ray Tracing`` forward and back projectors, and
   the ``Ray Tracing Matrix Parameters`` keys;
-* image: ``zoom``, ``XY output image size (in pixels)``, ``Z output image size
-  (in pixels)``;
+* image: ``zoom``, ``Z zoom``, ``XY output image size (in pixels)``, ``Z output
+  image size (in pixels)``, ``Z offset (in mm)``;
 * reconstruction: ``number of subsets``, ``number of subiterations`` (the last
   subiteration number), ``start at subiteration number``, ``start at subset``,
   ``save estimates at subiteration intervals``, ``initial estimate``, ``enforce
@@ -72,8 +72,6 @@ def _unsupported(par: Dict[str, Any]) -> None:
         v = _get(par, key)
         if v is not None and str(v).strip().lower() not in ("", "none"):
             raise ValueError(f"{key} := {v} is not supported")
-    if float(_get(par, "z zoom", 1.0)) != 1.0:
-        raise ValueError("Z zoom other than 1 is not supported (image planes must be half the ring spacing)")
 
 
 def _path(base: str, name) -> str:
@@ -158,7 +156,9 @@ def set_up_from_par(par_path: str, device: int = 0):
     pdi = pdi_full if max_seg is None else pdi_full.restrict_segments(max_seg)
     img = VoxelsOnCartesianGrid.from_proj_data_info(pdi, zoom=float(_get(par, "zoom", 1.0)),
                                                     xy_size=int(_get(par, "xy output image size (in pixels)", -1)),
-                                                    z_size=int(_get(par, "z output image size (in pixels)", -1)))
+                                                    z_size=int(_get(par, "z output image size (in pixels)", -1)),
+                                                    zoom_z=float(_get(par, "z zoom", 1.0)),
+                                                    z_offset=float(_get(par, "z offset (in mm)", 0.0)))
     nsub = int(_get(par, "number of subsets", 1))
     keys = {
         "num_subsets": nsub,

@@ -249,3 +249,21 @@ def test_default_image_size_covers_the_fov():
     from stir_amd.geometry import VoxelsOnCartesianGrid
     pdi = config1()[0]
     assert VoxelsOnCartesianGrid.from_proj_data_info(pdi).shape == (127, 279, 279)
+
+
+def test_image_grid_zooms_and_z_offset():
+    """VoxelsOnCartesianGrid(proj_data_info, zooms, origin, sizes)
+    (VoxelsOnCartesianGrid.cxx:230-288): voxel size = sampling / zoom per axis,
+    the default z size independent of the z zoom, origin z = "Z offset (in mm)"."""
+    from stir_amd.geometry import ProjDataInfoCylindrical, Scanner, VoxelsOnCartesianGrid
+
+    sc = Scanner.cylindrical(4, 64, 80.0, 4.0)
+    pdi = ProjDataInfoCylindrical.construct(sc, span=1, max_delta=3, num_views=32, num_tangential_poss=32)
+    base = VoxelsOnCartesianGrid.from_proj_data_info(pdi)
+    img = VoxelsOnCartesianGrid.from_proj_data_info(pdi, zoom=1.0, zoom_z=2.0, z_offset=3.0)
+    assert base.vox_z == 2.0 and img.vox_z == 1.0
+    assert img.nz == base.nz == 2 * 4 - 1
+    assert img.origin_z == 3.0 and base.origin_z == 0.0
+    assert (img.vox_y, img.nx) == (base.vox_y, base.nx)
+    with pytest.raises(ValueError):
+        VoxelsOnCartesianGrid.from_proj_data_info(pdi, zoom_z=0.0)
