@@ -142,3 +142,13 @@ def test_random_subset_order_is_a_permutation_per_cycle():
     assert len({tuple(seq[0:5]), tuple(seq[5:10]), tuple(seq[10:15])}) > 1
     r2 = OSMAPOSLReconstruction(_Obj(), 15, uniformly_randomise_subset_order=True, seed=7)
     assert [r2.get_subset_num(k) for k in range(1, 16)] == seq
+
+
+def test_hessian_entry_point_rejects_null_arguments():
+    """mi_obj_hessian_times_input_dev checks its arguments before any device
+    call: a NULL context or image is MI_ERR_ARGUMENT."""
+    L = _native.load()
+    buf = (ctypes.c_float * 4)()
+    rc = L.mi_obj_hessian_times_input_dev(None, 0, ctypes.cast(buf, ctypes.c_void_p), ctypes.cast(buf, ctypes.c_void_p),
+                                          0, ctypes.cast(buf, ctypes.c_void_p), None)
+    assert rc != 0 and "NULL" in L.mi_last_error().decode()
