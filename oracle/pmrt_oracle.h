@@ -4,8 +4,8 @@
  * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of STIR's
  * ProjMatrixByBinUsingRayTracing (PMRT) projector pair, its PET
  * Cartesian-grid symmetries, the Poisson log-likelihood sub-gradient and
- * the OSMAPOSL multiplicative update.  It is the checker that the HIP
- * path is compared against; it is never called by the product library.
+ * the OSMAPOSL multiplicative update.  The HIP path is compared against
+ * it in the tests; it is never called by the product library.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
  * load it.
  *
